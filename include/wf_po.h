@@ -1,0 +1,180 @@
+/*
+ * wf_po.h -- C-ABI of the MI355X-native physical-optics (PO) far-field solver.
+ *
+ * Drop-in boundary for WaveFiber3d's PO path.  The reference has no library
+ * API: its "entry point" is the driver main() (fibersim/treePO.cu:21-992,
+ * compareMie/sphere.cu:21-966, compareBEM3d/ellipsoid.cu:20-990) calling the
+ * header-level functions of util/*.h over caller-owned thrust vectors.  Each
+ * declaration below names the reference code it replaces.
+ *
+ * Conventions
+ *  - Plain C types only; every function returns wf_status (never exits; the
+ *    reference's checkCudaErrors() -> exit() at util/helper_cuda.h:582-595 is
+ *    replaced by status codes plus a thread-local message, wf_last_error()).
+ *  - Units are SI (metres), as in the reference (util/util.h:16-19).
+ *  - Directions use the reference's latitude convention (treePO.cu:378-391):
+ *    s(theta, phi) = (cos t cos p, cos t sin p, sin t); output grids are
+ *    theta_j = -pi/2 + 1e-10 + pi j/Nt, phi_k = 2 pi k/Np, flat index j*Np + k.
+ *  - Output intensity is float32 [n_inc][Nt][Np] (treePO.cu:953-957 layout).
+ *    The complex far field E'' = E_t - Z0 s x H_t (util/fields.h:189-200),
+ *    which the reference never stores, is returned as float32 pairs
+ *    [n_inc][3][Nt][Np][2]; its global phase uses the exact FP64
+ *    e^{-i k0 D}, D = 1 m (treePO.cu:508-509 evaluates it in FP32).
+ *  - memspace WF_MEM_DEVICE: inputs and outputs are device pointers and the
+ *    call is asynchronous on `stream` (a hipStream_t; NULL = default stream).
+ *    WF_MEM_HOST: inputs/outputs are host pointers and the call returns after
+ *    the results are in host memory.
+ *  - A handle owns all device workspaces and the cached octree of the current
+ *    geometry.  One handle per host thread; handles are not shared concurrently.
+ */
+#ifndef WF_PO_H
+#define WF_PO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_ABI_VERSION 1
+
+typedef enum wf_status {
+  WF_OK = 0,
+  WF_ERR_INVALID_ARG = 1,     /* bad pointer/size/value; message names the field */
+  WF_ERR_GRID_TOO_COARSE = 2, /* strict_grid and output grid coarser than level 1 (reference Q7) */
+  WF_ERR_OOM = 3,             /* device allocation failed or workspace cap exceeded */
+  WF_ERR_HIP = 4,             /* HIP runtime error (message carries hipGetErrorString) */
+  WF_ERR_CANCELLED = 5,       /* *cancel_flag became nonzero between stages */
+  WF_ERR_UNSUPPORTED = 6,     /* e.g. an FFT length with a prime factor above 7 */
+  WF_ERR_NO_GEOMETRY = 7      /* wf_solve before wf_set_geometry */
+} wf_status;
+
+enum { WF_MEM_HOST = 0, WF_MEM_DEVICE = 1 };
+
+/* Leaf-GEMM arithmetic.  All modes accumulate in FP32; phases are range
+ * reduced from FP64 arguments. */
+enum { WF_PREC_FP32 = 0, WF_PREC_BF16X3 = 1 };
+
+/* Surface samples, SoA float32, n each (compareMie/sphere.cu:92-125 input
+ * form; util/tree.h:67-97 Points and :38-62 Area). */
+typedef struct wf_surface {
+  const float *x, *y, *z;    /* positions (m) */
+  const float *nx, *ny, *nz; /* outward unit normals */
+  const float* area;         /* area per sample (m^2) */
+  int64_t n;
+  int32_t memspace; /* WF_MEM_HOST or WF_MEM_DEVICE */
+} wf_surface;
+
+/* One solve = one (wavelength, incidence direction, polarization).
+ * treePO.cu:501-509 (lambda, k0), :559-601 (theta_i, phi_i, eta, pol, basis). */
+typedef struct wf_incidence {
+  double theta, phi;    /* incidence latitude / azimuth (rad) */
+  double lambda_m;      /* vacuum wavelength (m) */
+  char pol;             /* 'M' = TM, 'E' = TE (treePO.cu:573-601) */
+  double eta_re, eta_im;/* complex IOR of the body, outside medium = 1 (treePO.cu:566-568) */
+  double beam_sigma_m;  /* Gaussian taper exp(-z^2/(2 sigma^2)) (tangentplane.h:33-34);
+                           <= 0 or non-finite: taper off */
+} wf_incidence;
+
+typedef struct wf_grid {
+  int32_t n_theta, n_phi; /* output far-field grid (treePO.cu:63-77) */
+} wf_grid;
+
+typedef struct wf_opts {
+  int32_t struct_size;   /* sizeof(wf_opts) (ABI guard); wf_default_opts sets it */
+  int32_t device;        /* HIP device ordinal; -1 = current device */
+  int32_t precision;     /* WF_PREC_* (leaf radiation GEMM) */
+  int32_t strict_grid;   /* 1: refuse output grids coarser than level 1, like the
+                            reference's upsample-only interpolation (interpolation.h:27-31) */
+  double beta;           /* excess-bandwidth constant of the level sampling rule
+                            (treePO.cu:92 uses 6 on its own extent definition) */
+  double box_enlarge;    /* automatic root box = bbox centre +- enlarge * half extent
+                            (treePO.cu:95-98 uses 1.1) */
+  double lambda_sample_m;/* wavelength that sets the level grids; 0 = the group's own
+                            (the reference fixes 0.4 um, treePO.cu:347) */
+  int32_t has_box;       /* 1: use box[] as the root box (treePO.cu:220-230) */
+  double box[6];         /* {min x, min y, min z, max x, max y, max z} (m) */
+  int64_t workspace_cap_bytes;   /* 0 = unlimited; exceeded -> WF_ERR_OOM */
+  volatile int32_t* cancel_flag; /* nullable; polled between stages */
+  int32_t field_chunk;   /* max complex fields (6 per solve) per tree pass; 0 = auto */
+} wf_opts;
+
+typedef struct wf_handle wf_handle;
+
+/* Timing/roofline breakdown of the last solve (hipEvent based, milliseconds). */
+typedef struct wf_stats {
+  double ms_currents, ms_leaf, ms_aggregate, ms_final, ms_total;
+  double ms_tree_build;
+  int64_t leaf_pairs;       /* sum over leaves of D_leaf * n_leaf (per lambda group, summed) */
+  int64_t agg_bytes;        /* compulsory aggregation bytes (child read + parent write) */
+  int32_t depth, n_levels;
+  int32_t level_nt[16], level_np[16]; /* level grids of the last lambda group */
+  int32_t level_nodes[16];            /* active nodes per level */
+} wf_stats;
+
+int32_t wf_abi_version(void);
+const char* wf_last_error(void);
+void wf_default_opts(wf_opts* opts);
+
+wf_status wf_create(wf_handle** out, const wf_opts* opts);
+void wf_destroy(wf_handle* h);
+
+/* Geometry upload + octree build (replaces treePO.cu:137-337: H2D copies,
+ * build_octree_kernel<256> at util/tree.h:289-715, the active lists and
+ * per-point centres).  The points are validated (finite, unit-ish normals,
+ * non-negative area), Morton-keyed with the reference's float split planes,
+ * stably sorted and cut into per-level active-node lists.  depth in [0, 10]. */
+wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void* stream);
+
+/* PO far-field solve on the current geometry (the treePO.cu:501-961 loop body:
+ * currents tangentplane.h:84-92, field_times_area fields.h:123,
+ * multiply_current_factor fields.h:147, leaf sums fields.h:41-50 +
+ * treePO.cu:652-683, copyandinterp interpolation.h:82-92, translation_compute
+ * fields.h:92-103, sumchildren fields.h:153-170, computeintensity
+ * fields.h:172-204).  intensity [n_inc][Nt][Np] float32; E_cplx nullable,
+ * [n_inc][3][Nt][Np][2] float32.  Output memspace as given. */
+wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                   float* intensity, float* E_cplx, int32_t out_memspace, void* stream);
+
+/* One-shot: wf_set_geometry + wf_solve (outputs in the surface's memspace). */
+wf_status wf_po_solve(wf_handle* h, const wf_surface* s, const wf_incidence* inc, int32_t n_inc,
+                      wf_grid grid, int32_t depth, float* intensity, float* E_cplx, void* stream);
+
+/* Plain method in FP64 (depth-0 semantics of sphere.cu:592-658 and
+ * :909-923; FP64 precedent compareBEM3d/ellipsoid_simple.cu:229-261): the
+ * oracle-on-GPU and the speedup denominator.  Centre = root box centre.
+ * intensity [n_inc][Nt][Np] double; E_cplx nullable [n_inc][3][Nt][Np][2] double. */
+wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
+                                  int32_t n_inc, wf_grid grid, double* intensity, double* E_cplx,
+                                  void* stream);
+
+/* Stage entry points (parity tests per stage; all device pointers, async). */
+
+/* computeLocalTangentPlane + field_times_area + multiply_current_factor
+ * (tangentplane.h:84-92, fields.h:123-126, :147-150): currents of the surface
+ * (in the given order) for n_inc incidences, complex [n_inc][n][6] as float
+ * pairs (fp64 = 0) or double pairs (fp64 = 1). */
+wf_status wf_currents(wf_handle* h, const wf_surface* s, const wf_incidence* inc, int32_t n_inc,
+                      int32_t fp64, void* out, void* stream);
+
+/* build_octree_kernel (tree.h:289-715) output in the reference's form: perm[n]
+ * (original index at sorted position), node_begin/node_end for all
+ * (8^(depth+1)-1)/7 nodes in level order.  Uses the current geometry. */
+wf_status wf_octree_nodes(wf_handle* h, int32_t* perm, int32_t* node_begin, int32_t* node_end,
+                          void* stream);
+
+/* copyandinterp/interpFFT (interpolation.h:11-92) for the tree's patterns:
+ * `count` child patterns [count][ntc][npc] complex64 on the half-sample theta
+ * grid theta_j = -pi/2 + pi (j + 1/2)/ntc are evaluated on the parent grid
+ * [ntp][npp] with theta origin off_tp (phi origin 0), using the 2 pi
+ * great-circle extension in theta (npp even).  Device pointers. */
+wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const float* child,
+                    int32_t ntp, int32_t npp, double off_tp, float* parent, void* stream);
+
+/* Statistics of the last wf_solve on this handle. */
+wf_status wf_get_stats(wf_handle* h, wf_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WF_PO_H */

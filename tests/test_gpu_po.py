@@ -1,0 +1,152 @@
+"""GPU parity tests: each HIP stage through the C-ABI vs the FP64 oracle.
+
+Bars: FP64 paths <= 1e-12 relative; FP32 stages <= 1e-5; tree solves <= 1e-3
+relative L2 of intensity against the FP64 depth-0 direct sum (the gate of the
+tree path), plus exact integer parity for the octree.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import wavefiber3d_amd as wf
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(s):
+    return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in s.arrays()]
+
+
+def _rand_surface(n, seed, scale=2e-6):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-scale, scale, (3, n))
+    q = rng.standard_normal((3, n))
+    q /= np.linalg.norm(q, axis=0)
+    a = rng.uniform(1e-15, 5e-15, n)
+    return wf.SurfaceSamples(*[v.astype(np.float32) for v in (p[0], p[1], p[2], q[0], q[1], q[2], a)])
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+INCS = [
+    wf.Incidence(np.pi / 2, 0.0, 0.5e-6, "M", 1.55 + 0.0j),
+    wf.Incidence(np.pi / 2, 0.0, 0.5e-6, "E", 1.55 + 0.0j),
+    wf.Incidence(-0.4, 2.1, 0.61e-6, "M", 1.55 + 0.1j, 3e-6),
+    wf.Incidence(0.7, -1.3, 0.43e-6, "E", 1.3 + 0.02j),
+]
+
+
+def test_currents_fp64_and_fp32(solver):
+    s = _rand_surface(3000, 1)
+    print("case: currents n=3000", flush=True)
+    C64 = solver.currents(_dev(s), INCS, fp64=True).cpu().numpy()
+    C32 = solver.currents(_dev(s), INCS, fp64=False).cpu().numpy()
+    for k, inc in enumerate(INCS):
+        ref = oracle.currents(s.arrays(), inc.as_tuple())
+        assert rel(C64[k], ref) < 1e-12
+        assert rel(C32[k], ref) < 1e-6
+
+
+def test_direct_fp64_matches_oracle(solver):
+    s = wf.fibonacci_sphere(3000, 2e-6)
+    nt, np_ = 24, 48
+    print("case: fp64 direct sphere n=3000 24x48", flush=True)
+    I, E = solver.solve_fp64_direct(s, INCS, (nt, np_), return_E=True)
+    Iref, Eref = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS], nt, np_)
+    for k in range(len(INCS)):
+        assert rel(I[k], Iref[k]) < 1e-12
+        assert rel(E[k].reshape(3, -1), Eref[k]) < 1e-12
+
+
+def test_direct_fp64_device_inputs(solver):
+    s = wf.fibonacci_sphere(1000, 1e-6)
+    print("case: fp64 direct device tensors", flush=True)
+    I = solver.solve_fp64_direct(_dev(s), INCS[:2], (12, 24))
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS[:2]], 12, 24)
+    assert rel(I.cpu().numpy().reshape(2, -1), Iref) < 1e-12
+
+
+@pytest.mark.parametrize("depth", [0, 1, 3, 4])
+def test_octree_matches_reference_partition(solver, depth):
+    s = _rand_surface(20000, 7)
+    # points on split planes and outside the box exercise the float predicates
+    s.x[:50] = 0.0
+    s.y[50:100] = 0.0
+    box = [-2e-6, -2e-6, -1.5e-6, 2e-6, 2e-6, 1.5e-6]
+    print(f"case: octree depth={depth}", flush=True)
+    with wf.POSolver(device=0, box=box) as sv:
+        sv.set_geometry(s, depth)
+        perm, nb, ne = [t.cpu().numpy() for t in sv.octree_nodes()]
+    rperm, rb, re_, _ = oracle.octree([s.x, s.y, s.z], box, depth)
+    np.testing.assert_array_equal(perm, rperm)
+    np.testing.assert_array_equal(nb, rb)
+    np.testing.assert_array_equal(ne, re_)
+
+
+@pytest.mark.parametrize("ntc,npc,ntp,npp,off", [
+    (9, 16, 15, 28, None), (12, 20, 20, 36, None), (10, 18, 7, 12, "out"), (14, 26, 30, 60, "out")])
+def test_interp_matches_bruteforce(solver, ntc, npc, ntp, npp, off):
+    rng = np.random.default_rng(ntc * 100 + npp)
+    # band-limited pattern: a few point sources radiating on the child grid
+    off_c = -np.pi / 2 + np.pi / (2 * ntc)
+    t = off_c + np.pi * np.arange(ntc) / ntc
+    p = 2 * np.pi * np.arange(npc) / npc
+    T, P = np.meshgrid(t, p, indexing="ij")
+    sdir = np.stack([np.cos(T) * np.cos(P), np.cos(T) * np.sin(P), np.sin(T)], -1)
+    pats = []
+    for _ in range(3):
+        src = rng.uniform(-1, 1, (4, 3)) * 1.2
+        amp = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+        pats.append(np.einsum("s,ijs->ij", amp, np.exp(1j * np.einsum("ijc,sc->ijs", sdir, src))))
+    child = np.stack(pats)
+    off_p = -np.pi / 2 + 1e-10 if off == "out" else -np.pi / 2 + np.pi / (2 * ntp)
+    print(f"case: interp {ntc}x{npc} -> {ntp}x{npp}", flush=True)
+    out = solver.interp(torch.from_numpy(child).cuda(), ntp, npp, off_p).cpu().numpy()
+    for k in range(child.shape[0]):
+        ref = oracle.sphere_interp(child[k], ntp, npp, off_p)
+        assert rel(out[k], ref) < 2e-5
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_tree_sphere_vs_fp64(solver, depth):
+    s = wf.fibonacci_sphere(12000, 2e-6)
+    nt, np_ = 90, 180
+    print(f"case: tree sphere n=12000 depth={depth}", flush=True)
+    solver.set_geometry(s, depth)
+    I = solver.solve(INCS, (nt, np_))
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS], nt, np_)
+    for k in range(len(INCS)):
+        assert rel(I[k], Iref[k]) < 1e-3, (k, rel(I[k], Iref[k]))
+
+
+def test_tree_complex_E_vs_fp64(solver):
+    s = wf.fibonacci_sphere(6000, 1.5e-6)
+    nt, np_ = 36, 72
+    print("case: tree complex E depth=2", flush=True)
+    solver.set_geometry(s, 2)
+    I, E = solver.solve(INCS[:2], (nt, np_), return_E=True)
+    Iref, Eref = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS[:2]], nt, np_)
+    for k in range(2):
+        assert rel(E[k].reshape(3, -1), Eref[k]) < 1e-3
+
+
+def test_tree_ellipsoid_fixture(solver):
+    import os
+
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    s = wf.read_soa_mesh(os.path.join(g, "ellipsoid_3_3_2_float.binary"),
+                         os.path.join(g, "ellipsoid_3_3_2_normal_float.binary"),
+                         os.path.join(g, "ellipsoid_3_3_2_area_float.binary"))
+    # compareBEM3d/ellipsoid.cu: lambda 500 nm, kappa 0.3, incidence +x, TM and TE
+    incs = [wf.Incidence(0.0, 0.0, 500e-9, "M", 1.55 + 0.3j), wf.Incidence(0.0, 0.0, 500e-9, "E", 1.55 + 0.3j)]
+    nt, np_ = 60, 120
+    print("case: ellipsoid fixture depth=3", flush=True)
+    solver.set_geometry(s, 3)
+    I = solver.solve(incs, (nt, np_))
+    Id = solver.solve_fp64_direct(s, incs, (nt, np_))
+    for k in range(2):
+        assert rel(I[k], Id[k]) < 1e-3

@@ -1,0 +1,872 @@
+// wf_api.hip -- C-ABI of the MI355X PO solver: handle, geometry/octree cache,
+// per-wavelength solve orchestration (row B; drivers treePO.cu:21-992,
+// sphere.cu:21-966 restructured as a library).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wf_po.h"
+#include "wf_common.h"
+#include "wf_fft.h"
+#include "wf_interp.h"
+#include "wf_octree.h"
+
+namespace {
+
+thread_local char g_err[1024] = "";
+
+wf_status set_err(wf_status s, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return s;
+}
+
+#define WF_HIP(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return set_err(WF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+#define WF_TRY(expr)                  \
+  do {                                \
+    wf_status s_ = (expr);            \
+    if (s_ != WF_OK) return s_;       \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct LevelDev {
+  int count = 0;
+  uint32_t* key = nullptr;   // level-local node ids (level order)
+  int32_t* begin = nullptr;  // first sorted point
+  int32_t* end = nullptr;
+  int32_t* cb = nullptr;     // children range in the next level's list
+  int32_t* ce = nullptr;
+};
+
+struct FftEntry {
+  FftPlanDev plan;
+  float2* tw = nullptr;
+};
+
+}  // namespace
+
+struct wf_handle {
+  wf_opts opts;
+  int device = 0;
+  size_t bytes_in_use = 0;
+  std::map<std::string, DevBuf> bufs;
+  std::map<int, FftEntry> ffts;
+  // geometry
+  bool has_geom = false;
+  int64_t n = 0;
+  int depth = 0;
+  float boxf[6] = {0, 0, 0, 0, 0, 0};
+  double box[6] = {0, 0, 0, 0, 0, 0};
+  float* pts = nullptr;     // sorted SoA: x y z nx ny nz area (7 n)
+  int32_t* perm = nullptr;  // sorted position -> original index
+  uint32_t* keys = nullptr; // sorted keys
+  std::vector<LevelDev> levels;
+  // incidence upload (pinned staging)
+  WfIncDev* inc_host = nullptr;
+  int inc_host_cap = 0;
+  hipEvent_t inc_evt = nullptr;
+  hipEvent_t ev[8] = {};
+  wf_stats stats;
+};
+
+namespace {
+
+wf_status dalloc(wf_handle* h, const char* name, size_t bytes, void** out) {
+  DevBuf& b = h->bufs[name];
+  if (bytes == 0) bytes = 16;
+  if (b.cap >= bytes) {
+    *out = b.p;
+    return WF_OK;
+  }
+  if (b.p) {
+    hipFree(b.p);
+    h->bytes_in_use -= b.cap;
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  const size_t want = bytes + bytes / 8;  // headroom against regrowth
+  if (h->opts.workspace_cap_bytes > 0 && h->bytes_in_use + want > (size_t)h->opts.workspace_cap_bytes)
+    return set_err(WF_ERR_OOM, "workspace cap %lld bytes exceeded (buffer %s needs %zu)",
+                   (long long)h->opts.workspace_cap_bytes, name, want);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    hipGetLastError();
+    return set_err(WF_ERR_OOM, "hipMalloc(%zu) for %s failed: %s", want, name, hipGetErrorString(e));
+  }
+  b.cap = want;
+  h->bytes_in_use += want;
+  *out = b.p;
+  return WF_OK;
+}
+
+template <typename T>
+wf_status dalloc_t(wf_handle* h, const char* name, size_t count, T** out) {
+  void* p = nullptr;
+  wf_status s = dalloc(h, name, count * sizeof(T), &p);
+  *out = static_cast<T*>(p);
+  return s;
+}
+
+bool cancelled(wf_handle* h) { return h->opts.cancel_flag && *h->opts.cancel_flag != 0; }
+
+// 13-smooth lengths: radices 4, 2, 3, 5, 7, 11, 13 of the in-house FFT.
+bool factor_len(int n, FftPlanDev& p) {
+  if (n < 1) return false;
+  p.n = n;
+  p.nstages = 0;
+  int m = n;
+  const int rad[] = {4, 2, 3, 5, 7, 11, 13};
+  for (int r : rad) {
+    while (m % r == 0) {
+      if (p.nstages >= WF_FFT_MAX_STAGES) return false;
+      p.radix[p.nstages++] = (int8_t)r;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+bool is_smooth(int n) {
+  FftPlanDev p;
+  return factor_len(n, p);
+}
+
+int smooth_up(int n) {
+  while (!is_smooth(n)) ++n;
+  return n;
+}
+int smooth_even_up(int n) {
+  n = smooth_up(n);
+  while (n % 2) n = smooth_up(n + 1);
+  return n;
+}
+
+wf_status get_fft(wf_handle* h, int n, FftPlanDev* out) {
+  auto it = h->ffts.find(n);
+  if (it != h->ffts.end()) {
+    *out = it->second.plan;
+    return WF_OK;
+  }
+  FftEntry e;
+  if (!factor_len(n, e.plan))
+    return set_err(WF_ERR_UNSUPPORTED, "FFT length %d has a prime factor above 13", n);
+  std::vector<float2> tw(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * WF_PI * (double)m / (double)n;
+    tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  WF_HIP(hipMalloc(&e.tw, sizeof(float2) * n));
+  WF_HIP(hipMemcpy(e.tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  e.plan.tw = e.tw;
+  h->ffts[n] = e;
+  *out = e.plan;
+  return WF_OK;
+}
+
+// Incidence parameters in FP64 (treePO.cu:501-509, :580-601).
+WfIncDev make_inc(const wf_incidence& in) {
+  WfIncDev d;
+  memset(&d, 0, sizeof d);
+  const double ti = in.theta, pi_ = in.phi;
+  d.w[0] = std::cos(ti) * std::cos(pi_);
+  d.w[1] = std::cos(ti) * std::sin(pi_);
+  d.w[2] = std::sin(ti);
+  const double a[3] = {-std::sin(ti) * std::cos(pi_), -std::sin(ti) * std::sin(pi_), std::cos(ti)};
+  const double b[3] = {std::sin(pi_), -std::cos(pi_), 0.0};
+  const double c[3] = {-std::sin(pi_), std::cos(pi_), 0.0};
+  for (int q = 0; q < 3; ++q) {
+    if (in.pol == 'M') {
+      d.E[q] = a[q];
+      d.H[q] = b[q];
+    } else {
+      d.E[q] = c[q];
+      d.H[q] = a[q];
+    }
+  }
+  const double mu0 = 4e-7 * WF_PI, eps0 = 8.8541878e-12, c0 = 299792458.0;
+  d.k0 = 2.0 * WF_PI / in.lambda_m;
+  d.z0 = std::sqrt(mu0 / eps0);
+  d.sigma = (std::isfinite(in.beam_sigma_m) && in.beam_sigma_m > 0) ? in.beam_sigma_m : 0.0;
+  d.eta_re = in.eta_re;
+  d.eta_im = in.eta_im;
+  const double omega = d.k0 * c0;
+  // e^{-i k0 D}/D, D = 1 m; E_f = -i w mu0/(4 pi) g, H_f = -i w eps0/(4 pi) g
+  const double gr = std::cos(-d.k0), gi = std::sin(-d.k0);
+  const double fe = omega * mu0 / (4 * WF_PI), fh = omega * eps0 / (4 * WF_PI);
+  d.ef_re = fe * gi;   // -i * fe * (gr + i gi) = fe*gi - i fe*gr
+  d.ef_im = -fe * gr;
+  d.hf_re = fh * gi;
+  d.hf_im = -fh * gr;
+  d.mode = in.pol;
+  return d;
+}
+
+wf_status check_inc(const wf_incidence* inc, int n_inc) {
+  if (!inc || n_inc <= 0) return set_err(WF_ERR_INVALID_ARG, "incidences: null or n_inc <= 0");
+  for (int i = 0; i < n_inc; ++i) {
+    const wf_incidence& t = inc[i];
+    if (!(t.pol == 'M' || t.pol == 'E'))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: pol must be 'M' or 'E'", i);
+    if (!(std::isfinite(t.lambda_m) && t.lambda_m > 0))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: lambda_m must be finite and > 0", i);
+    if (!(std::isfinite(t.theta) && std::isfinite(t.phi)))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: theta/phi must be finite", i);
+    if (!(std::isfinite(t.eta_re) && std::isfinite(t.eta_im)) || t.eta_im < 0 ||
+        (t.eta_re == -1.0 && t.eta_im == 0.0) || (t.eta_re == 0.0 && t.eta_im == 0.0))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: eta must be finite with kappa >= 0", i);
+  }
+  return WF_OK;
+}
+
+wf_status upload_incs(wf_handle* h, const wf_incidence* inc, int n_inc, WfIncDev** dev,
+                      hipStream_t st) {
+  if (h->inc_host_cap < n_inc) {
+    if (h->inc_host) {
+      hipEventSynchronize(h->inc_evt);
+      hipHostFree(h->inc_host);
+    }
+    WF_HIP(hipHostMalloc((void**)&h->inc_host, sizeof(WfIncDev) * n_inc));
+    h->inc_host_cap = n_inc;
+  } else {
+    WF_HIP(hipEventSynchronize(h->inc_evt));
+  }
+  for (int i = 0; i < n_inc; ++i) h->inc_host[i] = make_inc(inc[i]);
+  WF_TRY(dalloc_t(h, "inc", n_inc, dev));
+  WF_HIP(hipMemcpyAsync(*dev, h->inc_host, sizeof(WfIncDev) * n_inc, hipMemcpyHostToDevice, st));
+  WF_HIP(hipEventRecord(h->inc_evt, st));
+  return WF_OK;
+}
+
+struct Grid {
+  int nt, np;
+};
+
+// Level sampling (row A7).  The reference rule (treePO.cu:342-375) is
+// N = ceil(k d + beta (k d)^(1/3)) with d = 0.5^i * (extent * rho).  Here the
+// extent is the node's own bound: theta samples the great circle through the
+// poles (band limit k R, R = node half-diagonal, 2 pi extension -> Nt >= L),
+// phi samples the azimuth (band limit k R_xy, Np >= 2 L), rounded up to FFT
+// lengths (Np even for the extension).
+Grid level_grid(const wf_handle* h, int level, double lambda_s) {
+  const double ks = 2.0 * WF_PI / lambda_s;
+  double hx = 0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, level);
+  double hy = 0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, level);
+  double hz = 0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, level);
+  const double R = std::sqrt(hx * hx + hy * hy + hz * hz);
+  const double Rxy = std::sqrt(hx * hx + hy * hy);
+  const double beta = h->opts.beta;
+  auto L = [&](double r) { return ks * r + beta * std::cbrt(ks * r); };
+  Grid g;
+  g.nt = smooth_up(std::max(4, (int)std::ceil(L(R))));
+  g.np = smooth_even_up(std::max(8, (int)std::ceil(2.0 * L(Rxy))));
+  return g;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+wf_status set_device(wf_handle* h) {
+  if (h->device >= 0) WF_HIP(hipSetDevice(h->device));
+  return WF_OK;
+}
+
+}  // namespace
+
+// ============================================================== C-ABI =====
+extern "C" {
+
+int32_t wf_abi_version(void) { return WF_ABI_VERSION; }
+
+const char* wf_last_error(void) { return g_err; }
+
+void wf_default_opts(wf_opts* o) {
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->struct_size = (int32_t)sizeof(wf_opts);
+  o->device = -1;
+  o->precision = WF_PREC_FP32;
+  o->strict_grid = 0;
+  o->beta = 4.0;
+  o->box_enlarge = 1.1;
+  o->lambda_sample_m = 0.0;
+  o->has_box = 0;
+  o->workspace_cap_bytes = 0;
+  o->cancel_flag = nullptr;
+  o->field_chunk = 0;
+}
+
+wf_status wf_create(wf_handle** out, const wf_opts* opts) {
+  if (!out) return set_err(WF_ERR_INVALID_ARG, "wf_create: out is null");
+  *out = nullptr;
+  wf_opts o;
+  wf_default_opts(&o);
+  if (opts) {
+    if (opts->struct_size != (int32_t)sizeof(wf_opts))
+      return set_err(WF_ERR_INVALID_ARG, "wf_create: opts.struct_size %d != %d (ABI mismatch)",
+                     opts->struct_size, (int)sizeof(wf_opts));
+    o = *opts;
+  }
+  if (!(o.beta >= 0 && o.beta < 100)) return set_err(WF_ERR_INVALID_ARG, "opts.beta out of range");
+  if (!(o.box_enlarge >= 1.0 && o.box_enlarge < 100))
+    return set_err(WF_ERR_INVALID_ARG, "opts.box_enlarge must be >= 1");
+  if (o.precision != WF_PREC_FP32 && o.precision != WF_PREC_BF16X3)
+    return set_err(WF_ERR_INVALID_ARG, "opts.precision unknown");
+  if (o.has_box) {
+    for (int q = 0; q < 3; ++q)
+      if (!(std::isfinite(o.box[q]) && std::isfinite(o.box[q + 3]) && o.box[q + 3] >= o.box[q]))
+        return set_err(WF_ERR_INVALID_ARG, "opts.box must be finite with max >= min");
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    hipGetLastError();
+    return set_err(WF_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  }
+  wf_handle* h = new wf_handle();
+  h->opts = o;
+  if (o.device >= 0) {
+    if (o.device >= ndev) {
+      delete h;
+      return set_err(WF_ERR_INVALID_ARG, "opts.device %d >= device count %d", o.device, ndev);
+    }
+    h->device = o.device;
+  } else {
+    hipGetDevice(&h->device);
+  }
+  hipSetDevice(h->device);
+  if (hipEventCreateWithFlags(&h->inc_evt, hipEventDisableTiming) != hipSuccess) {
+    delete h;
+    return set_err(WF_ERR_HIP, "hipEventCreate failed");
+  }
+  for (auto& ev : h->ev) hipEventCreate(&ev);
+  memset(&h->stats, 0, sizeof h->stats);
+  *out = h;
+  return WF_OK;
+}
+
+void wf_destroy(wf_handle* h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  hipDeviceSynchronize();
+  for (auto& kv : h->bufs)
+    if (kv.second.p) hipFree(kv.second.p);
+  for (auto& kv : h->ffts)
+    if (kv.second.tw) hipFree(kv.second.tw);
+  if (h->inc_host) hipHostFree(h->inc_host);
+  if (h->inc_evt) hipEventDestroy(h->inc_evt);
+  for (auto& ev : h->ev)
+    if (ev) hipEventDestroy(ev);
+  delete h;
+}
+
+wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!s) return set_err(WF_ERR_INVALID_ARG, "surface is null");
+  if (s->n <= 0) return set_err(WF_ERR_INVALID_ARG, "surface.n must be > 0 (got %lld)", (long long)s->n);
+  if (s->n > 0x7fffffffLL) return set_err(WF_ERR_INVALID_ARG, "surface.n too large");
+  if (depth < 0 || depth > 10) return set_err(WF_ERR_INVALID_ARG, "depth must be in [0, 10]");
+  if (!s->x || !s->y || !s->z || !s->nx || !s->ny || !s->nz || !s->area)
+    return set_err(WF_ERR_INVALID_ARG, "surface: null array pointer");
+  if (s->memspace != WF_MEM_HOST && s->memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "surface.memspace unknown");
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  h->has_geom = false;
+  const int64_t n = s->n;
+  // raw device copies (host input) or the caller's device arrays
+  const float* src[7] = {s->x, s->y, s->z, s->nx, s->ny, s->nz, s->area};
+  if (s->memspace == WF_MEM_HOST) {
+    float* raw = nullptr;
+    WF_TRY(dalloc_t(h, "raw7", 7 * (size_t)n, &raw));
+    for (int q = 0; q < 7; ++q) {
+      WF_HIP(hipMemcpyAsync(raw + q * n, src[q], sizeof(float) * n, hipMemcpyHostToDevice, st));
+      src[q] = raw + q * n;
+    }
+  }
+  // validation + bounding box
+  float* bb = nullptr;
+  int* bad = nullptr;
+  WF_TRY(dalloc_t(h, "bbox", 16, &bb));
+  WF_TRY(dalloc_t(h, "bad", 4, &bad));
+  WF_HIP(wf::launch_validate(src[0], src[1], src[2], src[3], src[4], src[5], src[6], n, bb, bad, st));
+  float hb[6];
+  int hbad = 0;
+  WF_HIP(hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, st));
+  WF_HIP(hipMemcpyAsync(&hbad, bad, sizeof hbad, hipMemcpyDeviceToHost, st));
+  WF_HIP(hipStreamSynchronize(st));
+  if (hbad)
+    return set_err(WF_ERR_INVALID_ARG,
+                   "surface: %d samples with non-finite values, |normal| far from 1 or area < 0", hbad);
+  // root box (treePO.cu:220-230; automatic: bbox centre +- enlarge * half extent)
+  if (h->opts.has_box) {
+    for (int q = 0; q < 6; ++q) h->boxf[q] = (float)h->opts.box[q];
+  } else {
+    double half[3], cen[3], hmax = 0;
+    for (int q = 0; q < 3; ++q) {
+      cen[q] = 0.5 * ((double)hb[q] + (double)hb[q + 3]);
+      half[q] = 0.5 * ((double)hb[q + 3] - (double)hb[q]) * h->opts.box_enlarge;
+      hmax = std::max(hmax, half[q]);
+    }
+    if (hmax <= 0) hmax = 1e-9;
+    for (int q = 0; q < 3; ++q) {
+      if (half[q] < 1e-6 * hmax) half[q] = 1e-6 * hmax;
+      h->boxf[q] = (float)(cen[q] - half[q]);
+      h->boxf[q + 3] = (float)(cen[q] + half[q]);
+    }
+  }
+  for (int q = 0; q < 6; ++q) h->box[q] = (double)h->boxf[q];
+  // float split planes per axis and level (tree.h:116-120 centre, :672-687 boxes)
+  const int per_axis = (1 << depth) - 1;
+  std::vector<float> planes(3 * (size_t)std::max(per_axis, 1));
+  for (int ax = 0; ax < 3; ++ax) {
+    std::vector<float> lo(1, h->boxf[ax]), hi(1, h->boxf[ax + 3]);
+    for (int l = 0; l < depth; ++l) {
+      std::vector<float> nlo(2 << l), nhi(2 << l);
+      for (int i = 0; i < (1 << l); ++i) {
+        const float c = 0.5f * (lo[i] + hi[i]);
+        planes[ax * per_axis + (1 << l) - 1 + i] = c;
+        nlo[2 * i] = lo[i];
+        nhi[2 * i] = c;
+        nlo[2 * i + 1] = c;
+        nhi[2 * i + 1] = hi[i];
+      }
+      lo.swap(nlo);
+      hi.swap(nhi);
+    }
+  }
+  float* planes_dev = nullptr;
+  WF_TRY(dalloc_t(h, "planes", planes.size(), &planes_dev));
+  WF_HIP(hipMemcpyAsync(planes_dev, planes.data(), sizeof(float) * planes.size(),
+                        hipMemcpyHostToDevice, st));
+  // keys + stable sort
+  uint32_t *keys, *keys_tmp;
+  int32_t *idx, *idx_tmp, *counts, *scan_tmp, *flags, *pos, *cnt_dev, *pidx;
+  WF_TRY(dalloc_t(h, "keys", n, &keys));
+  WF_TRY(dalloc_t(h, "keys_tmp", n, &keys_tmp));
+  WF_TRY(dalloc_t(h, "perm", n, &idx));
+  WF_TRY(dalloc_t(h, "idx_tmp", n, &idx_tmp));
+  WF_TRY(dalloc_t(h, "sort_counts", wf::sort_count_ints(n) + 16, &counts));
+  WF_TRY(dalloc_t(h, "scan_tmp", wf::scan_tmp_ints(std::max<int64_t>(n, wf::sort_count_ints(n))) + 16,
+                  &scan_tmp));
+  WF_TRY(dalloc_t(h, "flags", n, &flags));
+  WF_TRY(dalloc_t(h, "pos", n, &pos));
+  WF_TRY(dalloc_t(h, "level_counts", 16, &cnt_dev));
+  WF_TRY(dalloc_t(h, "pidx", n, &pidx));
+  WF_HIP(wf::octree_sort(src[0], src[1], src[2], n, planes_dev, depth, keys, idx, keys_tmp, idx_tmp,
+                         counts, scan_tmp, st));
+  float* pts = nullptr;
+  WF_TRY(dalloc_t(h, "pts7", 7 * (size_t)n, &pts));
+  WF_HIP(wf::gather_points(idx, n, src, pts, st));
+  // per-level active lists
+  h->levels.assign(depth + 1, LevelDev());
+  uint32_t* lk = nullptr;
+  int32_t *lb = nullptr, *le = nullptr, *lcb = nullptr, *lce = nullptr;
+  WF_TRY(dalloc_t(h, "lvl_key", (size_t)(depth + 1) * n, &lk));
+  WF_TRY(dalloc_t(h, "lvl_begin", (size_t)(depth + 1) * n, &lb));
+  WF_TRY(dalloc_t(h, "lvl_end", (size_t)(depth + 1) * n, &le));
+  WF_TRY(dalloc_t(h, "lvl_cb", (size_t)(depth + 1) * n, &lcb));
+  WF_TRY(dalloc_t(h, "lvl_ce", (size_t)(depth + 1) * n, &lce));
+  for (int l = 0; l <= depth; ++l) {
+    LevelDev& L = h->levels[l];
+    L.key = lk + (size_t)l * n;
+    L.begin = lb + (size_t)l * n;
+    L.end = le + (size_t)l * n;
+    L.cb = lcb + (size_t)l * n;
+    L.ce = lce + (size_t)l * n;
+    WF_HIP(wf::octree_level(keys, n, 3 * (depth - l), flags, pos, scan_tmp, cnt_dev + l, L.key,
+                            L.begin, st));
+  }
+  int hc[16] = {0};
+  WF_HIP(hipMemcpyAsync(hc, cnt_dev, sizeof(int) * (depth + 1), hipMemcpyDeviceToHost, st));
+  WF_HIP(hipStreamSynchronize(st));
+  for (int l = 0; l <= depth; ++l) {
+    h->levels[l].count = hc[l];
+    WF_HIP(wf::octree_level_finish(h->levels[l].begin, hc[l], (int32_t)n, h->levels[l].end, st));
+  }
+  for (int l = 0; l < depth; ++l)
+    WF_HIP(wf::octree_links(h->levels[l + 1].begin, hc[l + 1], h->levels[l].begin, hc[l], pidx,
+                            h->levels[l].cb, h->levels[l].ce, st));
+  h->n = n;
+  h->depth = depth;
+  h->pts = pts;
+  h->perm = idx;
+  h->keys = keys;
+  h->has_geom = true;
+  return WF_OK;
+}
+
+wf_status wf_octree_nodes(wf_handle* h, int32_t* perm, int32_t* node_begin, int32_t* node_end,
+                          void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "no geometry set");
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  if (perm)
+    WF_HIP(hipMemcpyAsync(perm, h->perm, sizeof(int32_t) * h->n, hipMemcpyDeviceToDevice, st));
+  if (node_begin && node_end)
+    WF_HIP(wf::octree_node_ranges(h->keys, h->n, h->depth, node_begin, node_end, st));
+  return WF_OK;
+}
+
+wf_status wf_currents(wf_handle* h, const wf_surface* s, const wf_incidence* inc, int32_t n_inc,
+                      int32_t fp64, void* out, void* stream) {
+  if (!h || !s || !out) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  if (s->memspace != WF_MEM_DEVICE) return set_err(WF_ERR_INVALID_ARG, "wf_currents needs device arrays");
+  if (s->n <= 0) return set_err(WF_ERR_INVALID_ARG, "surface.n must be > 0");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  WfIncDev* dinc = nullptr;
+  WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
+  WF_HIP(wf::launch_currents(s->x, s->y, s->z, s->nx, s->ny, s->nz, s->area, s->n, dinc, n_inc,
+                             fp64 ? 1 : 0, out, 6, 6 * s->n, st));
+  return WF_OK;
+}
+
+wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const float* child,
+                    int32_t ntp, int32_t npp, double off_tp, float* parent, void* stream) {
+  if (!h || !child || !parent) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  if (count <= 0 || ntc < 1 || npc < 1 || ntp < 1 || npp < 2 || (npp % 2))
+    return set_err(WF_ERR_INVALID_ARG, "wf_interp: bad sizes (npp must be even)");
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  FftPlanDev pc, pp, tc, tp;
+  WF_TRY(get_fft(h, npc, &pc));
+  WF_TRY(get_fft(h, npp, &pp));
+  WF_TRY(get_fft(h, 2 * ntc, &tc));
+  WF_TRY(get_fft(h, 2 * ntp, &tp));
+  float2* G = nullptr;
+  WF_TRY(dalloc_t(h, "interp_G", (size_t)count * ntc * npp, &G));
+  WF_HIP(wf::launch_phi_pass((const float2*)child, G, count, ntc, npc, npp, pc, pp, st));
+  const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
+  const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
+  WF_HIP(wf::launch_theta_pass(G, 1, ntc, npp, ntp, count, nullptr, nullptr, nullptr, nullptr,
+                               off_tp - off_tc, mirror, tc, tp, (float2*)parent, nullptr, st));
+  return WF_OK;
+}
+
+wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
+                                  int32_t n_inc, wf_grid grid, double* intensity, double* E_cplx,
+                                  void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
+  if (grid.n_theta < 1 || grid.n_phi < 1) return set_err(WF_ERR_INVALID_ARG, "grid must be >= 1x1");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_TRY(wf_set_geometry(h, s, 0, stream));
+  hipStream_t st = as_stream(stream);
+  const int64_t D = (int64_t)grid.n_theta * grid.n_phi;
+  const int64_t n = h->n;
+  WfIncDev* dinc = nullptr;
+  WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
+  double* I_dev = intensity;
+  double* E_dev = E_cplx;
+  if (s->memspace == WF_MEM_HOST) {
+    WF_TRY(dalloc_t(h, "out_Id", (size_t)n_inc * D, &I_dev));
+    if (E_cplx) WF_TRY(dalloc_t(h, "out_Ed", (size_t)n_inc * 6 * D, &E_dev));
+  }
+  const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
+                            0.5 * (h->box[2] + h->box[5])};
+  const float* px = h->pts;
+  // group solves sharing lambda (phase shared), at most 4 per launch (F <= 24)
+  std::vector<int> order(n_inc);
+  for (int i = 0; i < n_inc; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
+  double2 *C = nullptr, *F0 = nullptr;
+  WF_TRY(dalloc_t(h, "direct_C", (size_t)n * 24, &C));
+  WF_TRY(dalloc_t(h, "direct_F0", (size_t)D * 24, &F0));
+  WF_HIP(hipEventRecord(h->ev[0], st));
+  for (int i0 = 0; i0 < n_inc;) {
+    int i1 = i0 + 1;
+    while (i1 < n_inc && i1 - i0 < 4 && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
+    const int S = i1 - i0;
+    if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+    for (int q = 0; q < S; ++q)
+      WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                 px + 6 * n, n, dinc + order[i0 + q], 1, 1, C + 6 * q, 6 * S, 0, st));
+    WF_HIP(wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, 6 * S, centre,
+                                  2.0 * WF_PI / inc[order[i0]].lambda_m, grid.n_theta, grid.n_phi,
+                                  F0, st));
+    for (int q = 0; q < S; ++q) {
+      const int slot = order[i0 + q];
+      WF_HIP(wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, grid.n_theta, grid.n_phi, nullptr,
+                                  nullptr, I_dev + (int64_t)slot * D,
+                                  E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, st));
+    }
+    i0 = i1;
+  }
+  WF_HIP(hipEventRecord(h->ev[1], st));
+  if (s->memspace == WF_MEM_HOST) {
+    WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(double) * n_inc * D, hipMemcpyDeviceToHost, st));
+    if (E_cplx)
+      WF_HIP(hipMemcpyAsync(E_cplx, E_dev, sizeof(double) * n_inc * 6 * D, hipMemcpyDeviceToHost, st));
+    WF_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+    h->stats.ms_total = ms;
+  }
+  return WF_OK;
+}
+
+wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                   float* intensity, float* E_cplx, int32_t out_memspace, void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "wf_solve: no geometry (call wf_set_geometry)");
+  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
+  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
+  if (out_memspace != WF_MEM_HOST && out_memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "out_memspace unknown");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  const int64_t n = h->n;
+  const int depth = h->depth;
+  const int NT = grid.n_theta, NP = grid.n_phi;
+  const int64_t D = (int64_t)NT * NP;
+  if (depth > 0) {
+    if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
+    if (!is_smooth(NP) || !is_smooth(2 * NT))
+      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth", NT, NP);
+  }
+  memset(&h->stats, 0, sizeof h->stats);
+  h->stats.depth = depth;
+  WfIncDev* dinc = nullptr;
+  WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
+  float* I_dev = intensity;
+  float* E_dev = E_cplx;
+  if (out_memspace == WF_MEM_HOST) {
+    WF_TRY(dalloc_t(h, "out_I", (size_t)n_inc * D, &I_dev));
+    if (E_cplx) WF_TRY(dalloc_t(h, "out_E", (size_t)n_inc * 6 * D, &E_dev));
+  }
+  const float* px = h->pts;
+  std::vector<int> order(n_inc);
+  for (int i = 0; i < n_inc; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
+  const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
+                            0.5 * (h->box[2] + h->box[5])};
+  float t_cur = 0, t_leaf = 0, t_agg = 0, t_fin = 0;
+  WF_HIP(hipEventRecord(h->ev[0], st));
+  int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6) : 8;
+  if (depth == 0) max_solves = std::min(max_solves, 4);
+  std::vector<Grid> g(depth + 1);
+  for (int i0 = 0; i0 < n_inc;) {
+    // wavelength group
+    int i1 = i0 + 1;
+    while (i1 < n_inc && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
+    const double lam = inc[order[i0]].lambda_m;
+    const double k0 = 2.0 * WF_PI / lam;
+    const double lam_s = h->opts.lambda_sample_m > 0 ? h->opts.lambda_sample_m : lam;
+    // level grids and translation tables for this wavelength
+    float2* T = nullptr;
+    std::vector<int64_t> toff(depth + 1, 0);
+    if (depth > 0) {
+      g[0] = {NT, NP};
+      for (int l = 1; l <= depth; ++l) g[l] = level_grid(h, l, lam_s);
+      if (h->opts.strict_grid && (NT < g[1].nt || NP < g[1].np))
+        return set_err(WF_ERR_GRID_TOO_COARSE, "output grid %dx%d coarser than level-1 grid %dx%d",
+                       NT, NP, g[1].nt, g[1].np);
+      int64_t tt = 0;
+      for (int l = 0; l < depth; ++l) {
+        toff[l] = tt;
+        tt += 8LL * g[l].nt * g[l].np;
+      }
+      WF_TRY(dalloc_t(h, "trans", (size_t)tt, &T));
+      for (int l = 0; l < depth; ++l) {
+        const double th0 = (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * g[l].nt));
+        const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
+                              0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
+                              0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
+        WF_HIP(wf::launch_trans_table(T + toff[l], g[l].nt, g[l].np, th0, hc, k0, st));
+      }
+      for (int l = 0; l <= depth && l < 16; ++l) {
+        h->stats.level_nt[l] = g[l].nt;
+        h->stats.level_np[l] = g[l].np;
+        h->stats.level_nodes[l] = h->levels[l].count;
+      }
+      h->stats.n_levels = depth + 1;
+    }
+    for (int c0 = i0; c0 < i1; c0 += max_solves) {
+      const int S = std::min(max_solves, i1 - c0);
+      const int F = 6 * S;
+      if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+      if (depth == 0) {
+        double2 *C = nullptr, *F0 = nullptr;
+        WF_TRY(dalloc_t(h, "direct_C", (size_t)n * F, &C));
+        WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
+        for (int q = 0; q < S; ++q)
+          WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                     px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st));
+        WF_HIP(wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0, st));
+        for (int q = 0; q < S; ++q) {
+          const int slot = order[c0 + q];
+          WF_HIP(wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
+                                      E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
+                                      nullptr, st));
+        }
+        continue;
+      }
+      // ---- currents (sorted order), [n][F]
+      float2* C = nullptr;
+      WF_TRY(dalloc_t(h, "C", (size_t)n * F, &C));
+      WF_HIP(hipEventRecord(h->ev[2], st));
+      for (int q = 0; q < S; ++q)
+        WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                   px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F, 0, st));
+      WF_HIP(hipEventRecord(h->ev[3], st));
+      // ---- buffers: patterns per level (ping-pong) and phi-pass scratch
+      size_t pat_max = 0, g_max = 0;
+      for (int l = 1; l <= depth; ++l)
+        pat_max = std::max(pat_max, (size_t)h->levels[l].count * F * g[l].nt * g[l].np);
+      for (int l = 0; l < depth; ++l)
+        g_max = std::max(g_max, (size_t)h->levels[l + 1].count * F * g[l + 1].nt * g[l].np);
+      float2 *patA = nullptr, *patB = nullptr, *G = nullptr;
+      double2* F0 = nullptr;
+      WF_TRY(dalloc_t(h, "patA", pat_max, &patA));
+      WF_TRY(dalloc_t(h, "patB", pat_max, &patB));
+      WF_TRY(dalloc_t(h, "G", g_max, &G));
+      WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
+      // ---- leaf radiation sums (level depth)
+      const LevelDev& LL = h->levels[depth];
+      wf::LeafArgs la;
+      la.px = px;
+      la.py = px + n;
+      la.pz = px + 2 * n;
+      la.n = n;
+      la.C = C;
+      la.Ftot = F;
+      la.f0 = 0;
+      la.F = F;
+      la.leaf_begin = LL.begin;
+      la.leaf_end = LL.end;
+      la.leaf_key = LL.key;
+      la.nleaves = LL.count;
+      la.depth = depth;
+      for (int q = 0; q < 3; ++q) {
+        la.box_lo[q] = h->box[q];
+        la.leaf_size[q] = (h->box[q + 3] - h->box[q]) / std::ldexp(1.0, depth);
+      }
+      la.k0 = k0;
+      la.nt = g[depth].nt;
+      la.np = g[depth].np;
+      float2* cur = (depth % 2) ? patA : patB;
+      la.out = cur;
+      la.precision = h->opts.precision;
+      WF_HIP(wf::launch_leaf(la, st));
+      h->stats.leaf_pairs += (int64_t)n * g[depth].nt * g[depth].np;
+      WF_HIP(hipEventRecord(h->ev[4], st));
+      // ---- aggregation levels depth-1 .. 1
+      for (int l = depth - 1; l >= 1; --l) {
+        if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+        const Grid gc = g[l + 1], gp = g[l];
+        FftPlanDev pc, pp, tc, tp;
+        WF_TRY(get_fft(h, gc.np, &pc));
+        WF_TRY(get_fft(h, gp.np, &pp));
+        WF_TRY(get_fft(h, 2 * gc.nt, &tc));
+        WF_TRY(get_fft(h, 2 * gp.nt, &tp));
+        const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
+        WF_HIP(wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gc.nt, gc.np, gp.np, pc, pp, st));
+        float2* nxt = (cur == patA) ? patB : patA;
+        const double off_c = -WF_PI / 2 + WF_PI / (2.0 * gc.nt);
+        const double off_p = -WF_PI / 2 + WF_PI / (2.0 * gp.nt);
+        const int mirror = (int)std::lround(gp.nt * (1.0 - 2.0 * off_p / WF_PI));
+        WF_HIP(wf::launch_theta_pass(G, F, gc.nt, gp.np, gp.nt, npar, h->levels[l].cb,
+                                     h->levels[l].ce, h->levels[l + 1].key, T + toff[l],
+                                     off_p - off_c, mirror, tc, tp, nxt, nullptr, st));
+        h->stats.agg_bytes += (int64_t)8 * F *
+                              ((int64_t)nchild * gc.nt * gc.np + (int64_t)npar * gp.nt * gp.np);
+        cur = nxt;
+      }
+      WF_HIP(hipEventRecord(h->ev[5], st));
+      // ---- root: level 1 -> output grid, FP64 accumulation
+      {
+        const Grid gc = g[1];
+        FftPlanDev pc, pp, tc, tp;
+        WF_TRY(get_fft(h, gc.np, &pc));
+        WF_TRY(get_fft(h, NP, &pp));
+        WF_TRY(get_fft(h, 2 * gc.nt, &tc));
+        WF_TRY(get_fft(h, 2 * NT, &tp));
+        const int nchild = h->levels[1].count;
+        float2* G0 = nullptr;
+        WF_TRY(dalloc_t(h, "G0", (size_t)nchild * F * gc.nt * NP, &G0));
+        WF_HIP(wf::launch_phi_pass(cur, G0, (int64_t)nchild * F, gc.nt, gc.np, NP, pc, pp, st));
+        const double off_c = -WF_PI / 2 + WF_PI / (2.0 * gc.nt);
+        const double off_p = -WF_PI / 2 + 1e-10;
+        const int mirror = (int)std::lround(NT * (1.0 - 2.0 * off_p / WF_PI));
+        WF_HIP(wf::launch_theta_pass(G0, F, gc.nt, NP, NT, 1, h->levels[0].cb, h->levels[0].ce,
+                                     h->levels[1].key, T + toff[0], off_p - off_c, mirror, tc, tp,
+                                     nullptr, F0, st));
+        for (int q = 0; q < S; ++q) {
+          const int slot = order[c0 + q];
+          WF_HIP(wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
+                                      E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
+                                      nullptr, st));
+        }
+      }
+      WF_HIP(hipEventRecord(h->ev[6], st));
+      // stage timings are only read on host-synchronous calls
+      if (out_memspace == WF_MEM_HOST) {
+        WF_HIP(hipEventSynchronize(h->ev[6]));
+        float a = 0, b = 0, c = 0, d = 0;
+        hipEventElapsedTime(&a, h->ev[2], h->ev[3]);
+        hipEventElapsedTime(&b, h->ev[3], h->ev[4]);
+        hipEventElapsedTime(&c, h->ev[4], h->ev[5]);
+        hipEventElapsedTime(&d, h->ev[5], h->ev[6]);
+        t_cur += a;
+        t_leaf += b;
+        t_agg += c;
+        t_fin += d;
+      }
+    }
+    i0 = i1;
+  }
+  WF_HIP(hipEventRecord(h->ev[1], st));
+  if (out_memspace == WF_MEM_HOST) {
+    WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
+    if (E_cplx)
+      WF_HIP(hipMemcpyAsync(E_cplx, E_dev, sizeof(float) * n_inc * 6 * D, hipMemcpyDeviceToHost, st));
+    WF_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+    h->stats.ms_total = ms;
+    h->stats.ms_currents = t_cur;
+    h->stats.ms_leaf = t_leaf;
+    h->stats.ms_aggregate = t_agg;
+    h->stats.ms_final = t_fin;
+  }
+  (void)centre;
+  return WF_OK;
+}
+
+wf_status wf_po_solve(wf_handle* h, const wf_surface* s, const wf_incidence* inc, int32_t n_inc,
+                      wf_grid grid, int32_t depth, float* intensity, float* E_cplx, void* stream) {
+  WF_TRY(wf_set_geometry(h, s, depth, stream));
+  return wf_solve(h, inc, n_inc, grid, intensity, E_cplx, s->memspace, stream);
+}
+
+wf_status wf_get_stats(wf_handle* h, wf_stats* out) {
+  if (!h || !out) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  *out = h->stats;
+  return WF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// wf_interp.h -- host launchers of the aggregation kernels and leaf stage.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wf_fft.h"
+
+namespace wf {
+
+hipError_t launch_phi_pass(const float2* in, float2* out, int64_t planes, int ntc, int npc, int npp,
+                           const FftPlanDev& pc, const FftPlanDev& pp, hipStream_t st);
+
+// cb/ce: per-parent child range in the child list (nullable: parent p <- child p).
+// child_key: level-local node id of each child (octant = key & 7), nullable.
+// T: translation table [8][ntp][npp] (nullable: no translation).
+// Exactly one of out / out_root is non-null.
+hipError_t launch_theta_pass(const float2* G, int F, int ntc, int npp, int ntp, int nparents,
+                             const int32_t* cb, const int32_t* ce, const uint32_t* child_key,
+                             const float2* T, double shift, int mirror_base,
+                             const FftPlanDev& pc, const FftPlanDev& pp, float2* out,
+                             double2* out_root, hipStream_t st);
+
+hipError_t launch_trans_table(float2* T, int ntp, int npp, double theta0, const double h[3],
+                              double k0, hipStream_t st);
+
+// Leaf radiation sum: pattern[leaf][F][nt][np] of
+// sum_{n in leaf} C[n][f] exp(+i k0 s.(x_n - c_leaf)) on the leaf grid
+// theta_j = -pi/2 + pi (j + 1/2)/nt, phi_k = 2 pi k/np.
+struct LeafArgs {
+  const float* px;  // sorted positions (x, y, z consecutive arrays of n)
+  const float* py;
+  const float* pz;
+  int64_t n;
+  const float2* C;  // currents [n][Ftot] (sorted order)
+  int Ftot;         // fields per point in C
+  int f0, F;        // field window handled (pattern holds F fields)
+  const int32_t* leaf_begin;
+  const int32_t* leaf_end;
+  const uint32_t* leaf_key;
+  int nleaves;
+  int depth;
+  double box_lo[3];
+  double leaf_size[3];  // full leaf box size per axis
+  double k0;
+  int nt, np;
+  float2* out;
+  int precision;
+};
+hipError_t launch_leaf(const LeafArgs& a, hipStream_t st);
+
+}  // namespace wf

@@ -1,0 +1,278 @@
+"""Host-side mirror of WaveFiber3d's PO solver interface (over the C-ABI).
+
+The reference exposes its solver as driver programs (fibersim/treePO.cu,
+compareMie/sphere.cu, compareBEM3d/ellipsoid.cu) calling header functions
+of util/*.h.  ``POSolver`` exposes the same steps -- geometry + octree
+(tree.h:289), currents (tangentplane.h:84), the tree solve (treePO.cu:501-961)
+and the FP64 plain method (depth 0) -- as methods over libwfpo.so.
+
+Arrays may be numpy (host memory; the call returns when results are in host
+memory) or torch CUDA tensors (device memory; asynchronous on the current
+torch stream).  Every entry point raises ``WfError`` with the library's status
+and message on failure; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import WF_MEM_DEVICE, WF_MEM_HOST, check, lib
+from .geometry import SurfaceSamples
+
+# Reference parameter grids -------------------------------------------------
+
+
+def reference_lambda(index: int) -> float:
+    """Wavelength of reference index l: 400 + 300/25 l nm (treePO.cu:504)."""
+    return (400.0 + 300.0 / 25.0 * index) * 1e-9
+
+
+def reference_incidence_theta(index: int, count: int = 180) -> float:
+    """thetai = -pi/2 + 1e-10 + pi t/180 (treePO.cu:560)."""
+    return -math.pi / 2 + 1e-10 + math.pi / count * index
+
+
+def reference_incidence_phi(index: int, count: int = 360) -> float:
+    """phii = 2 pi p/360 (treePO.cu:564)."""
+    return 2 * math.pi / count * index
+
+
+@dataclass
+class Incidence:
+    theta: float
+    phi: float
+    lambda_m: float
+    pol: str = "M"
+    eta: complex = 1.55 + 0j
+    beam_sigma_m: float = 0.0
+
+    def to_c(self) -> _lib.Incidence:
+        return _lib.Incidence(self.theta, self.phi, self.lambda_m, self.pol.encode(),
+                              complex(self.eta).real, complex(self.eta).imag, self.beam_sigma_m)
+
+    def as_tuple(self):
+        return (self.theta, self.phi, self.lambda_m, self.pol, complex(self.eta), self.beam_sigma_m)
+
+
+def _incs(incs) -> tuple:
+    incs = list(incs)
+    arr = (_lib.Incidence * len(incs))()
+    for i, t in enumerate(incs):
+        if not isinstance(t, Incidence):
+            t = Incidence(*t)
+        arr[i] = t.to_c()
+    return arr, len(incs)
+
+
+def output_directions(ntheta: int, nphi: int) -> np.ndarray:
+    """Output grid directions (treePO.cu:378-391), float64 [ntheta*nphi, 3]."""
+    t = -np.pi / 2 + 1e-10 + np.pi / ntheta * np.arange(ntheta)
+    p = 2 * np.pi / nphi * np.arange(nphi)
+    T, P = np.meshgrid(t, p, indexing="ij")
+    return np.stack([np.cos(T) * np.cos(P), np.cos(T) * np.sin(P), np.sin(T)], -1).reshape(-1, 3)
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a) -> int:
+    if _is_torch(a):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _stream_ptr(arrays) -> int | None:
+    for a in arrays:
+        if _is_torch(a) and a.is_cuda:
+            import torch
+
+            return torch.cuda.current_stream(a.device).cuda_stream
+    return None
+
+
+class POSolver:
+    """Handle over the C-ABI (one per host thread)."""
+
+    def __init__(self, device: int = -1, precision: str = "fp32", beta: float = 4.0,
+                 box=None, box_enlarge: float = 1.1, lambda_sample_m: float = 0.0,
+                 strict_grid: bool = False, workspace_cap_bytes: int = 0,
+                 field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None):
+        L = lib()
+        o = _lib.Opts()
+        L.wf_default_opts(ct.byref(o))
+        o.device = device
+        o.precision = {"fp32": _lib.WF_PREC_FP32, "bf16x3": _lib.WF_PREC_BF16X3}[precision]
+        o.beta = beta
+        o.box_enlarge = box_enlarge
+        o.lambda_sample_m = lambda_sample_m
+        o.strict_grid = int(strict_grid)
+        o.workspace_cap_bytes = workspace_cap_bytes
+        o.field_chunk = field_chunk
+        if box is not None:
+            o.has_box = 1
+            for i in range(6):
+                o.box[i] = float(box[i])
+        self._cancel = cancel_flag
+        if cancel_flag is not None:
+            o.cancel_flag = ct.pointer(cancel_flag)
+        self._h = ct.c_void_p()
+        check(L.wf_create(ct.byref(self._h), ct.byref(o)))
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().wf_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -------------------------------------------------------------- geometry
+    def _surface(self, s):
+        arrs = s.arrays() if isinstance(s, SurfaceSamples) else list(s)
+        if len(arrs) != 7:
+            raise ValueError("surface needs x, y, z, nx, ny, nz, area")
+        dev = _is_torch(arrs[0]) and arrs[0].is_cuda
+        if dev:
+            import torch
+
+            arrs = [a.to(torch.float32).contiguous() for a in arrs]
+        else:
+            arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in arrs]
+        n = int(arrs[0].shape[0])
+        if any(int(a.shape[0]) != n for a in arrs):
+            raise ValueError("surface arrays differ in length")
+        c = _lib.Surface(*[_ptr(a) for a in arrs], n, WF_MEM_DEVICE if dev else WF_MEM_HOST)
+        return c, arrs, dev
+
+    def set_geometry(self, surface, depth: int):
+        """Upload + octree build (tree.h:289-715; treePO.cu:137-337)."""
+        c, arrs, dev = self._surface(surface)
+        self._keep = arrs
+        check(lib().wf_set_geometry(self._h, ct.byref(c), depth, _stream_ptr(arrs)))
+        self.depth = depth
+        self.n = c.n
+        self._dev_geom = dev
+
+    # ----------------------------------------------------------------- solves
+    def solve(self, incidences, grid, return_E: bool = False, out: str = "numpy"):
+        """Tree (or depth-0) PO solve on the current geometry.
+
+        Returns intensity [n_inc, Nt, Np] float32 (and E [n_inc, 3, Nt, Np]
+        complex64 when return_E).  out = "numpy" (host) or "torch" (device)."""
+        arr, k = _incs(incidences)
+        nt, np_ = grid
+        g = _lib.Grid(nt, np_)
+        if out == "torch":
+            import torch
+
+            I = torch.empty((k, nt, np_), dtype=torch.float32, device="cuda")
+            E = torch.empty((k, 3, nt, np_, 2), dtype=torch.float32, device="cuda") if return_E else None
+            st = torch.cuda.current_stream().cuda_stream
+            check(lib().wf_solve(self._h, arr, k, g, I.data_ptr(), E.data_ptr() if E is not None else None,
+                                 WF_MEM_DEVICE, st))
+            if return_E:
+                return I, torch.view_as_complex(E)
+            return I
+        I = np.empty((k, nt, np_), dtype=np.float32)
+        E = np.empty((k, 3, nt, np_, 2), dtype=np.float32) if return_E else None
+        check(lib().wf_solve(self._h, arr, k, g, I.ctypes.data,
+                             E.ctypes.data if E is not None else None, WF_MEM_HOST, None))
+        if return_E:
+            return I, E.view(np.complex64)[..., 0]
+        return I
+
+    def po_solve(self, surface, incidences, grid, depth: int, return_E: bool = False):
+        self.set_geometry(surface, depth)
+        return self.solve(incidences, grid, return_E=return_E,
+                          out="torch" if self._dev_geom else "numpy")
+
+    def solve_fp64_direct(self, surface, incidences, grid, return_E: bool = False):
+        """FP64 plain method on the GPU (sphere.cu:592-658 with depth 0)."""
+        c, arrs, dev = self._surface(surface)
+        arr, k = _incs(incidences)
+        nt, np_ = grid
+        g = _lib.Grid(nt, np_)
+        if dev:
+            import torch
+
+            I = torch.empty((k, nt, np_), dtype=torch.float64, device=arrs[0].device)
+            E = torch.empty((k, 3, nt, np_, 2), dtype=torch.float64, device=arrs[0].device) if return_E else None
+            check(lib().wf_po_solve_fp64_direct(self._h, ct.byref(c), arr, k, g, I.data_ptr(),
+                                                E.data_ptr() if E is not None else None,
+                                                _stream_ptr(arrs)))
+            self.depth = 0
+            if return_E:
+                return I, torch.view_as_complex(E)
+            return I
+        I = np.empty((k, nt, np_), dtype=np.float64)
+        E = np.empty((k, 3, nt, np_, 2), dtype=np.float64) if return_E else None
+        check(lib().wf_po_solve_fp64_direct(self._h, ct.byref(c), arr, k, g, I.ctypes.data,
+                                            E.ctypes.data if E is not None else None, None))
+        self.depth = 0
+        if return_E:
+            return I, E.view(np.complex128)[..., 0]
+        return I
+
+    # ------------------------------------------------------------ stage APIs
+    def currents(self, surface_dev, incidences, fp64: bool = False):
+        """computeLocalTangentPlane + area + E_f/H_f (device tensors in, [n_inc, n, 6] complex out)."""
+        import torch
+
+        c, arrs, dev = self._surface(surface_dev)
+        if not dev:
+            raise ValueError("currents() needs CUDA tensors")
+        arr, k = _incs(incidences)
+        dt = torch.float64 if fp64 else torch.float32
+        out = torch.empty((k, c.n, 6, 2), dtype=dt, device=arrs[0].device)
+        check(lib().wf_currents(self._h, ct.byref(c), arr, k, int(fp64), out.data_ptr(),
+                                _stream_ptr(arrs)))
+        return torch.view_as_complex(out)
+
+    def octree_nodes(self):
+        """(perm [n], node_begin, node_end [(8^(d+1)-1)/7]) as int32 CUDA tensors."""
+        import torch
+
+        nn = (8 ** (self.depth + 1) - 1) // 7
+        perm = torch.empty(self.n, dtype=torch.int32, device="cuda")
+        nb = torch.empty(nn, dtype=torch.int32, device="cuda")
+        ne = torch.empty(nn, dtype=torch.int32, device="cuda")
+        check(lib().wf_octree_nodes(self._h, perm.data_ptr(), nb.data_ptr(), ne.data_ptr(),
+                                    torch.cuda.current_stream().cuda_stream))
+        return perm, nb, ne
+
+    def interp(self, child, ntp: int, npp: int, off_tp: float):
+        """copyandinterp/interpFFT on a batch of child patterns (complex64 CUDA tensor [count, ntc, npc])."""
+        import torch
+
+        child = child.to(torch.complex64).contiguous()
+        count, ntc, npc = child.shape
+        out = torch.empty((count, ntp, npp), dtype=torch.complex64, device=child.device)
+        check(lib().wf_interp(self._h, count, ntc, npc, child.data_ptr(), ntp, npp, off_tp,
+                              out.data_ptr(), torch.cuda.current_stream().cuda_stream))
+        return out
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(lib().wf_get_stats(self._h, ct.byref(s)))
+        nl = s.n_levels
+        return {
+            "ms_currents": s.ms_currents, "ms_leaf": s.ms_leaf, "ms_aggregate": s.ms_aggregate,
+            "ms_final": s.ms_final, "ms_total": s.ms_total, "leaf_pairs": s.leaf_pairs,
+            "agg_bytes": s.agg_bytes, "depth": s.depth,
+            "levels": [(s.level_nt[i], s.level_np[i], s.level_nodes[i]) for i in range(nl)],
+        }
