@@ -45,15 +45,18 @@ typedef enum wf_status {
   WF_ERR_OOM = 3,             /* device allocation failed or workspace cap exceeded */
   WF_ERR_HIP = 4,             /* HIP runtime error (message carries hipGetErrorString) */
   WF_ERR_CANCELLED = 5,       /* *cancel_flag became nonzero between stages */
-  WF_ERR_UNSUPPORTED = 6,     /* e.g. an FFT length with a prime factor above 7 */
+  WF_ERR_UNSUPPORTED = 6,     /* e.g. an FFT length with a prime factor above 13 */
   WF_ERR_NO_GEOMETRY = 7      /* wf_solve before wf_set_geometry */
 } wf_status;
 
 enum { WF_MEM_HOST = 0, WF_MEM_DEVICE = 1 };
 
 /* Leaf-GEMM arithmetic.  All modes accumulate in FP32; phases are range
- * reduced from FP64 arguments. */
-enum { WF_PREC_FP32 = 0, WF_PREC_BF16X3 = 1 };
+ * reduced exactly in revolutions.
+ *  WF_PREC_FP32       FP32 MFMA (v_mfma_f32_16x16x4_f32)
+ *  WF_PREC_BF16X3     split hi/lo BF16 MFMA, 3 products (v_mfma_f32_16x16x32_bf16)
+ *  WF_PREC_FP32_VALU  FP32 on the vector ALU (check path for the MFMA kernels) */
+enum { WF_PREC_FP32 = 0, WF_PREC_BF16X3 = 1, WF_PREC_FP32_VALU = 2 };
 
 /* Surface samples, SoA float32, n each (compareMie/sphere.cu:92-125 input
  * form; util/tree.h:67-97 Points and :38-62 Area). */
@@ -101,12 +104,27 @@ typedef struct wf_opts {
 
 typedef struct wf_handle wf_handle;
 
-/* Timing/roofline breakdown of the last solve (hipEvent based, milliseconds). */
+/* Kernel kinds of the per-kernel timing table in wf_stats. */
+enum {
+  WF_K_CURRENTS = 0, /* work unit: algorithmic HBM bytes */
+  WF_K_LEAF = 1,     /* work unit: real FLOP (8 per complex MAC) */
+  WF_K_PHI = 2,      /* bytes */
+  WF_K_THETA = 3,    /* bytes */
+  WF_K_TRANS = 4,    /* bytes */
+  WF_K_EPILOGUE = 5, /* bytes */
+  WF_K_DIRECT = 6,   /* real FLOP (48 per point-direction-solve) */
+  WF_K_COUNT = 8
+};
+
+/* Timing/roofline breakdown of the last solve, from HIP events recorded around
+ * every kernel launch on the solve's stream (milliseconds).  For an
+ * asynchronous (device-memspace) solve, wf_get_stats waits for those events. */
 typedef struct wf_stats {
-  double ms_currents, ms_leaf, ms_aggregate, ms_final, ms_total;
-  double ms_tree_build;
-  int64_t leaf_pairs;       /* sum over leaves of D_leaf * n_leaf (per lambda group, summed) */
-  int64_t agg_bytes;        /* compulsory aggregation bytes (child read + parent write) */
+  double ms_total;
+  double kern_ms[WF_K_COUNT];       /* summed launch durations per kernel kind */
+  int32_t kern_launches[WF_K_COUNT];
+  double kern_work[WF_K_COUNT];     /* summed algorithmic work (see kinds) */
+  int64_t leaf_pairs;               /* sum over lambda groups of D_leaf * N */
   int32_t depth, n_levels;
   int32_t level_nt[16], level_np[16]; /* level grids of the last lambda group */
   int32_t level_nodes[16];            /* active nodes per level */

@@ -150,3 +150,22 @@ def test_tree_ellipsoid_fixture(solver):
     Id = solver.solve_fp64_direct(s, incs, (nt, np_))
     for k in range(2):
         assert rel(I[k], Id[k]) < 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp32_valu"])
+def test_tree_precision_modes(precision):
+    """Leaf GEMM on FP32 MFMA, split-BF16x3 MFMA and the vector ALU: each tree
+    solve within 1e-3 of the FP64 oracle, and MFMA FP32 vs vector FP32 close."""
+    s = wf.fibonacci_sphere(8000, 1.5e-6)
+    # 7 identical solves share one wavelength group (F = 42 fields, 6 MFMA tiles)
+    incs = INCS[:2] + [wf.Incidence(0.2, 0.4, 0.45e-6, "M", 1.55 + 0.05j)] * 7
+    nt, np_ = 40, 80
+    print(f"case: tree precision {precision} depth=3", flush=True)
+    with wf.POSolver(device=0, precision=precision) as sv:
+        sv.set_geometry(s, 3)
+        I = sv.solve(incs, (nt, np_))
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs[:3]], nt, np_)
+    for k in range(3):
+        assert rel(I[k], Iref[k]) < 1e-3, (precision, k, rel(I[k], Iref[k]))
+    for k in range(3, len(incs)):
+        assert rel(I[k], I[2]) < 1e-6  # identical incidences in other field slots

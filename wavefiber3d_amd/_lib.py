@@ -23,7 +23,7 @@ STATUS_NAMES = {
     7: "WF_ERR_NO_GEOMETRY",
 }
 WF_MEM_HOST, WF_MEM_DEVICE = 0, 1
-WF_PREC_FP32, WF_PREC_BF16X3 = 0, 1
+WF_PREC_FP32, WF_PREC_BF16X3, WF_PREC_FP32_VALU = 0, 1, 2
 
 EXPORTS = (
     "wf_abi_version", "wf_last_error", "wf_default_opts", "wf_create", "wf_destroy",
@@ -78,11 +78,18 @@ class Opts(ct.Structure):
     ]
 
 
+KERNEL_KINDS = ("currents", "leaf", "phi_pass", "theta_pass", "trans_table", "epilogue",
+                "direct_fp64", "other")
+WF_K_COUNT = 8
+
+
 class Stats(ct.Structure):
     _fields_ = [
-        ("ms_currents", ct.c_double), ("ms_leaf", ct.c_double), ("ms_aggregate", ct.c_double),
-        ("ms_final", ct.c_double), ("ms_total", ct.c_double), ("ms_tree_build", ct.c_double),
-        ("leaf_pairs", ct.c_int64), ("agg_bytes", ct.c_int64),
+        ("ms_total", ct.c_double),
+        ("kern_ms", ct.c_double * WF_K_COUNT),
+        ("kern_launches", ct.c_int32 * WF_K_COUNT),
+        ("kern_work", ct.c_double * WF_K_COUNT),
+        ("leaf_pairs", ct.c_int64),
         ("depth", ct.c_int32), ("n_levels", ct.c_int32),
         ("level_nt", ct.c_int32 * 16), ("level_np", ct.c_int32 * 16),
         ("level_nodes", ct.c_int32 * 16),

@@ -84,7 +84,16 @@ struct wf_handle {
   WfIncDev* inc_host = nullptr;
   int inc_host_cap = 0;
   hipEvent_t inc_evt = nullptr;
-  hipEvent_t ev[8] = {};
+  // per-launch timing: events from a reusable pool, summed lazily
+  std::vector<hipEvent_t> evpool;
+  int ev_used = 0;
+  struct Rec {
+    int kind, e0, e1;
+    double work;
+  };
+  std::vector<Rec> recs;
+  int ev_t0 = -1, ev_t1 = -1;
+  bool stats_final = true;
   wf_stats stats;
 };
 
@@ -289,6 +298,57 @@ wf_status set_device(wf_handle* h) {
   return WF_OK;
 }
 
+// ---------------------------------------------------------------- timing --
+wf_status prof_event(wf_handle* h, hipStream_t st, int* idx) {
+  if (h->ev_used == (int)h->evpool.size()) {
+    hipEvent_t e;
+    WF_HIP(hipEventCreate(&e));
+    h->evpool.push_back(e);
+  }
+  *idx = h->ev_used++;
+  WF_HIP(hipEventRecord(h->evpool[*idx], st));
+  return WF_OK;
+}
+
+void prof_reset(wf_handle* h) {
+  h->ev_used = 0;
+  h->recs.clear();
+  h->ev_t0 = h->ev_t1 = -1;
+  memset(&h->stats, 0, sizeof h->stats);
+  h->stats_final = false;
+}
+
+// Runs `launch` (returning hipError_t) between two events and books it.
+template <typename Fn>
+wf_status timed(wf_handle* h, hipStream_t st, int kind, double work, Fn&& launch) {
+  int e0, e1;
+  WF_TRY(prof_event(h, st, &e0));
+  hipError_t e = launch();
+  if (e != hipSuccess) return set_err(WF_ERR_HIP, "kernel launch (kind %d) failed: %s", kind, hipGetErrorString(e));
+  WF_TRY(prof_event(h, st, &e1));
+  h->recs.push_back({kind, e0, e1, work});
+  return WF_OK;
+}
+
+wf_status prof_finalize(wf_handle* h) {
+  if (h->stats_final) return WF_OK;
+  if (h->ev_used > 0) WF_HIP(hipEventSynchronize(h->evpool[h->ev_used - 1]));
+  for (const auto& r : h->recs) {
+    float ms = 0;
+    WF_HIP(hipEventElapsedTime(&ms, h->evpool[r.e0], h->evpool[r.e1]));
+    h->stats.kern_ms[r.kind] += ms;
+    h->stats.kern_launches[r.kind] += 1;
+    h->stats.kern_work[r.kind] += r.work;
+  }
+  if (h->ev_t0 >= 0 && h->ev_t1 >= 0) {
+    float ms = 0;
+    WF_HIP(hipEventElapsedTime(&ms, h->evpool[h->ev_t0], h->evpool[h->ev_t1]));
+    h->stats.ms_total = ms;
+  }
+  h->stats_final = true;
+  return WF_OK;
+}
+
 }  // namespace
 
 // ============================================================== C-ABI =====
@@ -303,7 +363,7 @@ void wf_default_opts(wf_opts* o) {
   memset(o, 0, sizeof *o);
   o->struct_size = (int32_t)sizeof(wf_opts);
   o->device = -1;
-  o->precision = WF_PREC_FP32;
+  o->precision = WF_PREC_BF16X3;
   o->strict_grid = 0;
   o->beta = 4.0;
   o->box_enlarge = 1.1;
@@ -328,7 +388,7 @@ wf_status wf_create(wf_handle** out, const wf_opts* opts) {
   if (!(o.beta >= 0 && o.beta < 100)) return set_err(WF_ERR_INVALID_ARG, "opts.beta out of range");
   if (!(o.box_enlarge >= 1.0 && o.box_enlarge < 100))
     return set_err(WF_ERR_INVALID_ARG, "opts.box_enlarge must be >= 1");
-  if (o.precision != WF_PREC_FP32 && o.precision != WF_PREC_BF16X3)
+  if (o.precision != WF_PREC_FP32 && o.precision != WF_PREC_BF16X3 && o.precision != WF_PREC_FP32_VALU)
     return set_err(WF_ERR_INVALID_ARG, "opts.precision unknown");
   if (o.has_box) {
     for (int q = 0; q < 3; ++q)
@@ -357,7 +417,6 @@ wf_status wf_create(wf_handle** out, const wf_opts* opts) {
     delete h;
     return set_err(WF_ERR_HIP, "hipEventCreate failed");
   }
-  for (auto& ev : h->ev) hipEventCreate(&ev);
   memset(&h->stats, 0, sizeof h->stats);
   *out = h;
   return WF_OK;
@@ -373,8 +432,7 @@ void wf_destroy(wf_handle* h) {
     if (kv.second.tw) hipFree(kv.second.tw);
   if (h->inc_host) hipHostFree(h->inc_host);
   if (h->inc_evt) hipEventDestroy(h->inc_evt);
-  for (auto& ev : h->ev)
-    if (ev) hipEventDestroy(ev);
+  for (auto ev : h->evpool) hipEventDestroy(ev);
   delete h;
 }
 
@@ -586,7 +644,7 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
   const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
                             0.5 * (h->box[2] + h->box[5])};
   const float* px = h->pts;
-  // group solves sharing lambda (phase shared), at most 4 per launch (F <= 24)
+  // solves sharing a wavelength share the phase; at most 4 per launch (F <= 24)
   std::vector<int> order(n_inc);
   for (int i = 0; i < n_inc; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
@@ -594,35 +652,39 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
   double2 *C = nullptr, *F0 = nullptr;
   WF_TRY(dalloc_t(h, "direct_C", (size_t)n * 24, &C));
   WF_TRY(dalloc_t(h, "direct_F0", (size_t)D * 24, &F0));
-  WF_HIP(hipEventRecord(h->ev[0], st));
+  prof_reset(h);
+  WF_TRY(prof_event(h, st, &h->ev_t0));
   for (int i0 = 0; i0 < n_inc;) {
     int i1 = i0 + 1;
     while (i1 < n_inc && i1 - i0 < 4 && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
     const int S = i1 - i0;
     if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
     for (int q = 0; q < S; ++q)
-      WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                 px + 6 * n, n, dinc + order[i0 + q], 1, 1, C + 6 * q, 6 * S, 0, st));
-    WF_HIP(wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, 6 * S, centre,
-                                  2.0 * WF_PI / inc[order[i0]].lambda_m, grid.n_theta, grid.n_phi,
-                                  F0, st));
+      WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 96.0) * n, [&] {
+        return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                   px + 6 * n, n, dinc + order[i0 + q], 1, 1, C + 6 * q, 6 * S, 0, st);
+      }));
+    WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
+      return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, 6 * S, centre,
+                                    2.0 * WF_PI / inc[order[i0]].lambda_m, grid.n_theta,
+                                    grid.n_phi, F0, st);
+    }));
     for (int q = 0; q < S; ++q) {
       const int slot = order[i0 + q];
-      WF_HIP(wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, grid.n_theta, grid.n_phi, nullptr,
-                                  nullptr, I_dev + (int64_t)slot * D,
-                                  E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, st));
+      WF_TRY(timed(h, st, WF_K_EPILOGUE, (96.0 + 8.0 + (E_dev ? 48.0 : 0.0)) * D, [&] {
+        return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, grid.n_theta, grid.n_phi, nullptr,
+                                    nullptr, I_dev + (int64_t)slot * D,
+                                    E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, st);
+      }));
     }
     i0 = i1;
   }
-  WF_HIP(hipEventRecord(h->ev[1], st));
+  WF_TRY(prof_event(h, st, &h->ev_t1));
   if (s->memspace == WF_MEM_HOST) {
     WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(double) * n_inc * D, hipMemcpyDeviceToHost, st));
     if (E_cplx)
       WF_HIP(hipMemcpyAsync(E_cplx, E_dev, sizeof(double) * n_inc * 6 * D, hipMemcpyDeviceToHost, st));
     WF_HIP(hipStreamSynchronize(st));
-    float ms = 0;
-    hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
-    h->stats.ms_total = ms;
   }
   return WF_OK;
 }
@@ -645,10 +707,9 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
   if (depth > 0) {
     if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
     if (!is_smooth(NP) || !is_smooth(2 * NT))
-      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth", NT, NP);
+      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
+                     NT, NP);
   }
-  memset(&h->stats, 0, sizeof h->stats);
-  h->stats.depth = depth;
   WfIncDev* dinc = nullptr;
   WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
   float* I_dev = intensity;
@@ -664,10 +725,12 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
                    [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
   const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
                             0.5 * (h->box[2] + h->box[5])};
-  float t_cur = 0, t_leaf = 0, t_agg = 0, t_fin = 0;
-  WF_HIP(hipEventRecord(h->ev[0], st));
+  prof_reset(h);
+  h->stats.depth = depth;
+  WF_TRY(prof_event(h, st, &h->ev_t0));
   int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6) : 8;
   if (depth == 0) max_solves = std::min(max_solves, 4);
+  const double epi_bytes = (96.0 + 4.0 + (E_dev ? 24.0 : 0.0)) * D;
   std::vector<Grid> g(depth + 1);
   for (int i0 = 0; i0 < n_inc;) {
     // wavelength group
@@ -696,7 +759,9 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
         const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
                               0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
                               0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
-        WF_HIP(wf::launch_trans_table(T + toff[l], g[l].nt, g[l].np, th0, hc, k0, st));
+        WF_TRY(timed(h, st, WF_K_TRANS, 64.0 * g[l].nt * g[l].np, [&] {
+          return wf::launch_trans_table(T + toff[l], g[l].nt, g[l].np, th0, hc, k0, st);
+        }));
       }
       for (int l = 0; l <= depth && l < 16; ++l) {
         h->stats.level_nt[l] = g[l].nt;
@@ -714,25 +779,31 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
         WF_TRY(dalloc_t(h, "direct_C", (size_t)n * F, &C));
         WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
         for (int q = 0; q < S; ++q)
-          WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                     px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st));
-        WF_HIP(wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0, st));
+          WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 96.0) * n, [&] {
+            return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                       px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st);
+          }));
+        WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
+          return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0, st);
+        }));
         for (int q = 0; q < S; ++q) {
           const int slot = order[c0 + q];
-          WF_HIP(wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
-                                      E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
-                                      nullptr, st));
+          WF_TRY(timed(h, st, WF_K_EPILOGUE, epi_bytes, [&] {
+            return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
+                                        E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
+                                        nullptr, st);
+          }));
         }
         continue;
       }
-      // ---- currents (sorted order), [n][F]
+      // ---- currents (sorted order), [n][F] complex64
       float2* C = nullptr;
       WF_TRY(dalloc_t(h, "C", (size_t)n * F, &C));
-      WF_HIP(hipEventRecord(h->ev[2], st));
       for (int q = 0; q < S; ++q)
-        WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                   px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F, 0, st));
-      WF_HIP(hipEventRecord(h->ev[3], st));
+        WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n, [&] {
+          return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                     px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F, 0, st);
+        }));
       // ---- buffers: patterns per level (ping-pong) and phi-pass scratch
       size_t pat_max = 0, g_max = 0;
       for (int l = 1; l <= depth; ++l)
@@ -771,9 +842,12 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
       float2* cur = (depth % 2) ? patA : patB;
       la.out = cur;
       la.precision = h->opts.precision;
-      WF_HIP(wf::launch_leaf(la, st));
-      h->stats.leaf_pairs += (int64_t)n * g[depth].nt * g[depth].np;
-      WF_HIP(hipEventRecord(h->ev[4], st));
+      const double dleaf = (double)g[depth].nt * g[depth].np;
+      WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F, [&] {
+        if (h->opts.precision == WF_PREC_FP32_VALU) return wf::launch_leaf(la, st);
+        return wf::launch_leaf_mfma(la, h->opts.precision == WF_PREC_BF16X3, st);
+      }));
+      h->stats.leaf_pairs += (int64_t)(dleaf * n);
       // ---- aggregation levels depth-1 .. 1
       for (int l = depth - 1; l >= 1; --l) {
         if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
@@ -784,19 +858,22 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
         WF_TRY(get_fft(h, 2 * gc.nt, &tc));
         WF_TRY(get_fft(h, 2 * gp.nt, &tp));
         const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
-        WF_HIP(wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gc.nt, gc.np, gp.np, pc, pp, st));
+        const double planes = (double)nchild * F;
+        WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gc.nt * ((double)gc.np + gp.np), [&] {
+          return wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gc.nt, gc.np, gp.np, pc, pp, st);
+        }));
         float2* nxt = (cur == patA) ? patB : patA;
         const double off_c = -WF_PI / 2 + WF_PI / (2.0 * gc.nt);
         const double off_p = -WF_PI / 2 + WF_PI / (2.0 * gp.nt);
         const int mirror = (int)std::lround(gp.nt * (1.0 - 2.0 * off_p / WF_PI));
-        WF_HIP(wf::launch_theta_pass(G, F, gc.nt, gp.np, gp.nt, npar, h->levels[l].cb,
-                                     h->levels[l].ce, h->levels[l + 1].key, T + toff[l],
-                                     off_p - off_c, mirror, tc, tp, nxt, nullptr, st));
-        h->stats.agg_bytes += (int64_t)8 * F *
-                              ((int64_t)nchild * gc.nt * gc.np + (int64_t)npar * gp.nt * gp.np);
+        WF_TRY(timed(h, st, WF_K_THETA,
+                     8.0 * F * ((double)nchild * gc.nt * gp.np + (double)npar * gp.nt * gp.np), [&] {
+          return wf::launch_theta_pass(G, F, gc.nt, gp.np, gp.nt, npar, h->levels[l].cb,
+                                       h->levels[l].ce, h->levels[l + 1].key, T + toff[l],
+                                       off_p - off_c, mirror, tc, tp, nxt, nullptr, st);
+        }));
         cur = nxt;
       }
-      WF_HIP(hipEventRecord(h->ev[5], st));
       // ---- root: level 1 -> output grid, FP64 accumulation
       {
         const Grid gc = g[1];
@@ -808,63 +885,51 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
         const int nchild = h->levels[1].count;
         float2* G0 = nullptr;
         WF_TRY(dalloc_t(h, "G0", (size_t)nchild * F * gc.nt * NP, &G0));
-        WF_HIP(wf::launch_phi_pass(cur, G0, (int64_t)nchild * F, gc.nt, gc.np, NP, pc, pp, st));
+        const double planes = (double)nchild * F;
+        WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gc.nt * ((double)gc.np + NP), [&] {
+          return wf::launch_phi_pass(cur, G0, (int64_t)nchild * F, gc.nt, gc.np, NP, pc, pp, st);
+        }));
         const double off_c = -WF_PI / 2 + WF_PI / (2.0 * gc.nt);
         const double off_p = -WF_PI / 2 + 1e-10;
         const int mirror = (int)std::lround(NT * (1.0 - 2.0 * off_p / WF_PI));
-        WF_HIP(wf::launch_theta_pass(G0, F, gc.nt, NP, NT, 1, h->levels[0].cb, h->levels[0].ce,
-                                     h->levels[1].key, T + toff[0], off_p - off_c, mirror, tc, tp,
-                                     nullptr, F0, st));
+        WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gc.nt * NP + 16.0 * F * D, [&] {
+          return wf::launch_theta_pass(G0, F, gc.nt, NP, NT, 1, h->levels[0].cb, h->levels[0].ce,
+                                       h->levels[1].key, T + toff[0], off_p - off_c, mirror, tc, tp,
+                                       nullptr, F0, st);
+        }));
         for (int q = 0; q < S; ++q) {
           const int slot = order[c0 + q];
-          WF_HIP(wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
-                                      E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
-                                      nullptr, st));
+          WF_TRY(timed(h, st, WF_K_EPILOGUE, epi_bytes, [&] {
+            return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
+                                        E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
+                                        nullptr, st);
+          }));
         }
-      }
-      WF_HIP(hipEventRecord(h->ev[6], st));
-      // stage timings are only read on host-synchronous calls
-      if (out_memspace == WF_MEM_HOST) {
-        WF_HIP(hipEventSynchronize(h->ev[6]));
-        float a = 0, b = 0, c = 0, d = 0;
-        hipEventElapsedTime(&a, h->ev[2], h->ev[3]);
-        hipEventElapsedTime(&b, h->ev[3], h->ev[4]);
-        hipEventElapsedTime(&c, h->ev[4], h->ev[5]);
-        hipEventElapsedTime(&d, h->ev[5], h->ev[6]);
-        t_cur += a;
-        t_leaf += b;
-        t_agg += c;
-        t_fin += d;
       }
     }
     i0 = i1;
   }
-  WF_HIP(hipEventRecord(h->ev[1], st));
+  WF_TRY(prof_event(h, st, &h->ev_t1));
   if (out_memspace == WF_MEM_HOST) {
     WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
     if (E_cplx)
       WF_HIP(hipMemcpyAsync(E_cplx, E_dev, sizeof(float) * n_inc * 6 * D, hipMemcpyDeviceToHost, st));
     WF_HIP(hipStreamSynchronize(st));
-    float ms = 0;
-    hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
-    h->stats.ms_total = ms;
-    h->stats.ms_currents = t_cur;
-    h->stats.ms_leaf = t_leaf;
-    h->stats.ms_aggregate = t_agg;
-    h->stats.ms_final = t_fin;
   }
-  (void)centre;
   return WF_OK;
 }
 
 wf_status wf_po_solve(wf_handle* h, const wf_surface* s, const wf_incidence* inc, int32_t n_inc,
                       wf_grid grid, int32_t depth, float* intensity, float* E_cplx, void* stream) {
+  if (!s) return set_err(WF_ERR_INVALID_ARG, "surface is null");
   WF_TRY(wf_set_geometry(h, s, depth, stream));
   return wf_solve(h, inc, n_inc, grid, intensity, E_cplx, s->memspace, stream);
 }
 
 wf_status wf_get_stats(wf_handle* h, wf_stats* out) {
   if (!h || !out) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  WF_TRY(set_device(h));
+  WF_TRY(prof_finalize(h));
   *out = h->stats;
   return WF_OK;
 }

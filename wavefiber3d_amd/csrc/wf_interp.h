@@ -47,5 +47,7 @@ struct LeafArgs {
   int precision;
 };
 hipError_t launch_leaf(const LeafArgs& a, hipStream_t st);
+// MFMA variants (F <= 48 fields per launch): split BF16 x3 or FP32 MFMA.
+hipError_t launch_leaf_mfma(const LeafArgs& a, bool bf16, hipStream_t st);
 
 }  // namespace wf

@@ -1,0 +1,305 @@
+// wf_leaf_mfma.hip -- leaf radiation sum on the MFMA units (row A10).
+//
+// Per leaf L the pattern is the complex GEMM Y[d][f] = sum_n P[d][n] C[n][f],
+// P[d][n] = exp(+i k s_d.(x_n - c_L)) (util/fields.h:8-38 summed per leaf as
+// in treePO.cu:652-670).  Realified and transposed so that output lanes run
+// along directions (coalesced stores):
+//
+//   Y^T[c][d] = sum_k Bt[c][k] Pt[k][d],   k = 2n + {re, im}, c = 2f + {re, im}
+//   Pt[2n][d] = cos, Pt[2n+1][d] = sin of k s_d.(x_n - c_L)
+//   Bt[2f][2n] = Re C, Bt[2f][2n+1] = -Im C, Bt[2f+1][2n] = Im C, Bt[2f+1][2n+1] = Re C
+//
+// MFMA A operand = Bt (currents of 16 points x 8 fields, staged once per leaf
+// chunk in LDS in the per-lane register layout), MFMA B operand = Pt (phases
+// generated in registers: each lane owns one direction and four points),
+// accumulated in FP32.
+//   * BF16X3: v_mfma_f32_16x16x32_bf16 with hi/lo splits of both operands,
+//     Y += Ahi Bhi + Ahi Blo + Alo Bhi (Henry, Tang & Heinecke, ARITH 2019);
+//     relative error ~1e-5 per product sum.
+//   * FP32:   v_mfma_f32_16x16x4_f32.
+// Operand lane layouts (gfx950, wave64):
+//   16x16x32 bf16: A lane l -> row l%16, k = 8(l/16)+j; B lane l -> col l%16,
+//                  k = 8(l/16)+j; D lane l -> col l%16, rows 4(l/16)+j.
+//   16x16x4 f32:   A lane l -> row l%16, k = l/16; B lane l -> col l%16,
+//                  k = l/16; D as above.
+// Phases: local coordinates scaled to revolutions, reduced by rint, then the
+// hardware sin/cos (inputs in revolutions).
+#include "wf_common.h"
+#include "wf_interp.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+constexpr int kThreads = 256;  // 4 waves
+
+__device__ inline void leaf_centre(uint32_t key, int depth, const double lo[3],
+                                   const double size[3], double c[3]) {
+  int ix = 0, iy = 0, iz = 0;
+  for (int l = depth - 1; l >= 0; --l) {
+    const uint32_t o = (key >> (3 * l)) & 7u;
+    ix = 2 * ix + (int)(o & 1u);
+    iy = 2 * iy + (int)((o & 2u) == 0);
+    iz = 2 * iz + (int)((o & 4u) == 0);
+  }
+  c[0] = lo[0] + (ix + 0.5) * size[0];
+  c[1] = lo[1] + (iy + 0.5) * size[1];
+  c[2] = lo[2] + (iz + 0.5) * size[2];
+}
+
+__device__ inline void leaf_dir(int d, int nt, int np, float s[3]) {
+  const int j = d / np, kk = d - j * np;
+  const double th = -WF_PI / 2 + WF_PI * (j + 0.5) / nt;
+  const double ph = WF_TWO_PI * kk / np;
+  double st, ct, sp, cp;
+  sincos(th, &st, &ct);
+  sincos(ph, &sp, &cp);
+  s[0] = (float)(ct * cp);
+  s[1] = (float)(ct * sp);
+  s[2] = (float)st;
+}
+
+__device__ inline float rev_reduce(float r) { return r - rintf(r); }
+
+// Stage the scaled local coordinates of one chunk of points (zero-padded).
+__device__ inline void stage_points(const wf::LeafArgs& a, int base, int cnt, int npts,
+                                    const double c[3], double k_rev, float4* spt) {
+  for (int t = threadIdx.x; t < npts; t += kThreads) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < cnt) {
+      const int i = base + t;
+      v.x = (float)(((double)a.px[i] - c[0]) * k_rev);
+      v.y = (float)(((double)a.py[i] - c[1]) * k_rev);
+      v.z = (float)(((double)a.pz[i] - c[2]) * k_rev);
+    }
+    spt[t] = v;
+  }
+}
+
+// ---------------------------------------------------------------- BF16 x3 --
+template <int MT, int R>
+__global__ __launch_bounds__(kThreads) void leaf_bf16x3_kernel(wf::LeafArgs a, int KS) {
+  extern __shared__ float4 smem4[];
+  float4* spt = smem4;                                          // KS*16 points
+  bf16x8* scur = reinterpret_cast<bf16x8*>(smem4 + KS * 16);    // [ks][mt][hi,lo][64]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+  const int D = a.nt * a.np;
+  const int dtiles = (D + 15) / 16;
+  const int per_block = 4 * R;
+  const int nsuper = (dtiles + per_block - 1) / per_block;
+  const int leaf = (int)(blockIdx.x / nsuper), sup = (int)(blockIdx.x % nsuper);
+  const int b = a.leaf_begin[leaf], e = a.leaf_end[leaf];
+  double c[3];
+  leaf_centre(a.leaf_key[leaf], a.depth, a.box_lo, a.leaf_size, c);
+  const double k_rev = a.k0 / WF_TWO_PI;
+  float s[R][3];
+  int dd[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    dd[r] = (sup * per_block + wave * R + r) * 16 + col;
+    s[r][0] = s[r][1] = s[r][2] = 0.f;
+    if (dd[r] < D) leaf_dir(dd[r], a.nt, a.np, s[r]);
+  }
+  floatx4 acc[R][MT];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[r][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int chunk = KS * 16;
+  for (int base = b; base < e; base += chunk) {
+    const int cnt = min(chunk, e - base);
+    const int ksteps = (cnt + 15) / 16;
+    __syncthreads();
+    stage_points(a, base, cnt, ksteps * 16, c, k_rev, spt);
+    // currents in the MFMA A-operand lane layout, hi and lo parts
+    for (int idx = threadIdx.x; idx < ksteps * MT * 64; idx += kThreads) {
+      const int ln = idx & 63, mt = (idx >> 6) % MT, ks = idx / (64 * MT);
+      const int cc = ln & 15, gg = ln >> 4;
+      const int f = mt * 8 + (cc >> 1), part = cc & 1;
+      bf16x8 hi, lo;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int p = ks * 16 + 4 * gg + q;
+        float2 C = make_float2(0.f, 0.f);
+        if (p < cnt && f < a.F) C = a.C[(int64_t)(base + p) * a.Ftot + a.f0 + f];
+        const float vr = part ? C.y : C.x;   // row 2n   (multiplies cos)
+        const float vi = part ? C.x : -C.y;  // row 2n+1 (multiplies sin)
+        const __bf16 hr = (__bf16)vr, hi_ = (__bf16)vi;
+        hi[2 * q] = hr;
+        hi[2 * q + 1] = hi_;
+        lo[2 * q] = (__bf16)(vr - (float)hr);
+        lo[2 * q + 1] = (__bf16)(vi - (float)hi_);
+      }
+      scur[(ks * MT + mt) * 128 + ln] = hi;
+      scur[(ks * MT + mt) * 128 + 64 + ln] = lo;
+    }
+    __syncthreads();
+    for (int ks = 0; ks < ksteps; ++ks) {
+      float4 pt[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pt[q] = spt[ks * 16 + 4 * g + q];
+      bf16x8 ahi[MT], alo[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        ahi[m] = scur[(ks * MT + m) * 128 + lane];
+        alo[m] = scur[(ks * MT + m) * 128 + 64 + lane];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        bf16x8 phi, plo;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float f = rev_reduce(s[r][0] * pt[q].x + s[r][1] * pt[q].y + s[r][2] * pt[q].z);
+          const float cs = __builtin_amdgcn_cosf(f), sn = __builtin_amdgcn_sinf(f);
+          const __bf16 hc = (__bf16)cs, hs = (__bf16)sn;
+          phi[2 * q] = hc;
+          phi[2 * q + 1] = hs;
+          plo[2 * q] = (__bf16)(cs - (float)hc);
+          plo[2 * q + 1] = (__bf16)(sn - (float)hs);
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          acc[r][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[m], phi, acc[r][m], 0, 0, 0);
+          acc[r][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[m], plo, acc[r][m], 0, 0, 0);
+          acc[r][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[m], phi, acc[r][m], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D lane l: column = direction dd, rows 4g + j -> fields mt*8 + 2g + j/2, part j%2
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (dd[r] >= D) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int f = m * 8 + 2 * g + h;
+        if (f < a.F)
+          a.out[((int64_t)leaf * a.F + f) * D + dd[r]] =
+              make_float2(acc[r][m][2 * h], acc[r][m][2 * h + 1]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- FP32 --
+template <int MT, int R>
+__global__ __launch_bounds__(kThreads) void leaf_f32_kernel(wf::LeafArgs a, int KS) {
+  extern __shared__ float4 smem4[];
+  float4* spt = smem4;                                      // KS*16 points
+  float* scur = reinterpret_cast<float*>(smem4 + KS * 16);  // [kstep(2 pts)][mt][64]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+  const int D = a.nt * a.np;
+  const int dtiles = (D + 15) / 16;
+  const int per_block = 4 * R;
+  const int nsuper = (dtiles + per_block - 1) / per_block;
+  const int leaf = (int)(blockIdx.x / nsuper), sup = (int)(blockIdx.x % nsuper);
+  const int b = a.leaf_begin[leaf], e = a.leaf_end[leaf];
+  double c[3];
+  leaf_centre(a.leaf_key[leaf], a.depth, a.box_lo, a.leaf_size, c);
+  const double k_rev = a.k0 / WF_TWO_PI;
+  float s[R][3];
+  int dd[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    dd[r] = (sup * per_block + wave * R + r) * 16 + col;
+    s[r][0] = s[r][1] = s[r][2] = 0.f;
+    if (dd[r] < D) leaf_dir(dd[r], a.nt, a.np, s[r]);
+  }
+  floatx4 acc[R][MT];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[r][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // lane's k = g: point 2*step + g/2, component g%2 (0: cos row, 1: sin row)
+  const float quarter = (g & 1) ? 0.f : 0.25f;  // sin(2 pi (f + 1/4)) = cos(2 pi f)
+  const int chunk = KS * 16;
+  for (int base = b; base < e; base += chunk) {
+    const int cnt = min(chunk, e - base);
+    const int ksteps = (cnt + 15) / 16;
+    const int nsub = ksteps * 8;  // MFMA k-steps of two points
+    __syncthreads();
+    stage_points(a, base, cnt, ksteps * 16, c, k_rev, spt);
+    for (int idx = threadIdx.x; idx < nsub * MT * 64; idx += kThreads) {
+      const int ln = idx & 63, mt = (idx >> 6) % MT, st = idx / (64 * MT);
+      const int cc = ln & 15, gg = ln >> 4;
+      const int f = mt * 8 + (cc >> 1), part = cc & 1;
+      const int p = 2 * st + (gg >> 1);
+      float2 C = make_float2(0.f, 0.f);
+      if (p < cnt && f < a.F) C = a.C[(int64_t)(base + p) * a.Ftot + a.f0 + f];
+      // Bt[c][k]: k even (cos row) -> part ? Im : Re ; k odd (sin row) -> part ? Re : -Im
+      scur[(st * MT + mt) * 64 + ln] = (gg & 1) ? (part ? C.x : -C.y) : (part ? C.y : C.x);
+    }
+    __syncthreads();
+    for (int st = 0; st < nsub; ++st) {
+      const float4 pt = spt[2 * st + (g >> 1)];
+      float am[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) am[m] = scur[(st * MT + m) * 64 + lane];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float f = rev_reduce(s[r][0] * pt.x + s[r][1] * pt.y + s[r][2] * pt.z) + quarter;
+        const float ph = __builtin_amdgcn_sinf(f);
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          acc[r][m] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[m], ph, acc[r][m], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (dd[r] >= D) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int f = m * 8 + 2 * g + h;
+        if (f < a.F)
+          a.out[((int64_t)leaf * a.F + f) * D + dd[r]] =
+              make_float2(acc[r][m][2 * h], acc[r][m][2 * h + 1]);
+      }
+    }
+  }
+}
+
+template <int MT>
+hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
+  constexpr int R = (MT <= 2) ? 4 : (MT <= 4 ? 2 : 1);
+  const int D = a.nt * a.np;
+  const int dtiles = (D + 15) / 16;
+  const int nsuper = (dtiles + 4 * R - 1) / (4 * R);
+  const int64_t nblk = (int64_t)nsuper * a.nleaves;
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  const size_t per_ks = bf16 ? (size_t)MT * 128 * sizeof(bf16x8) : (size_t)8 * MT * 64 * sizeof(float);
+  int KS = (int)((48 * 1024) / per_ks);
+  KS = KS < 1 ? 1 : (KS > 8 ? 8 : KS);
+  const size_t sh = (size_t)KS * 16 * sizeof(float4) + (size_t)KS * per_ks;
+  if (bf16)
+    hipLaunchKernelGGL((leaf_bf16x3_kernel<MT, R>), dim3((unsigned)nblk), dim3(kThreads), sh, st, a, KS);
+  else
+    hipLaunchKernelGGL((leaf_f32_kernel<MT, R>), dim3((unsigned)nblk), dim3(kThreads), sh, st, a, KS);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace wf {
+
+// F complex fields handled per launch: up to 48 (MT = 6 tiles of 8 fields).
+hipError_t launch_leaf_mfma(const LeafArgs& a, bool bf16, hipStream_t st) {
+  if (a.nleaves <= 0) return hipSuccess;
+  const int MT = (a.F + 7) / 8;
+  switch (MT) {
+    case 1: return launch_mt<1>(a, bf16, st);
+    case 2: return launch_mt<2>(a, bf16, st);
+    case 3: return launch_mt<3>(a, bf16, st);
+    case 4: return launch_mt<4>(a, bf16, st);
+    case 5: return launch_mt<5>(a, bf16, st);
+    case 6: return launch_mt<6>(a, bf16, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace wf

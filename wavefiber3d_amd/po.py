@@ -98,7 +98,7 @@ def _stream_ptr(arrays) -> int | None:
 class POSolver:
     """Handle over the C-ABI (one per host thread)."""
 
-    def __init__(self, device: int = -1, precision: str = "fp32", beta: float = 4.0,
+    def __init__(self, device: int = -1, precision: str = "bf16x3", beta: float = 4.0,
                  box=None, box_enlarge: float = 1.1, lambda_sample_m: float = 0.0,
                  strict_grid: bool = False, workspace_cap_bytes: int = 0,
                  field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None):
@@ -106,7 +106,8 @@ class POSolver:
         o = _lib.Opts()
         L.wf_default_opts(ct.byref(o))
         o.device = device
-        o.precision = {"fp32": _lib.WF_PREC_FP32, "bf16x3": _lib.WF_PREC_BF16X3}[precision]
+        o.precision = {"fp32": _lib.WF_PREC_FP32, "bf16x3": _lib.WF_PREC_BF16X3,
+                       "fp32_valu": _lib.WF_PREC_FP32_VALU}[precision]
         o.beta = beta
         o.box_enlarge = box_enlarge
         o.lambda_sample_m = lambda_sample_m
@@ -267,12 +268,16 @@ class POSolver:
         return out
 
     def stats(self) -> dict:
+        """Per-kernel HIP-event timings of the last solve (waits for its events)."""
         s = _lib.Stats()
         check(lib().wf_get_stats(self._h, ct.byref(s)))
         nl = s.n_levels
+        kern = {}
+        for i, name in enumerate(_lib.KERNEL_KINDS):
+            if s.kern_launches[i]:
+                kern[name] = {"ms": s.kern_ms[i], "launches": s.kern_launches[i],
+                              "work": s.kern_work[i]}
         return {
-            "ms_currents": s.ms_currents, "ms_leaf": s.ms_leaf, "ms_aggregate": s.ms_aggregate,
-            "ms_final": s.ms_final, "ms_total": s.ms_total, "leaf_pairs": s.leaf_pairs,
-            "agg_bytes": s.agg_bytes, "depth": s.depth,
+            "ms_total": s.ms_total, "kernels": kern, "leaf_pairs": s.leaf_pairs, "depth": s.depth,
             "levels": [(s.level_nt[i], s.level_np[i], s.level_nodes[i]) for i in range(nl)],
         }
