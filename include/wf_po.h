@@ -189,6 +189,21 @@ wf_status wf_octree_nodes(wf_handle* h, int32_t* perm, int32_t* node_begin, int3
 wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const float* child,
                     int32_t ntp, int32_t npp, double off_tp, float* parent, void* stream);
 
+/* Reverse mode (no reference counterpart; for callers that fit geometry to
+ * measured patterns).  Vector-Jacobian product of the solve on the current
+ * geometry: given grad_I = dL/dI [n_inc][Nt][Np] (float32), returns
+ * grad_x = dL/dx_n and grad_n = dL/dn_n ([3][n] each) and grad_a = dL/da_n
+ * ([n]), all in the caller's original point order (any may be null), and the
+ * forward intensity (nullable).  The forward pass is recomputed; the reverse
+ * pass needs no stored per-level fields.  fp64 = 1 (or depth 0) runs the FP64
+ * direct sum forward and backward (the reference gradient); otherwise the
+ * tree and its exact adjoint.  Normal derivatives of the Fresnel currents use
+ * FP64 central differences (h = 1e-6); they are one-sided quantities at the
+ * lit/shadow boundary and at exact normal incidence. */
+wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                    const float* grad_I, float* intensity, float* grad_x, float* grad_n,
+                    float* grad_a, int32_t fp64, int32_t memspace, void* stream);
+
 /* Statistics of the last wf_solve on this handle. */
 wf_status wf_get_stats(wf_handle* h, wf_stats* out);
 

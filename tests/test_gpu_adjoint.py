@@ -1,0 +1,143 @@
+"""Reverse mode (wf_po_vjp) parity on the GPU.
+
+* FP64 direct-sum gradient vs central finite differences of the FP64 forward
+  (h = 1e-4 um); the FD quotient is compared with the gradient contracted
+  with the displacement actually applied after float32 rounding of the
+  coordinates, so the step is exact.
+* Tree gradient (FP32 leaf, exact adjoint of the tree) vs the FP64 gradient.
+* Cuticle-tilt gradient through the differentiable fiber generator vs a
+  Richardson-extrapolated central difference of the FP64 forward.
+Gate: 1e-4 relative for gradient-vs-FD, 1e-3 for tree-vs-FP64 gradients.
+"""
+import numpy as np
+import pytest
+
+import wavefiber3d_amd as wf
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from wavefiber3d_amd.autograd import fiber_surface_torch, log_intensity_loss, po_intensity  # noqa: E402
+
+INCS = [wf.Incidence(np.pi / 2 - 0.3, 0.4, 0.5e-6, "M", 1.55 + 0.02j),
+        wf.Incidence(0.2, 1.1, 0.5e-6, "E", 1.55 + 0.02j)]
+GRID = (24, 48)
+
+
+def _loss64(solver, surf, target):
+    I = solver.solve_fp64_direct(surf, INCS, GRID)  # float64
+    return float(((torch.log(I) - torch.log(target.double())) ** 2).sum().item())
+
+
+def _setup(n=3000, seed=0):
+    s = wf.fibonacci_sphere(n, 1.5e-6)
+    dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
+    rng = np.random.default_rng(seed)
+    with wf.POSolver(device=0) as sv:
+        I0 = sv.solve_fp64_direct(dev, INCS, GRID)
+    target = (I0 * torch.from_numpy(np.exp(0.3 * rng.standard_normal(I0.shape))).cuda()).float()
+    return s, dev, target
+
+
+def test_fp64_vjp_matches_central_fd():
+    s, dev, target = _setup()
+    print("case: fp64 vjp vs FD n=3000", flush=True)
+    with wf.POSolver(device=0) as sv:
+        x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(dev)]
+        I = po_intensity(x, INCS, GRID, 0, sv, fp64=True)
+        L = log_intensity_loss(I, target)
+        L.backward()
+        gx = torch.stack([x[0].grad, x[1].grad, x[2].grad]).double()
+        rng = np.random.default_rng(5)
+        for trial in range(3):
+            v = torch.from_numpy(rng.standard_normal((3, s.n))).cuda()
+            h = 1e-10  # 1e-4 um
+            xp = [(dev[c].double() + h * v[c]).float() for c in range(3)]
+            xm = [(dev[c].double() - h * v[c]).float() for c in range(3)]
+            Lp = _loss64(sv, xp + dev[3:], target)
+            Lm = _loss64(sv, xm + dev[3:], target)
+            dx = torch.stack([xp[c].double() - xm[c].double() for c in range(3)])
+            pred = float((gx * dx).sum().item())
+            err = abs((Lp - Lm) - pred) / abs(Lp - Lm)
+            print(f"  trial {trial}: FD {Lp - Lm:.6e} adjoint {pred:.6e} rel {err:.2e}", flush=True)
+            assert err < 1e-4
+
+
+def test_normal_displacement_gradient():
+    """d L / d delta_n for x -> x + delta n (normals, areas fixed) = n . dL/dx."""
+    s, dev, target = _setup(2000, 1)
+    print("case: delta gradient n=2000", flush=True)
+    with wf.POSolver(device=0) as sv:
+        delta = torch.zeros(s.n, dtype=torch.float64, device="cuda", requires_grad=True)
+        nrm = torch.stack(dev[3:6]).double()
+        pos = torch.stack(dev[:3]).double() + delta * nrm
+        surf = [pos[0].float(), pos[1].float(), pos[2].float()] + dev[3:]
+        L = log_intensity_loss(po_intensity(surf, INCS, GRID, 0, sv, fp64=True), target)
+        L.backward()
+        g = delta.grad.clone()
+        rng = np.random.default_rng(2)
+        idx = rng.choice(s.n, 64, replace=False)
+        v = torch.zeros(s.n, dtype=torch.float64, device="cuda")
+        v[torch.from_numpy(idx).cuda()] = torch.from_numpy(rng.standard_normal(64)).cuda()
+        h = 1e-10
+        sp = [(torch.stack(dev[:3]).double() + h * v * nrm)[c].float() for c in range(3)]
+        sm = [(torch.stack(dev[:3]).double() - h * v * nrm)[c].float() for c in range(3)]
+        Lp = _loss64(sv, sp + dev[3:], target)
+        Lm = _loss64(sv, sm + dev[3:], target)
+        # contract dL/dx with the applied (float32-rounded) displacement
+        x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(dev)]
+        L2 = log_intensity_loss(po_intensity(x, INCS, GRID, 0, sv, fp64=True), target)
+        L2.backward()
+        gx = torch.stack([x[c].grad for c in range(3)]).double()
+        dx = torch.stack([sp[c].double() - sm[c].double() for c in range(3)])
+        pred = float((gx * dx).sum().item())
+        assert abs((Lp - Lm) - pred) / abs(Lp - Lm) < 1e-4
+        # and delta.grad is n . dL/dx
+        assert torch.allclose(g, (gx * nrm).sum(0), rtol=1e-5, atol=1e-6 * g.abs().max().item())
+
+
+@pytest.mark.parametrize("depth", [2, 3])
+def test_tree_vjp_matches_fp64_gradient(depth):
+    s, dev, target = _setup(6000, 3)
+    print(f"case: tree vjp depth={depth}", flush=True)
+    with wf.POSolver(device=0, precision="fp32") as sv:
+        grads = {}
+        for name, kw in (("fp64", dict(depth=0, fp64=True)), ("tree", dict(depth=depth, fp64=False))):
+            x = [t.clone().requires_grad_(True) for t in dev]
+            L = log_intensity_loss(po_intensity(x, INCS, GRID, kw["depth"], sv, fp64=kw["fp64"]), target)
+            L.backward()
+            grads[name] = [t.grad.double() for t in x]
+    for c in range(7):
+        a, b = grads["tree"][c], grads["fp64"][c]
+        rel = (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
+        print(f"  component {c}: rel {rel:.2e}", flush=True)
+        assert rel < 1e-3
+
+
+def test_tilt_gradient_vs_fd():
+    print("case: tilt gradient", flush=True)
+    kw = dict(r1=3e-6, r2=2e-6, length=1e-6, spacing=0.05e-6, seed=4)
+    incs = [wf.Incidence(wf.reference_incidence_theta(90), 0.0, 0.45e-6, "M", 1.55 + 0.02j)]
+    grid = (24, 48)
+    with wf.POSolver(device=0) as sv:
+        s0 = [t.float() for t in fiber_surface_torch(torch.tensor(3.0, dtype=torch.float64), **kw)]
+        I0 = sv.solve_fp64_direct(s0, incs, grid)
+        rng = np.random.default_rng(7)
+        target = (I0 * torch.from_numpy(np.exp(0.3 * rng.standard_normal(I0.shape))).cuda()).float()
+        tau = torch.tensor(3.0, dtype=torch.float64, device="cuda", requires_grad=True)
+        s = fiber_surface_torch(tau, **kw)
+        L = log_intensity_loss(po_intensity([t.float() for t in s], incs, grid, 0, sv, fp64=True), target)
+        L.backward()
+        g = tau.grad.item()
+
+        def Lof(t):
+            st = [a.float() for a in fiber_surface_torch(torch.tensor(t, dtype=torch.float64), **kw)]
+            I = sv.solve_fp64_direct(st, incs, grid)
+            return float(((torch.log(I) - torch.log(target.double())) ** 2).sum().item())
+
+        h = 0.02
+        d1 = (Lof(3.0 + h) - Lof(3.0 - h)) / (2 * h)
+        d2 = (Lof(3.0 + h / 2) - Lof(3.0 - h / 2)) / h
+        fd = (4 * d2 - d1) / 3  # Richardson
+        print(f"  dL/dtau adjoint {g:.8e} FD {fd:.8e} (h {d1:.6e}, h/2 {d2:.6e})", flush=True)
+        assert abs(g - fd) / abs(fd) < 1e-4

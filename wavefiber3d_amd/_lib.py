@@ -28,7 +28,7 @@ WF_PREC_FP32, WF_PREC_BF16X3, WF_PREC_FP32_VALU = 0, 1, 2
 EXPORTS = (
     "wf_abi_version", "wf_last_error", "wf_default_opts", "wf_create", "wf_destroy",
     "wf_set_geometry", "wf_solve", "wf_po_solve", "wf_po_solve_fp64_direct", "wf_currents",
-    "wf_octree_nodes", "wf_interp", "wf_get_stats",
+    "wf_octree_nodes", "wf_interp", "wf_get_stats", "wf_po_vjp",
 )
 
 
@@ -131,6 +131,8 @@ def lib(path: str | None = None):
     L.wf_interp.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_int32, vp, ct.c_int32, ct.c_int32,
                             ct.c_double, vp, vp]
     L.wf_get_stats.argtypes = [vp, ct.POINTER(Stats)]
+    L.wf_po_vjp.argtypes = [vp, ct.POINTER(Incidence), ct.c_int32, Grid, vp, vp, vp, vp, vp,
+                            ct.c_int32, ct.c_int32, vp]
     for name in EXPORTS:
         if name not in ("wf_abi_version", "wf_last_error", "wf_default_opts", "wf_destroy"):
             getattr(L, name).restype = ct.c_int

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/wf_po.h"
+#include "wf_adjoint.h"
 #include "wf_common.h"
 #include "wf_fft.h"
 #include "wf_interp.h"
@@ -55,6 +56,7 @@ struct LevelDev {
   int32_t* end = nullptr;
   int32_t* cb = nullptr;     // children range in the next level's list
   int32_t* ce = nullptr;
+  int32_t* parent = nullptr; // index of the parent in the previous level's list
 };
 
 struct FftEntry {
@@ -517,7 +519,7 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
                         hipMemcpyHostToDevice, st));
   // keys + stable sort
   uint32_t *keys, *keys_tmp;
-  int32_t *idx, *idx_tmp, *counts, *scan_tmp, *flags, *pos, *cnt_dev, *pidx;
+  int32_t *idx, *idx_tmp, *counts, *scan_tmp, *flags, *pos, *cnt_dev, *lpar;
   WF_TRY(dalloc_t(h, "keys", n, &keys));
   WF_TRY(dalloc_t(h, "keys_tmp", n, &keys_tmp));
   WF_TRY(dalloc_t(h, "perm", n, &idx));
@@ -528,7 +530,7 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
   WF_TRY(dalloc_t(h, "flags", n, &flags));
   WF_TRY(dalloc_t(h, "pos", n, &pos));
   WF_TRY(dalloc_t(h, "level_counts", 16, &cnt_dev));
-  WF_TRY(dalloc_t(h, "pidx", n, &pidx));
+  WF_TRY(dalloc_t(h, "lvl_parent", (size_t)(depth + 1) * n, &lpar));
   WF_HIP(wf::octree_sort(src[0], src[1], src[2], n, planes_dev, depth, keys, idx, keys_tmp, idx_tmp,
                          counts, scan_tmp, st));
   float* pts = nullptr;
@@ -550,6 +552,7 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
     L.end = le + (size_t)l * n;
     L.cb = lcb + (size_t)l * n;
     L.ce = lce + (size_t)l * n;
+    L.parent = lpar + (size_t)l * n;
     WF_HIP(wf::octree_level(keys, n, 3 * (depth - l), flags, pos, scan_tmp, cnt_dev + l, L.key,
                             L.begin, st));
   }
@@ -561,7 +564,8 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
     WF_HIP(wf::octree_level_finish(h->levels[l].begin, hc[l], (int32_t)n, h->levels[l].end, st));
   }
   for (int l = 0; l < depth; ++l)
-    WF_HIP(wf::octree_links(h->levels[l + 1].begin, hc[l + 1], h->levels[l].begin, hc[l], pidx,
+    WF_HIP(wf::octree_links(h->levels[l + 1].begin, hc[l + 1], h->levels[l].begin, hc[l],
+                            h->levels[l + 1].parent,
                             h->levels[l].cb, h->levels[l].ce, st));
   h->n = n;
   h->depth = depth;
@@ -649,9 +653,11 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
   for (int i = 0; i < n_inc; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
-  double2 *C = nullptr, *F0 = nullptr;
+  double2 *C = nullptr, *F0 = nullptr, *part = nullptr;
   WF_TRY(dalloc_t(h, "direct_C", (size_t)n * 24, &C));
   WF_TRY(dalloc_t(h, "direct_F0", (size_t)D * 24, &F0));
+  const int64_t part_elems = (int64_t)wf::direct_split(D) * 24 * D;
+  WF_TRY(dalloc_t(h, "direct_part", (size_t)part_elems, &part));
   prof_reset(h);
   WF_TRY(prof_event(h, st, &h->ev_t0));
   for (int i0 = 0; i0 < n_inc;) {
@@ -667,7 +673,7 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
     WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
       return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, 6 * S, centre,
                                     2.0 * WF_PI / inc[order[i0]].lambda_m, grid.n_theta,
-                                    grid.n_phi, F0, st);
+                                    grid.n_phi, F0, part, part_elems, st);
     }));
     for (int q = 0; q < S; ++q) {
       const int slot = order[i0 + q];
@@ -688,6 +694,240 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
   }
   return WF_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Per-wavelength tree context: level grids and translation tables.
+struct GroupCtx {
+  double lam = 0, k0 = 0;
+  std::vector<Grid> g;
+  float2* T = nullptr;
+  std::vector<int64_t> toff;
+};
+
+wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP, GroupCtx& gc) {
+  const int depth = h->depth;
+  gc.lam = lam;
+  gc.k0 = 2.0 * WF_PI / lam;
+  const double lam_s = h->opts.lambda_sample_m > 0 ? h->opts.lambda_sample_m : lam;
+  gc.g.assign(depth + 1, Grid{NT, NP});
+  gc.toff.assign(depth + 1, 0);
+  if (depth == 0) return WF_OK;
+  for (int l = 1; l <= depth; ++l) gc.g[l] = level_grid(h, l, lam_s);
+  if (h->opts.strict_grid && (NT < gc.g[1].nt || NP < gc.g[1].np))
+    return set_err(WF_ERR_GRID_TOO_COARSE, "output grid %dx%d coarser than level-1 grid %dx%d",
+                   NT, NP, gc.g[1].nt, gc.g[1].np);
+  int64_t tt = 0;
+  for (int l = 0; l < depth; ++l) {
+    gc.toff[l] = tt;
+    tt += 8LL * gc.g[l].nt * gc.g[l].np;
+  }
+  WF_TRY(dalloc_t(h, "trans", (size_t)tt, &gc.T));
+  for (int l = 0; l < depth; ++l) {
+    const double th0 =
+        (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * gc.g[l].nt));
+    const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
+                          0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
+                          0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
+    const Grid gl = gc.g[l];
+    float2* Tl = gc.T + gc.toff[l];
+    const double k0 = gc.k0;
+    WF_TRY(timed(h, st, WF_K_TRANS, 64.0 * gl.nt * gl.np,
+                 [&] { return wf::launch_trans_table(Tl, gl.nt, gl.np, th0, hc, k0, st); }));
+  }
+  for (int l = 0; l <= depth && l < 16; ++l) {
+    h->stats.level_nt[l] = gc.g[l].nt;
+    h->stats.level_np[l] = gc.g[l].np;
+    h->stats.level_nodes[l] = h->levels[l].count;
+  }
+  h->stats.n_levels = depth + 1;
+  return WF_OK;
+}
+
+double theta_origin(int l, int nt) {
+  return (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * nt));
+}
+
+int mirror_of(int nt, double off) { return (int)std::lround(nt * (1.0 - 2.0 * off / WF_PI)); }
+
+wf::LeafArgs leaf_args(wf_handle* h, const GroupCtx& gc, const float2* C, int F, float2* out) {
+  const int depth = h->depth;
+  const int64_t n = h->n;
+  const LevelDev& LL = h->levels[depth];
+  wf::LeafArgs la;
+  la.px = h->pts;
+  la.py = h->pts + n;
+  la.pz = h->pts + 2 * n;
+  la.n = n;
+  la.C = C;
+  la.Ftot = F;
+  la.f0 = 0;
+  la.F = F;
+  la.leaf_begin = LL.begin;
+  la.leaf_end = LL.end;
+  la.leaf_key = LL.key;
+  la.nleaves = LL.count;
+  la.depth = depth;
+  for (int q = 0; q < 3; ++q) {
+    la.box_lo[q] = h->box[q];
+    la.leaf_size[q] = (h->box[q + 3] - h->box[q]) / std::ldexp(1.0, depth);
+  }
+  la.k0 = gc.k0;
+  la.nt = gc.g[depth].nt;
+  la.np = gc.g[depth].np;
+  la.out = out;
+  la.precision = h->opts.precision;
+  return la;
+}
+
+struct TreeBufs {
+  float2 *patA = nullptr, *patB = nullptr, *G = nullptr, *G0 = nullptr;
+};
+
+wf_status tree_buffers(wf_handle* h, const GroupCtx& gc, int F, int NP, TreeBufs& tb) {
+  const int depth = h->depth;
+  size_t pat_max = 16, g_max = 16;
+  for (int l = 1; l <= depth; ++l)
+    pat_max = std::max(pat_max, (size_t)h->levels[l].count * F * gc.g[l].nt * gc.g[l].np);
+  for (int l = 1; l < depth; ++l)
+    g_max = std::max(g_max, (size_t)h->levels[l + 1].count * F * gc.g[l + 1].nt * gc.g[l].np);
+  WF_TRY(dalloc_t(h, "patA", pat_max, &tb.patA));
+  WF_TRY(dalloc_t(h, "patB", pat_max, &tb.patB));
+  WF_TRY(dalloc_t(h, "G", g_max, &tb.G));
+  WF_TRY(dalloc_t(h, "G0", (size_t)h->levels[1].count * F * gc.g[1].nt * NP, &tb.G0));
+  return WF_OK;
+}
+
+// Forward tree pass of one chunk of S solves sharing a wavelength: currents ->
+// leaf sums -> aggregation -> root evaluation on the output grid.  F0 receives
+// the FP64 far-field components [6 S][Nt*Np].
+wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const WfIncDev* dinc,
+                        const int* order, int c0, int S, int NT, int NP, const TreeBufs& tb,
+                        double2* F0) {
+  const int depth = h->depth;
+  const int64_t n = h->n;
+  const int64_t D = (int64_t)NT * NP;
+  const int F = 6 * S;
+  const float* px = h->pts;
+  float2* C = nullptr;
+  WF_TRY(dalloc_t(h, "C", (size_t)n * F, &C));
+  for (int q = 0; q < S; ++q)
+    WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n, [&] {
+      return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                 px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F, 0, st);
+    }));
+  float2* cur = (depth % 2) ? tb.patA : tb.patB;
+  const wf::LeafArgs la = leaf_args(h, gc, C, F, cur);
+  const double dleaf = (double)gc.g[depth].nt * gc.g[depth].np;
+  WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F, [&] {
+    if (h->opts.precision == WF_PREC_FP32_VALU) return wf::launch_leaf(la, st);
+    return wf::launch_leaf_mfma(la, h->opts.precision == WF_PREC_BF16X3, st);
+  }));
+  h->stats.leaf_pairs += (int64_t)(dleaf * n);
+  for (int l = depth - 1; l >= 0; --l) {
+    if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+    const Grid gcg = gc.g[l + 1], gp = gc.g[l];
+    FftPlanDev pc, pp, tc, tp;
+    WF_TRY(get_fft(h, gcg.np, &pc));
+    WF_TRY(get_fft(h, gp.np, &pp));
+    WF_TRY(get_fft(h, 2 * gcg.nt, &tc));
+    WF_TRY(get_fft(h, 2 * gp.nt, &tp));
+    const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
+    const double planes = (double)nchild * F;
+    float2* G = (l == 0) ? tb.G0 : tb.G;
+    WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
+      return wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gcg.nt, gcg.np, gp.np, pc, pp, st);
+    }));
+    const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
+    const int mirror = mirror_of(gp.nt, off_p);
+    const float2* Tl = gc.T + gc.toff[l];
+    if (l == 0) {
+      WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * NP + 16.0 * F * D, [&] {
+        return wf::launch_theta_pass(G, F, gcg.nt, NP, NT, 1, h->levels[0].cb, h->levels[0].ce,
+                                     h->levels[1].key, Tl, off_p - off_c, mirror, tc, tp, nullptr,
+                                     F0, st);
+      }));
+    } else {
+      float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
+      WF_TRY(timed(h, st, WF_K_THETA,
+                   8.0 * F * ((double)nchild * gcg.nt * gp.np + (double)npar * gp.nt * gp.np), [&] {
+        return wf::launch_theta_pass(G, F, gcg.nt, gp.np, gp.nt, npar, h->levels[l].cb,
+                                     h->levels[l].ce, h->levels[l + 1].key, Tl, off_p - off_c,
+                                     mirror, tc, tp, nxt, nullptr, st);
+      }));
+      cur = nxt;
+    }
+  }
+  return WF_OK;
+}
+
+// Reverse tree pass of one chunk: root cogradient gF [F][D] (FP64) -> leaf
+// cogradients -> per-point back-projection -> gradient accumulation (acc).
+wf_status backward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const WfIncDev* dinc,
+                         const int* order, int c0, int S, int NT, int NP, const TreeBufs& tb,
+                         const double2* gF, double* acc) {
+  const int depth = h->depth;
+  const int64_t n = h->n;
+  const int F = 6 * S;
+  float2* gcur = nullptr;  // cogradient of the current level's patterns
+  for (int l = 0; l < depth; ++l) {
+    const Grid gp = gc.g[l], gch = gc.g[l + 1];
+    FftPlanDev pc, pp, tc, tp;
+    WF_TRY(get_fft(h, gch.np, &pc));
+    WF_TRY(get_fft(h, gp.np, &pp));
+    WF_TRY(get_fft(h, 2 * gch.nt, &tc));
+    WF_TRY(get_fft(h, 2 * gp.nt, &tp));
+    const int nchild = h->levels[l + 1].count;
+    const double off_c = theta_origin(l + 1, gch.nt), off_p = theta_origin(l, gp.nt);
+    const int mirror = mirror_of(gp.nt, off_p);
+    float2* G = (l == 0) ? tb.G0 : tb.G;
+    const float2* Tl = gc.T + gc.toff[l];
+    WF_TRY(timed(h, st, WF_K_THETA, 0.0, [&] {
+      return wf::launch_theta_adj(l == 0 ? nullptr : gcur, l == 0 ? gF : nullptr, F, gch.nt,
+                                  gp.np, gp.nt, nchild, h->levels[l + 1].parent,
+                                  h->levels[l + 1].key, Tl, off_p - off_c, mirror, tp, tc, G, st);
+    }));
+    float2* nxt = (gcur == tb.patA) ? tb.patB : tb.patA;
+    WF_TRY(timed(h, st, WF_K_PHI, 0.0, [&] {
+      return wf::launch_phi_adj(G, nxt, (int64_t)nchild * F, gch.nt, gch.np, gp.np, pp, pc, st);
+    }));
+    gcur = nxt;
+  }
+  float2* Q = nullptr;
+  WF_TRY(dalloc_t(h, "adj_Q", (size_t)n * F * 4, &Q));
+  wf::LeafAdjArgs aa;
+  const LevelDev& LL = h->levels[depth];
+  aa.px = h->pts;
+  aa.py = h->pts + n;
+  aa.pz = h->pts + 2 * n;
+  aa.leaf_begin = LL.begin;
+  aa.leaf_end = LL.end;
+  aa.leaf_key = LL.key;
+  aa.nleaves = LL.count;
+  aa.depth = depth;
+  for (int q = 0; q < 3; ++q) {
+    aa.box_lo[q] = h->box[q];
+    aa.leaf_size[q] = (h->box[q + 3] - h->box[q]) / std::ldexp(1.0, depth);
+  }
+  aa.k0 = gc.k0;
+  aa.nt = gc.g[depth].nt;
+  aa.np = gc.g[depth].np;
+  aa.F = F;
+  aa.g = gcur;
+  aa.Q = Q;
+  WF_TRY(timed(h, st, WF_K_LEAF, 0.0, [&] { return wf::launch_leaf_adj(aa, st); }));
+  for (int q = 0; q < S; ++q)
+    WF_TRY(timed(h, st, WF_K_CURRENTS, 0.0, [&] {
+      return wf::launch_grad_assemble(h->pts, n, Q, false, F, 6 * q, dinc + order[c0 + q], acc, st);
+    }));
+  return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
                    float* intensity, float* E_cplx, int32_t out_memspace, void* stream) {
@@ -729,182 +969,47 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
   h->stats.depth = depth;
   WF_TRY(prof_event(h, st, &h->ev_t0));
   int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6) : 8;
+  max_solves = std::min(max_solves, 8);
   if (depth == 0) max_solves = std::min(max_solves, 4);
   const double epi_bytes = (96.0 + 4.0 + (E_dev ? 24.0 : 0.0)) * D;
-  std::vector<Grid> g(depth + 1);
   for (int i0 = 0; i0 < n_inc;) {
-    // wavelength group
     int i1 = i0 + 1;
     while (i1 < n_inc && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
-    const double lam = inc[order[i0]].lambda_m;
-    const double k0 = 2.0 * WF_PI / lam;
-    const double lam_s = h->opts.lambda_sample_m > 0 ? h->opts.lambda_sample_m : lam;
-    // level grids and translation tables for this wavelength
-    float2* T = nullptr;
-    std::vector<int64_t> toff(depth + 1, 0);
-    if (depth > 0) {
-      g[0] = {NT, NP};
-      for (int l = 1; l <= depth; ++l) g[l] = level_grid(h, l, lam_s);
-      if (h->opts.strict_grid && (NT < g[1].nt || NP < g[1].np))
-        return set_err(WF_ERR_GRID_TOO_COARSE, "output grid %dx%d coarser than level-1 grid %dx%d",
-                       NT, NP, g[1].nt, g[1].np);
-      int64_t tt = 0;
-      for (int l = 0; l < depth; ++l) {
-        toff[l] = tt;
-        tt += 8LL * g[l].nt * g[l].np;
-      }
-      WF_TRY(dalloc_t(h, "trans", (size_t)tt, &T));
-      for (int l = 0; l < depth; ++l) {
-        const double th0 = (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * g[l].nt));
-        const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
-                              0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
-                              0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
-        WF_TRY(timed(h, st, WF_K_TRANS, 64.0 * g[l].nt * g[l].np, [&] {
-          return wf::launch_trans_table(T + toff[l], g[l].nt, g[l].np, th0, hc, k0, st);
-        }));
-      }
-      for (int l = 0; l <= depth && l < 16; ++l) {
-        h->stats.level_nt[l] = g[l].nt;
-        h->stats.level_np[l] = g[l].np;
-        h->stats.level_nodes[l] = h->levels[l].count;
-      }
-      h->stats.n_levels = depth + 1;
-    }
+    GroupCtx gc;
+    WF_TRY(prepare_group(h, st, inc[order[i0]].lambda_m, NT, NP, gc));
     for (int c0 = i0; c0 < i1; c0 += max_solves) {
       const int S = std::min(max_solves, i1 - c0);
       const int F = 6 * S;
       if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+      double2* F0 = nullptr;
+      WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
       if (depth == 0) {
-        double2 *C = nullptr, *F0 = nullptr;
+        double2 *C = nullptr, *part = nullptr;
         WF_TRY(dalloc_t(h, "direct_C", (size_t)n * F, &C));
-        WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
+        const int64_t part_elems = (int64_t)wf::direct_split(D) * F * D;
+        WF_TRY(dalloc_t(h, "direct_part", (size_t)part_elems, &part));
         for (int q = 0; q < S; ++q)
           WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 96.0) * n, [&] {
             return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
                                        px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st);
           }));
+        const double k0 = gc.k0;
         WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
-          return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0, st);
+          return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0,
+                                        part, part_elems, st);
         }));
-        for (int q = 0; q < S; ++q) {
-          const int slot = order[c0 + q];
-          WF_TRY(timed(h, st, WF_K_EPILOGUE, epi_bytes, [&] {
-            return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
-                                        E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
-                                        nullptr, st);
-          }));
-        }
-        continue;
+      } else {
+        TreeBufs tb;
+        WF_TRY(tree_buffers(h, gc, F, NP, tb));
+        WF_TRY(forward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, F0));
       }
-      // ---- currents (sorted order), [n][F] complex64
-      float2* C = nullptr;
-      WF_TRY(dalloc_t(h, "C", (size_t)n * F, &C));
-      for (int q = 0; q < S; ++q)
-        WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n, [&] {
-          return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                     px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F, 0, st);
+      for (int q = 0; q < S; ++q) {
+        const int slot = order[c0 + q];
+        WF_TRY(timed(h, st, WF_K_EPILOGUE, epi_bytes, [&] {
+          return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
+                                      E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
+                                      nullptr, st);
         }));
-      // ---- buffers: patterns per level (ping-pong) and phi-pass scratch
-      size_t pat_max = 0, g_max = 0;
-      for (int l = 1; l <= depth; ++l)
-        pat_max = std::max(pat_max, (size_t)h->levels[l].count * F * g[l].nt * g[l].np);
-      for (int l = 0; l < depth; ++l)
-        g_max = std::max(g_max, (size_t)h->levels[l + 1].count * F * g[l + 1].nt * g[l].np);
-      float2 *patA = nullptr, *patB = nullptr, *G = nullptr;
-      double2* F0 = nullptr;
-      WF_TRY(dalloc_t(h, "patA", pat_max, &patA));
-      WF_TRY(dalloc_t(h, "patB", pat_max, &patB));
-      WF_TRY(dalloc_t(h, "G", g_max, &G));
-      WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
-      // ---- leaf radiation sums (level depth)
-      const LevelDev& LL = h->levels[depth];
-      wf::LeafArgs la;
-      la.px = px;
-      la.py = px + n;
-      la.pz = px + 2 * n;
-      la.n = n;
-      la.C = C;
-      la.Ftot = F;
-      la.f0 = 0;
-      la.F = F;
-      la.leaf_begin = LL.begin;
-      la.leaf_end = LL.end;
-      la.leaf_key = LL.key;
-      la.nleaves = LL.count;
-      la.depth = depth;
-      for (int q = 0; q < 3; ++q) {
-        la.box_lo[q] = h->box[q];
-        la.leaf_size[q] = (h->box[q + 3] - h->box[q]) / std::ldexp(1.0, depth);
-      }
-      la.k0 = k0;
-      la.nt = g[depth].nt;
-      la.np = g[depth].np;
-      float2* cur = (depth % 2) ? patA : patB;
-      la.out = cur;
-      la.precision = h->opts.precision;
-      const double dleaf = (double)g[depth].nt * g[depth].np;
-      WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F, [&] {
-        if (h->opts.precision == WF_PREC_FP32_VALU) return wf::launch_leaf(la, st);
-        return wf::launch_leaf_mfma(la, h->opts.precision == WF_PREC_BF16X3, st);
-      }));
-      h->stats.leaf_pairs += (int64_t)(dleaf * n);
-      // ---- aggregation levels depth-1 .. 1
-      for (int l = depth - 1; l >= 1; --l) {
-        if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
-        const Grid gc = g[l + 1], gp = g[l];
-        FftPlanDev pc, pp, tc, tp;
-        WF_TRY(get_fft(h, gc.np, &pc));
-        WF_TRY(get_fft(h, gp.np, &pp));
-        WF_TRY(get_fft(h, 2 * gc.nt, &tc));
-        WF_TRY(get_fft(h, 2 * gp.nt, &tp));
-        const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
-        const double planes = (double)nchild * F;
-        WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gc.nt * ((double)gc.np + gp.np), [&] {
-          return wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gc.nt, gc.np, gp.np, pc, pp, st);
-        }));
-        float2* nxt = (cur == patA) ? patB : patA;
-        const double off_c = -WF_PI / 2 + WF_PI / (2.0 * gc.nt);
-        const double off_p = -WF_PI / 2 + WF_PI / (2.0 * gp.nt);
-        const int mirror = (int)std::lround(gp.nt * (1.0 - 2.0 * off_p / WF_PI));
-        WF_TRY(timed(h, st, WF_K_THETA,
-                     8.0 * F * ((double)nchild * gc.nt * gp.np + (double)npar * gp.nt * gp.np), [&] {
-          return wf::launch_theta_pass(G, F, gc.nt, gp.np, gp.nt, npar, h->levels[l].cb,
-                                       h->levels[l].ce, h->levels[l + 1].key, T + toff[l],
-                                       off_p - off_c, mirror, tc, tp, nxt, nullptr, st);
-        }));
-        cur = nxt;
-      }
-      // ---- root: level 1 -> output grid, FP64 accumulation
-      {
-        const Grid gc = g[1];
-        FftPlanDev pc, pp, tc, tp;
-        WF_TRY(get_fft(h, gc.np, &pc));
-        WF_TRY(get_fft(h, NP, &pp));
-        WF_TRY(get_fft(h, 2 * gc.nt, &tc));
-        WF_TRY(get_fft(h, 2 * NT, &tp));
-        const int nchild = h->levels[1].count;
-        float2* G0 = nullptr;
-        WF_TRY(dalloc_t(h, "G0", (size_t)nchild * F * gc.nt * NP, &G0));
-        const double planes = (double)nchild * F;
-        WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gc.nt * ((double)gc.np + NP), [&] {
-          return wf::launch_phi_pass(cur, G0, (int64_t)nchild * F, gc.nt, gc.np, NP, pc, pp, st);
-        }));
-        const double off_c = -WF_PI / 2 + WF_PI / (2.0 * gc.nt);
-        const double off_p = -WF_PI / 2 + 1e-10;
-        const int mirror = (int)std::lround(NT * (1.0 - 2.0 * off_p / WF_PI));
-        WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gc.nt * NP + 16.0 * F * D, [&] {
-          return wf::launch_theta_pass(G0, F, gc.nt, NP, NT, 1, h->levels[0].cb, h->levels[0].ce,
-                                       h->levels[1].key, T + toff[0], off_p - off_c, mirror, tc, tp,
-                                       nullptr, F0, st);
-        }));
-        for (int q = 0; q < S; ++q) {
-          const int slot = order[c0 + q];
-          WF_TRY(timed(h, st, WF_K_EPILOGUE, epi_bytes, [&] {
-            return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
-                                        E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
-                                        nullptr, st);
-          }));
-        }
       }
     }
     i0 = i1;
@@ -914,6 +1019,129 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
     WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
     if (E_cplx)
       WF_HIP(hipMemcpyAsync(E_cplx, E_dev, sizeof(float) * n_inc * 6 * D, hipMemcpyDeviceToHost, st));
+    WF_HIP(hipStreamSynchronize(st));
+  }
+  return WF_OK;
+}
+
+wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                    const float* grad_I, float* intensity, float* grad_x, float* grad_n,
+                    float* grad_a, int32_t fp64, int32_t memspace, void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "wf_po_vjp: no geometry");
+  if (!grad_I) return set_err(WF_ERR_INVALID_ARG, "grad_I is null");
+  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
+  if (memspace != WF_MEM_HOST && memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "memspace unknown");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  const int64_t n = h->n;
+  const int depth = h->depth;
+  const bool direct = fp64 || depth == 0;
+  const int NT = grid.n_theta, NP = grid.n_phi;
+  const int64_t D = (int64_t)NT * NP;
+  if (!direct) {
+    if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
+    if (!is_smooth(NP) || !is_smooth(2 * NT))
+      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
+                     NT, NP);
+  }
+  WfIncDev* dinc = nullptr;
+  WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
+  const float* gI = grad_I;
+  float* I_dev = intensity;
+  float *gx = grad_x, *gn = grad_n, *ga = grad_a;
+  if (memspace == WF_MEM_HOST) {
+    float* t = nullptr;
+    WF_TRY(dalloc_t(h, "vjp_gI", (size_t)n_inc * D, &t));
+    WF_HIP(hipMemcpyAsync(t, grad_I, sizeof(float) * n_inc * D, hipMemcpyHostToDevice, st));
+    gI = t;
+    if (intensity) WF_TRY(dalloc_t(h, "vjp_I", (size_t)n_inc * D, &I_dev));
+    if (grad_x) WF_TRY(dalloc_t(h, "vjp_gx", (size_t)3 * n, &gx));
+    if (grad_n) WF_TRY(dalloc_t(h, "vjp_gn", (size_t)3 * n, &gn));
+    if (grad_a) WF_TRY(dalloc_t(h, "vjp_ga", (size_t)n, &ga));
+  }
+  double* acc = nullptr;
+  WF_TRY(dalloc_t(h, "vjp_acc", (size_t)7 * n, &acc));
+  WF_HIP(hipMemsetAsync(acc, 0, sizeof(double) * 7 * n, st));
+  std::vector<int> order(n_inc);
+  for (int i = 0; i < n_inc; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
+  const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
+                            0.5 * (h->box[2] + h->box[5])};
+  prof_reset(h);
+  h->stats.depth = direct ? 0 : depth;
+  WF_TRY(prof_event(h, st, &h->ev_t0));
+  const float* px = h->pts;
+  const int max_solves = direct ? 1 : 8;
+  for (int i0 = 0; i0 < n_inc;) {
+    int i1 = i0 + 1;
+    while (i1 < n_inc && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
+    GroupCtx gc;
+    if (direct) {
+      gc.lam = inc[order[i0]].lambda_m;
+      gc.k0 = 2.0 * WF_PI / gc.lam;
+    } else {
+      WF_TRY(prepare_group(h, st, inc[order[i0]].lambda_m, NT, NP, gc));
+    }
+    for (int c0 = i0; c0 < i1; c0 += max_solves) {
+      const int S = std::min(max_solves, i1 - c0);
+      const int F = 6 * S;
+      if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+      double2 *F0 = nullptr, *gF = nullptr;
+      WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
+      WF_TRY(dalloc_t(h, "vjp_gF", (size_t)D * F, &gF));
+      TreeBufs tb;
+      if (direct) {
+        double2 *C = nullptr, *part = nullptr;
+        WF_TRY(dalloc_t(h, "direct_C", (size_t)n * F, &C));
+        const int64_t part_elems = (int64_t)wf::direct_split(D) * F * D;
+        WF_TRY(dalloc_t(h, "direct_part", (size_t)part_elems, &part));
+        WF_HIP(wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                   px + 6 * n, n, dinc + order[c0], 1, 1, C, F, 0, st));
+        const double k0 = gc.k0;
+        WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
+          return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0,
+                                        part, part_elems, st);
+        }));
+      } else {
+        WF_TRY(tree_buffers(h, gc, F, NP, tb));
+        WF_TRY(forward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, F0));
+      }
+      for (int q = 0; q < S; ++q) {
+        const int slot = order[c0 + q];
+        WF_TRY(timed(h, st, WF_K_EPILOGUE, 0.0, [&] {
+          return wf::launch_epilogue_adj(F0 + (int64_t)q * 6 * D, gI + (int64_t)slot * D, NT, NP,
+                                         gF + (int64_t)q * 6 * D,
+                                         I_dev ? I_dev + (int64_t)slot * D : nullptr, st);
+        }));
+      }
+      if (direct) {
+        double2* Qd = nullptr;
+        WF_TRY(dalloc_t(h, "adj_Qd", (size_t)n * 24, &Qd));
+        const double k0 = gc.k0;
+        WF_TRY(timed(h, st, WF_K_DIRECT, 0.0, [&] {
+          return wf::launch_direct_adj_fp64(px, n, centre, k0, NT, NP, gF, Qd, st);
+        }));
+        WF_TRY(timed(h, st, WF_K_CURRENTS, 0.0, [&] {
+          return wf::launch_grad_assemble(px, n, Qd, true, 6, 0, dinc + order[c0], acc, st);
+        }));
+      } else {
+        WF_TRY(backward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, gF, acc));
+      }
+    }
+    i0 = i1;
+  }
+  WF_HIP(wf::launch_grad_scatter(acc, h->perm, n, gx, gn, ga, st));
+  WF_TRY(prof_event(h, st, &h->ev_t1));
+  if (memspace == WF_MEM_HOST) {
+    if (intensity)
+      WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
+    if (grad_x) WF_HIP(hipMemcpyAsync(grad_x, gx, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (grad_n) WF_HIP(hipMemcpyAsync(grad_n, gn, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (grad_a) WF_HIP(hipMemcpyAsync(grad_a, ga, sizeof(float) * n, hipMemcpyDeviceToHost, st));
     WF_HIP(hipStreamSynchronize(st));
   }
   return WF_OK;

@@ -74,9 +74,14 @@ hipError_t launch_currents(const float* x, const float* y, const float* z, const
                            const WfIncDev* inc, int n_inc, int fp64, void* out, int64_t stride_n,
                            int64_t stride_s, hipStream_t st);
 
+// Number of point chunks the direct kernel splits into for D directions.
+int direct_split(int64_t D);
+// scratch: partial sums of the point-split grid (up to 32 x F x D double2;
+// fewer splits when smaller, none when null).
 hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, int64_t n,
                               const double2* C, int F, const double* centre, double k0,
-                              int ntheta, int nphi, double2* Fout, hipStream_t st);
+                              int ntheta, int nphi, double2* Fout, double2* scratch,
+                              int64_t scratch_elems, hipStream_t st);
 
 hipError_t launch_intensity(const double2* F0, int S, int ntheta, int nphi, float* I, float* E,
                             double* Id, double* Ed, hipStream_t st);

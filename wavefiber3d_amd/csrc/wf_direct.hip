@@ -9,6 +9,8 @@
 // all S solves of one wavelength share the phase (F = 6 S complex columns).
 // Phase k0 s.(x - c) is formed in FP64 in revolutions and reduced exactly
 // before the FP64 sincos.
+#include <algorithm>
+
 #include "wf_common.h"
 
 namespace {
@@ -37,13 +39,16 @@ __global__ __launch_bounds__(kDirThreads) void direct_fp64_kernel(
   __shared__ double2 sc[kPtTile][F];
   const int64_t D = (int64_t)ntheta * nphi;
   const int64_t d = (int64_t)blockIdx.x * kDirThreads + threadIdx.x;
+  // point range of this block (split over blockIdx.y; partial sums reduced in order)
+  const int64_t p0 = n * blockIdx.y / gridDim.y, p1 = n * (blockIdx.y + 1) / gridDim.y;
+  Fout += (int64_t)blockIdx.y * F * D;
   double s[3] = {0, 0, 0};
   if (d < D) grid_dir(d, nphi, ntheta, s);
   double2 acc[F];
 #pragma unroll
   for (int f = 0; f < F; ++f) acc[f] = make_double2(0.0, 0.0);
-  for (int64_t base = 0; base < n; base += kPtTile) {
-    const int cnt = (int)min<int64_t>(kPtTile, n - base);
+  for (int64_t base = p0; base < p1; base += kPtTile) {
+    const int cnt = (int)min<int64_t>(kPtTile, p1 - base);
     __syncthreads();
     for (int t = threadIdx.x; t < cnt; t += kDirThreads) {
       const int64_t i = base + t;
@@ -71,6 +76,17 @@ __global__ __launch_bounds__(kDirThreads) void direct_fp64_kernel(
   if (d < D) {
 #pragma unroll
     for (int f = 0; f < F; ++f) Fout[(int64_t)f * D + d] = acc[f];
+  }
+}
+
+// Ordered sum of the per-chunk partial patterns: out[f][d] = sum_p part[p][f][d].
+__global__ __launch_bounds__(256) void reduce_parts_kernel(const double2* __restrict__ part, int P,
+                                                           int64_t FD, double2* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < FD;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double2 a = part[i];
+    for (int p = 1; p < P; ++p) a = zadd(a, part[(int64_t)p * FD + i]);
+    out[i] = a;
   }
 }
 
@@ -117,16 +133,27 @@ __global__ __launch_bounds__(256) void intensity_kernel(const double2* __restric
 
 namespace wf {
 
+int direct_split(int64_t D) {
+  // split the points so the grid holds >= ~2048 blocks (8 per CU)
+  const int64_t dblocks = (D + kDirThreads - 1) / kDirThreads;
+  return (int)std::min<int64_t>(32, std::max<int64_t>(1, (2048 + dblocks - 1) / dblocks));
+}
+
 hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, int64_t n,
                               const double2* C, int F, const double* centre, double k0,
-                              int ntheta, int nphi, double2* Fout, hipStream_t st) {
+                              int ntheta, int nphi, double2* Fout, double2* scratch,
+                              int64_t scratch_elems, hipStream_t st) {
   const int64_t D = (int64_t)ntheta * nphi;
-  dim3 grid((unsigned)((D + kDirThreads - 1) / kDirThreads));
+  const int64_t dblocks = (D + kDirThreads - 1) / kDirThreads;
+  int P = direct_split(D);
+  while (P > 1 && ((int64_t)P * F * D > scratch_elems || n / P < 256)) --P;
+  dim3 grid((unsigned)dblocks, (unsigned)P);
+  double2* dst = (P > 1) ? scratch : Fout;
   const double k_rev = k0 / WF_TWO_PI;
 #define WF_DIRECT_CASE(FF)                                                                 \
   case FF:                                                                                 \
     hipLaunchKernelGGL(direct_fp64_kernel<FF>, grid, dim3(kDirThreads), 0, st, x, y, z, n, \
-                       C, centre[0], centre[1], centre[2], k_rev, ntheta, nphi, Fout);     \
+                       C, centre[0], centre[1], centre[2], k_rev, ntheta, nphi, dst);      \
     break;
   switch (F) {
     WF_DIRECT_CASE(6)
@@ -137,6 +164,11 @@ hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, in
       return hipErrorInvalidValue;
   }
 #undef WF_DIRECT_CASE
+  if (P > 1) {
+    const int64_t FD = (int64_t)F * D;
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3((unsigned)std::min<int64_t>(8192, (FD + 255) / 256)),
+                       dim3(256), 0, st, scratch, P, FD, Fout);
+  }
   return hipGetLastError();
 }
 

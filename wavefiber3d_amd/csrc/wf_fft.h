@@ -142,3 +142,35 @@ __device__ inline void spectral_resample(const float2* __restrict__ X, float2* _
     Y[line * ystride + q] = cscale(acc, scale);
   }
 }
+
+// Hermitian adjoint of spectral_resample (without its scale): for every
+// child bin m, gather the parent bins its band frequencies map to,
+//   X[m] = sum_{f in band(nc), f = m mod nc} w_f exp(-i f shift) Y[f mod np],
+// w_f = 1/2 for the two Nyquist halves (nc even), else 1.
+__device__ inline void spectral_resample_adj(const float2* __restrict__ Y, float2* __restrict__ X,
+                                             int nlines, int ystride, int xstride, int nc, int np,
+                                             float shift, float scale) {
+  const int total = nlines * nc;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int line = t / nc, m = t - line * nc;
+    const float2* y = Y + line * ystride;
+    float2 acc = make_float2(0.f, 0.f);
+    const bool nyq = (nc % 2 == 0) && (m == nc / 2);
+    const int nf = nyq ? 2 : 1;
+    for (int h = 0; h < nf; ++h) {
+      int f = (m <= (nc - 1) / 2) ? m : m - nc;
+      if (nyq) f = h ? -(nc / 2) : nc / 2;
+      int q = f % np;
+      if (q < 0) q += np;
+      float2 v = y[q];
+      if (nyq) v = cscale(v, 0.5f);
+      if (shift != 0.f) {
+        float s, c;
+        sincosf(-shift * (float)f, &s, &c);
+        v = cmul(v, make_float2(c, s));
+      }
+      acc = cadd(acc, v);
+    }
+    X[line * xstride + m] = cscale(acc, scale);
+  }
+}

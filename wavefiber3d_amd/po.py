@@ -267,6 +267,40 @@ class POSolver:
                               out.data_ptr(), torch.cuda.current_stream().cuda_stream))
         return out
 
+    def vjp(self, incidences, grid, grad_I, fp64: bool = False, return_I: bool = False):
+        """Reverse mode on the current geometry: dL/dI -> (dL/dx [3, n], dL/dn [3, n],
+        dL/da [n]) in the original point order.  grad_I: numpy (host) or torch CUDA
+        [n_inc, Nt, Np].  fp64 selects the FP64 direct-sum gradient."""
+        arr, k = _incs(incidences)
+        nt, np_ = grid
+        g = _lib.Grid(nt, np_)
+        n = self.n
+        if _is_torch(grad_I) and grad_I.is_cuda:
+            import torch
+
+            gI = grad_I.to(torch.float32).contiguous()
+            dev = gI.device
+            gx = torch.empty((3, n), dtype=torch.float32, device=dev)
+            gn = torch.empty((3, n), dtype=torch.float32, device=dev)
+            ga = torch.empty((n,), dtype=torch.float32, device=dev)
+            I = torch.empty((k, nt, np_), dtype=torch.float32, device=dev) if return_I else None
+            check(lib().wf_po_vjp(self._h, arr, k, g, gI.data_ptr(),
+                                  I.data_ptr() if I is not None else None, gx.data_ptr(),
+                                  gn.data_ptr(), ga.data_ptr(), int(fp64), WF_MEM_DEVICE,
+                                  torch.cuda.current_stream(dev).cuda_stream))
+        else:
+            gI = np.ascontiguousarray(grad_I, dtype=np.float32)
+            gx = np.empty((3, n), np.float32)
+            gn = np.empty((3, n), np.float32)
+            ga = np.empty((n,), np.float32)
+            I = np.empty((k, nt, np_), np.float32) if return_I else None
+            check(lib().wf_po_vjp(self._h, arr, k, g, gI.ctypes.data,
+                                  I.ctypes.data if I is not None else None, gx.ctypes.data,
+                                  gn.ctypes.data, ga.ctypes.data, int(fp64), WF_MEM_HOST, None))
+        if return_I:
+            return gx, gn, ga, I
+        return gx, gn, ga
+
     def stats(self) -> dict:
         """Per-kernel HIP-event timings of the last solve (waits for its events)."""
         s = _lib.Stats()
