@@ -100,6 +100,9 @@ typedef struct wf_opts {
   int64_t workspace_cap_bytes;   /* 0 = unlimited; exceeded -> WF_ERR_OOM */
   volatile int32_t* cancel_flag; /* nullable; polled between stages */
   int32_t field_chunk;   /* max complex fields (6 per solve) per tree pass; 0 = auto */
+  int32_t use_graph;     /* 1: capture wf_solve into a hipGraph and replay it when the same
+                            geometry, incidences, grid and outputs come back (needs a
+                            non-null stream; the first call runs eagerly, then captures) */
 } wf_opts;
 
 typedef struct wf_handle wf_handle;

@@ -75,6 +75,7 @@ class Opts(ct.Structure):
         ("workspace_cap_bytes", ct.c_int64),
         ("cancel_flag", ct.POINTER(ct.c_int32)),
         ("field_chunk", ct.c_int32),
+        ("use_graph", ct.c_int32),
     ]
 
 

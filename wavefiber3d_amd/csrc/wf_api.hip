@@ -97,6 +97,16 @@ struct wf_handle {
   int ev_t0 = -1, ev_t1 = -1;
   bool stats_final = true;
   wf_stats stats;
+  // hipGraph replay of wf_solve (opts.use_graph)
+  bool capturing = false;
+  uint64_t geom_gen = 0;
+  struct GraphCache {
+    bool valid = false;
+    uint64_t sig = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } gcache;
+  int32_t leaf_variant = -1;  // autotuned leaf kernel variant (-1: not tuned)
 };
 
 namespace {
@@ -108,6 +118,7 @@ wf_status dalloc(wf_handle* h, const char* name, size_t bytes, void** out) {
     *out = b.p;
     return WF_OK;
   }
+  h->gcache.valid = false;  // a captured graph may hold the old address
   if (b.p) {
     hipFree(b.p);
     h->bytes_in_use -= b.cap;
@@ -145,12 +156,13 @@ bool factor_len(int n, FftPlanDev& p) {
   if (n < 1) return false;
   p.n = n;
   p.nstages = 0;
+  p.radix_packed = 0;
   int m = n;
   const int rad[] = {4, 2, 3, 5, 7, 11, 13};
   for (int r : rad) {
     while (m % r == 0) {
       if (p.nstages >= WF_FFT_MAX_STAGES) return false;
-      p.radix[p.nstages++] = (int8_t)r;
+      p.radix_packed |= (uint64_t)r << (4 * p.nstages++);
       m /= r;
     }
   }
@@ -162,8 +174,15 @@ bool is_smooth(int n) {
   return factor_len(n, p);
 }
 
+// Internal level grids use 5-smooth lengths (radix 2, 3, 4, 5 butterflies);
+// 7, 11 and 13 are only needed for caller-chosen output grids.
+bool is_smooth5(int n) {
+  for (int r : {2, 3, 5})
+    while (n > 1 && n % r == 0) n /= r;
+  return n == 1;
+}
 int smooth_up(int n) {
-  while (!is_smooth(n)) ++n;
+  while (!is_smooth5(n)) ++n;
   return n;
 }
 int smooth_even_up(int n) {
@@ -250,7 +269,7 @@ wf_status check_inc(const wf_incidence* inc, int n_inc) {
 }
 
 wf_status upload_incs(wf_handle* h, const wf_incidence* inc, int n_inc, WfIncDev** dev,
-                      hipStream_t st) {
+                      hipStream_t st, const char* buf = "inc") {
   if (h->inc_host_cap < n_inc) {
     if (h->inc_host) {
       hipEventSynchronize(h->inc_evt);
@@ -262,7 +281,7 @@ wf_status upload_incs(wf_handle* h, const wf_incidence* inc, int n_inc, WfIncDev
     WF_HIP(hipEventSynchronize(h->inc_evt));
   }
   for (int i = 0; i < n_inc; ++i) h->inc_host[i] = make_inc(inc[i]);
-  WF_TRY(dalloc_t(h, "inc", n_inc, dev));
+  WF_TRY(dalloc_t(h, buf, n_inc, dev));
   WF_HIP(hipMemcpyAsync(*dev, h->inc_host, sizeof(WfIncDev) * n_inc, hipMemcpyHostToDevice, st));
   WF_HIP(hipEventRecord(h->inc_evt, st));
   return WF_OK;
@@ -302,6 +321,10 @@ wf_status set_device(wf_handle* h) {
 
 // ---------------------------------------------------------------- timing --
 wf_status prof_event(wf_handle* h, hipStream_t st, int* idx) {
+  if (h->capturing) {
+    *idx = -1;
+    return WF_OK;
+  }
   if (h->ev_used == (int)h->evpool.size()) {
     hipEvent_t e;
     WF_HIP(hipEventCreate(&e));
@@ -323,6 +346,12 @@ void prof_reset(wf_handle* h) {
 // Runs `launch` (returning hipError_t) between two events and books it.
 template <typename Fn>
 wf_status timed(wf_handle* h, hipStream_t st, int kind, double work, Fn&& launch) {
+  if (h->capturing) {  // graph capture: no timing events inside the graph
+    hipError_t e = launch();
+    if (e != hipSuccess)
+      return set_err(WF_ERR_HIP, "kernel launch (kind %d) failed: %s", kind, hipGetErrorString(e));
+    return WF_OK;
+  }
   int e0, e1;
   WF_TRY(prof_event(h, st, &e0));
   hipError_t e = launch();
@@ -374,6 +403,7 @@ void wf_default_opts(wf_opts* o) {
   o->workspace_cap_bytes = 0;
   o->cancel_flag = nullptr;
   o->field_chunk = 0;
+  o->use_graph = 0;
 }
 
 wf_status wf_create(wf_handle** out, const wf_opts* opts) {
@@ -435,6 +465,8 @@ void wf_destroy(wf_handle* h) {
   if (h->inc_host) hipHostFree(h->inc_host);
   if (h->inc_evt) hipEventDestroy(h->inc_evt);
   for (auto ev : h->evpool) hipEventDestroy(ev);
+  if (h->gcache.exec) hipGraphExecDestroy(h->gcache.exec);
+  if (h->gcache.graph) hipGraphDestroy(h->gcache.graph);
   delete h;
 }
 
@@ -569,6 +601,7 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
                             h->levels[l].cb, h->levels[l].ce, st));
   h->n = n;
   h->depth = depth;
+  h->geom_gen++;
   h->pts = pts;
   h->perm = idx;
   h->keys = keys;
@@ -837,12 +870,24 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
     const double planes = (double)nchild * F;
     float2* G = (l == 0) ? tb.G0 : tb.G;
-    WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
-      return wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gcg.nt, gcg.np, gp.np, pc, pp, st);
-    }));
     const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
     const int mirror = mirror_of(gp.nt, off_p);
     const float2* Tl = gc.T + gc.toff[l];
+    if (l > 0 && wf::fused_fits(gcg.nt, gcg.np, gp.nt, gp.np)) {
+      // one kernel: child read once, parent written once
+      float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
+      WF_TRY(timed(h, st, WF_K_THETA,
+                   8.0 * F * ((double)nchild * gcg.nt * gcg.np + (double)npar * gp.nt * gp.np), [&] {
+        return wf::launch_agg_fused(cur, nxt, F, gcg.nt, gcg.np, gp.nt, gp.np, npar,
+                                    h->levels[l].cb, h->levels[l].ce, h->levels[l + 1].key, Tl,
+                                    off_p - off_c, mirror, pc, pp, tc, tp, st);
+      }));
+      cur = nxt;
+      continue;
+    }
+    WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
+      return wf::launch_phi_pass(cur, G, (int64_t)nchild * F, gcg.nt, gcg.np, gp.np, pc, pp, st);
+    }));
     if (l == 0) {
       WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * NP + 16.0 * F * D, [&] {
         return wf::launch_theta_pass(G, F, gcg.nt, NP, NT, 1, h->levels[0].cb, h->levels[0].ce,
@@ -925,49 +970,16 @@ wf_status backward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const
   return WF_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
-                   float* intensity, float* E_cplx, int32_t out_memspace, void* stream) {
-  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
-  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "wf_solve: no geometry (call wf_set_geometry)");
-  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
-  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
-  if (out_memspace != WF_MEM_HOST && out_memspace != WF_MEM_DEVICE)
-    return set_err(WF_ERR_INVALID_ARG, "out_memspace unknown");
-  WF_TRY(check_inc(inc, n_inc));
-  WF_TRY(set_device(h));
-  hipStream_t st = as_stream(stream);
+// The launch sequence of one wf_solve (replayable: it allocates only on first
+// use and records nothing but kernels while a graph is being captured).
+wf_status solve_core(wf_handle* h, hipStream_t st, const wf_incidence* inc, int n_inc, int NT,
+                     int NP, const WfIncDev* dinc, const int* order, float* I_dev, float* E_dev) {
   const int64_t n = h->n;
   const int depth = h->depth;
-  const int NT = grid.n_theta, NP = grid.n_phi;
   const int64_t D = (int64_t)NT * NP;
-  if (depth > 0) {
-    if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
-    if (!is_smooth(NP) || !is_smooth(2 * NT))
-      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
-                     NT, NP);
-  }
-  WfIncDev* dinc = nullptr;
-  WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
-  float* I_dev = intensity;
-  float* E_dev = E_cplx;
-  if (out_memspace == WF_MEM_HOST) {
-    WF_TRY(dalloc_t(h, "out_I", (size_t)n_inc * D, &I_dev));
-    if (E_cplx) WF_TRY(dalloc_t(h, "out_E", (size_t)n_inc * 6 * D, &E_dev));
-  }
   const float* px = h->pts;
-  std::vector<int> order(n_inc);
-  for (int i = 0; i < n_inc; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
   const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
                             0.5 * (h->box[2] + h->box[5])};
-  prof_reset(h);
-  h->stats.depth = depth;
-  WF_TRY(prof_event(h, st, &h->ev_t0));
   int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6) : 8;
   max_solves = std::min(max_solves, 8);
   if (depth == 0) max_solves = std::min(max_solves, 4);
@@ -1001,7 +1013,7 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
       } else {
         TreeBufs tb;
         WF_TRY(tree_buffers(h, gc, F, NP, tb));
-        WF_TRY(forward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, F0));
+        WF_TRY(forward_chunk(h, st, gc, dinc, order, c0, S, NT, NP, tb, F0));
       }
       for (int q = 0; q < S; ++q) {
         const int slot = order[c0 + q];
@@ -1014,7 +1026,114 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
     }
     i0 = i1;
   }
+  return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                   float* intensity, float* E_cplx, int32_t out_memspace, void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "wf_solve: no geometry (call wf_set_geometry)");
+  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
+  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
+  if (out_memspace != WF_MEM_HOST && out_memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "out_memspace unknown");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_TRY(set_device(h));
+  hipStream_t st = as_stream(stream);
+  const int depth = h->depth;
+  const int NT = grid.n_theta, NP = grid.n_phi;
+  const int64_t D = (int64_t)NT * NP;
+  if (depth > 0) {
+    if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
+    if (!is_smooth(NP) || !is_smooth(2 * NT))
+      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
+                     NT, NP);
+  }
+  float* I_dev = intensity;
+  float* E_dev = E_cplx;
+  if (out_memspace == WF_MEM_HOST) {
+    WF_TRY(dalloc_t(h, "out_I", (size_t)n_inc * D, &I_dev));
+    if (E_cplx) WF_TRY(dalloc_t(h, "out_E", (size_t)n_inc * 6 * D, &E_dev));
+  }
+  // graph replay: same geometry, incidences, grid and output buffers
+  const bool graph_ok = h->opts.use_graph && st != nullptr;
+  uint64_t sig = 1469598103934665603ULL;
+  auto mix = [&](const void* p, size_t nb) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < nb; ++i) sig = (sig ^ c[i]) * 1099511628211ULL;
+  };
+  if (graph_ok) {
+    mix(&h->geom_gen, sizeof h->geom_gen);
+    mix(&n_inc, sizeof n_inc);
+    mix(&grid, sizeof grid);
+    mix(inc, sizeof(wf_incidence) * n_inc);
+    mix(&I_dev, sizeof I_dev);
+    mix(&E_dev, sizeof E_dev);
+    if (h->gcache.valid && h->gcache.sig == sig) {
+      WfIncDev* gdinc = nullptr;
+      WF_TRY(upload_incs(h, inc, n_inc, &gdinc, st, "inc_graph"));
+      if (h->gcache.valid) {  // the upload did not reallocate
+        prof_reset(h);
+        h->stats_final = true;
+        WF_HIP(hipGraphLaunch(h->gcache.exec, st));
+        if (out_memspace == WF_MEM_HOST) {
+          WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
+          if (E_cplx)
+            WF_HIP(hipMemcpyAsync(E_cplx, E_dev, sizeof(float) * n_inc * 6 * D, hipMemcpyDeviceToHost, st));
+          WF_HIP(hipStreamSynchronize(st));
+        }
+        return WF_OK;
+      }
+    }
+  }
+  WfIncDev* dinc = nullptr;
+  WF_TRY(upload_incs(h, inc, n_inc, &dinc, st, graph_ok ? "inc_graph" : "inc"));
+  std::vector<int> order(n_inc);
+  for (int i = 0; i < n_inc; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
+  prof_reset(h);
+  h->stats.depth = depth;
+  WF_TRY(prof_event(h, st, &h->ev_t0));
+  WF_TRY(solve_core(h, st, inc, n_inc, NT, NP, dinc, order.data(), I_dev, E_dev));
   WF_TRY(prof_event(h, st, &h->ev_t1));
+  if (graph_ok) {
+    // capture the same launch sequence (every buffer now exists) for replays
+    if (h->gcache.exec) {
+      hipGraphExecDestroy(h->gcache.exec);
+      h->gcache.exec = nullptr;
+    }
+    if (h->gcache.graph) {
+      hipGraphDestroy(h->gcache.graph);
+      h->gcache.graph = nullptr;
+    }
+    h->gcache.valid = false;
+    WF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    h->capturing = true;
+    const wf_status cs = solve_core(h, st, inc, n_inc, NT, NP, dinc, order.data(), I_dev, E_dev);
+    h->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &g);
+    if (cs != WF_OK) {
+      if (g) hipGraphDestroy(g);
+      return cs;
+    }
+    if (ce != hipSuccess) return set_err(WF_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      hipGraphDestroy(g);
+      return set_err(WF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    }
+    h->gcache.graph = g;
+    h->gcache.exec = ex;
+    h->gcache.sig = sig;
+    h->gcache.valid = true;
+  }
   if (out_memspace == WF_MEM_HOST) {
     WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
     if (E_cplx)

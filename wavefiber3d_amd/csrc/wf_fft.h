@@ -13,14 +13,20 @@
 
 #include "wf_common.h"
 
-#define WF_FFT_MAX_STAGES 20
+#define WF_FFT_MAX_STAGES 16
 
+// Radices are packed 4 bits per stage (stage s at bits 4s..4s+3) so that the
+// schedule stays in scalar registers (no indexed private array).
 struct FftPlanDev {
   int n;
   int nstages;
-  int8_t radix[WF_FFT_MAX_STAGES];
+  uint64_t radix_packed;
   const float2* tw;  // n entries, exp(-2 pi i m/n)
 };
+
+__host__ __device__ inline int fft_radix(const FftPlanDev& p, int s) {
+  return (int)((p.radix_packed >> (4 * s)) & 15u);
+}
 
 // Twiddle exp(sign * 2 pi i m/n) with sign = -1 (forward) or +1 (inverse).
 __device__ inline float2 fft_tw(const float2* tw, int m, int inverse) {
@@ -90,7 +96,7 @@ __device__ inline float2* fft_lines(float2* a, float2* b, int nlines, int lstrid
                                     const FftPlanDev& p, int inverse) {
   int Ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
-    const int R = p.radix[s];
+    const int R = fft_radix(p, s);
     switch (R) {
       case 2: fft_stage<2>(a, b, p.n, nlines, lstride, Ns, p.tw, inverse); break;
       case 3: fft_stage<3>(a, b, p.n, nlines, lstride, Ns, p.tw, inverse); break;

@@ -20,6 +20,15 @@ hipError_t launch_theta_pass(const float2* G, int F, int ntc, int npp, int ntp, 
                              const FftPlanDev& pc, const FftPlanDev& pp, float2* out,
                              double2* out_root, hipStream_t st);
 
+// Fused level transition (phi + theta + translation + child sum) for grids
+// whose phi-interpolated child and batch buffers fit in LDS.
+bool fused_fits(int ntc, int npc, int ntp, int npp);
+hipError_t launch_agg_fused(const float2* child, float2* parent, int F, int ntc, int npc, int ntp,
+                            int npp, int nparents, const int32_t* cb, const int32_t* ce,
+                            const uint32_t* child_key, const float2* T, double shift,
+                            int mirror_base, const FftPlanDev& fpc, const FftPlanDev& fpp,
+                            const FftPlanDev& ftc, const FftPlanDev& ftp, hipStream_t st);
+
 hipError_t launch_trans_table(float2* T, int ntp, int npp, double theta0, const double h[3],
                               double k0, hipStream_t st);
 

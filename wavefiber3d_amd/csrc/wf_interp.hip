@@ -19,6 +19,8 @@
 // The translation tables T[octant][theta][phi] are built per level and
 // wavelength on the device with the phase formed in FP64 and reduced exactly.
 // The root level (the output grid) accumulates in FP64.
+#include <algorithm>
+
 #include "wf_common.h"
 #include "wf_fft.h"
 #include "wf_interp.h"
@@ -60,7 +62,9 @@ __global__ __launch_bounds__(256) void phi_pass_kernel(const float2* __restrict_
 // G: [child node][F][ntc][npp] (phi-interpolated children)
 // parent out (float, levels >= 1): [parent][F][ntp][npp]
 // root out (double, level 0): [F][ntp][npp]
-template <bool ROOT>
+// The parent tile (W column pairs x 2 ntp) is accumulated in registers: thread
+// t owns slots t + 256 m (m < MAXS) of the tile, so no LDS is spent on it.
+template <bool ROOT, int MAXS>
 __global__ __launch_bounds__(256) void theta_pass_kernel(
     const float2* __restrict__ G, int F, int ntc, int npp, int ntp, int W,
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
@@ -72,8 +76,6 @@ __global__ __launch_bounds__(256) void theta_pass_kernel(
   const int lmax = n2c > n2p ? n2c : n2p;
   float2* a = sm;
   float2* b = sm + W * lmax;
-  float2* accf = sm + 2 * W * lmax;                          // W x 2 ntp (float)
-  double2* accd = reinterpret_cast<double2*>(sm + 2 * W * lmax);  // (double, ROOT)
   const int half = npp / 2;
   const int ntiles = (half + W - 1) / W;
   const int tile = (int)(blockIdx.x % ntiles);
@@ -82,21 +84,22 @@ __global__ __launch_bounds__(256) void theta_pass_kernel(
   const int p = (int)(pf / F);
   const int k0 = tile * W;
   const int wcnt = min(W, half - k0);
-  for (int t = threadIdx.x; t < W * n2p; t += blockDim.x) {
-    if (ROOT) accd[t] = make_double2(0.0, 0.0);
-    else accf[t] = make_float2(0.f, 0.f);
-  }
+  const int nslot = wcnt * n2p;
+  float2 accf[MAXS];
+  double2 accd[ROOT ? MAXS : 1];
+#pragma unroll
+  for (int m = 0; m < MAXS; ++m) accf[m] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int m = 0; m < (ROOT ? MAXS : 1); ++m) accd[m] = make_double2(0.0, 0.0);
   const int c_lo = cb ? cb[p] : p, c_hi = ce ? ce[p] : p + 1;
   for (int c = c_lo; c < c_hi; ++c) {
     __syncthreads();
     const float2* g = G + ((int64_t)c * F + f) * ntc * (int64_t)npp;
     // extended great-circle lines: e[j] = G[j][k], e[ntc + j] = G[ntc-1-j][k + npp/2]
-    for (int t = threadIdx.x; t < ntc * W; t += blockDim.x) {
-      const int j = t / W, w = t - j * W;
-      if (w < wcnt) {
-        a[w * lmax + j] = g[(int64_t)j * npp + k0 + w];
-        a[w * lmax + ntc + (ntc - 1 - j)] = g[(int64_t)j * npp + k0 + w + half];
-      }
+    for (int t = threadIdx.x; t < ntc * wcnt; t += blockDim.x) {
+      const int j = t / wcnt, w = t - j * wcnt;
+      a[w * lmax + j] = g[(int64_t)j * npp + k0 + w];
+      a[w * lmax + ntc + (ntc - 1 - j)] = g[(int64_t)j * npp + k0 + w + half];
     }
     __syncthreads();
     float2* X = fft_lines(a, b, wcnt, lmax, pc, 0);
@@ -106,40 +109,138 @@ __global__ __launch_bounds__(256) void theta_pass_kernel(
     float2* Z = fft_lines(Y, X, wcnt, lmax, pp, 1);
     const int oct = child_key ? (int)(child_key[c] & 7u) : 0;
     const float2* Tt = T ? T + (int64_t)oct * ntp * npp : nullptr;
-    // accumulate: slot (w, i) for column k0+w, slot (w, ntp + i) for column k0+w+half
-    for (int t = threadIdx.x; t < wcnt * n2p; t += blockDim.x) {
-      const int w = t / n2p, s = t - w * n2p;
-      int i, col, e;
-      if (s < ntp) {
-        i = s;
-        col = k0 + w;
-        e = i;
-      } else {
-        i = s - ntp;
-        col = k0 + w + half;
-        e = mirror_base - i;
-        e %= n2p;
-        if (e < 0) e += n2p;
-      }
-      float2 v = Z[w * lmax + e];
-      if (Tt) v = cmul(v, Tt[(int64_t)i * npp + col]);
-      if (ROOT) {
-        accd[w * n2p + s].x += (double)v.x;
-        accd[w * n2p + s].y += (double)v.y;
-      } else {
-        accf[w * n2p + s] = cadd(accf[w * n2p + s], v);
+    // slot s = (w, q): q < ntp -> column k0+w, theta q; else column k0+w+half, theta q-ntp
+#pragma unroll
+    for (int m = 0; m < MAXS; ++m) {
+      const int s = threadIdx.x + 256 * m;
+      if (s < nslot) {
+        const int w = s / n2p, q = s - w * n2p;
+        int i, col, e;
+        if (q < ntp) {
+          i = q;
+          col = k0 + w;
+          e = i;
+        } else {
+          i = q - ntp;
+          col = k0 + w + half;
+          e = (mirror_base - i) % n2p;
+          if (e < 0) e += n2p;
+        }
+        float2 v = Z[w * lmax + e];
+        if (Tt) v = cmul(v, Tt[(int64_t)i * npp + col]);
+        if (ROOT) {
+          accd[m].x += (double)v.x;
+          accd[m].y += (double)v.y;
+        } else {
+          accf[m] = cadd(accf[m], v);
+        }
       }
     }
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < wcnt * n2p; t += blockDim.x) {
-    const int w = t / n2p, s = t - w * n2p;
-    const int i = s < ntp ? s : s - ntp;
-    const int col = s < ntp ? k0 + w : k0 + w + half;
-    if (ROOT)
-      out_root[((int64_t)f * ntp + i) * npp + col] = accd[w * n2p + s];
-    else
-      out[(((int64_t)p * F + f) * ntp + i) * npp + col] = accf[w * n2p + s];
+#pragma unroll
+  for (int m = 0; m < MAXS; ++m) {
+    const int s = threadIdx.x + 256 * m;
+    if (s < nslot) {
+      const int w = s / n2p, q = s - w * n2p;
+      const int i = q < ntp ? q : q - ntp;
+      const int col = q < ntp ? k0 + w : k0 + w + half;
+      if (ROOT)
+        out_root[((int64_t)f * ntp + i) * npp + col] = accd[m];
+      else
+        out[(((int64_t)p * F + f) * ntp + i) * npp + col] = accf[m];
+    }
+  }
+}
+
+// Fused level transition for small grids: one block per (parent, field).  For
+// each active child: phi pass of all rows into G (kept in LDS), then theta
+// pass in batches of column pairs, translation and accumulation into the
+// parent held in registers; the parent pattern is written once and every
+// child pattern is read once (the compulsory traffic of the level).
+template <int MAXS>
+__global__ __launch_bounds__(256) void agg_fused_kernel(
+    const float2* __restrict__ child, float2* __restrict__ parent, int F, int ntc, int npc,
+    int ntp, int npp, int RB, int CB, const int32_t* __restrict__ cb,
+    const int32_t* __restrict__ ce, const uint32_t* __restrict__ child_key,
+    const float2* __restrict__ T, float shift, int mirror_base, FftPlanDev fpc, FftPlanDev fpp,
+    FftPlanDev ftc, FftPlanDev ftp) {
+  extern __shared__ float2 sm[];
+  const int n2c = 2 * ntc, n2p = 2 * ntp, half = npp / 2;
+  const int lphi = npc > npp ? npc : npp;
+  const int lth = n2c > n2p ? n2c : n2p;
+  const int buf = max(RB * lphi, CB * lth);
+  float2* Gs = sm;                      // ntc x npp
+  float2* a = sm + ntc * npp;
+  float2* b = a + buf;
+  const int f = (int)(blockIdx.x % F);
+  const int p = (int)(blockIdx.x / F);
+  const int npar = ntp * npp;
+  float2 acc[MAXS];
+#pragma unroll
+  for (int m = 0; m < MAXS; ++m) acc[m] = make_float2(0.f, 0.f);
+  for (int c = cb[p]; c < ce[p]; ++c) {
+    const float2* src = child + ((int64_t)c * F + f) * ntc * (int64_t)npc;
+    // phi pass, RB rows at a time
+    for (int r0 = 0; r0 < ntc; r0 += RB) {
+      const int nr = min(RB, ntc - r0);
+      __syncthreads();
+      for (int t = threadIdx.x; t < nr * npc; t += blockDim.x) {
+        const int r = t / npc, k = t - r * npc;
+        a[r * lphi + k] = src[(int64_t)(r0 + r) * npc + k];
+      }
+      __syncthreads();
+      float2* X = fft_lines(a, b, nr, lphi, fpc, 0);
+      float2* Y = (X == a) ? b : a;
+      spectral_resample(X, Y, nr, lphi, lphi, npc, npp, 0.f, 1.0f / npc);
+      __syncthreads();
+      float2* Z = fft_lines(Y, X, nr, lphi, fpp, 1);
+      for (int t = threadIdx.x; t < nr * npp; t += blockDim.x) {
+        const int r = t / npp, k = t - r * npp;
+        Gs[(r0 + r) * npp + k] = Z[r * lphi + k];
+      }
+    }
+    const int oct = (int)(child_key[c] & 7u);
+    const float2* Tt = T + (int64_t)oct * npar;
+    // theta pass, CB column pairs at a time
+    for (int k0 = 0; k0 < half; k0 += CB) {
+      const int nw = min(CB, half - k0);
+      __syncthreads();
+      for (int t = threadIdx.x; t < ntc * nw; t += blockDim.x) {
+        const int j = t / nw, w = t - j * nw;
+        a[w * lth + j] = Gs[j * npp + k0 + w];
+        a[w * lth + ntc + (ntc - 1 - j)] = Gs[j * npp + k0 + w + half];
+      }
+      __syncthreads();
+      float2* X = fft_lines(a, b, nw, lth, ftc, 0);
+      float2* Y = (X == a) ? b : a;
+      spectral_resample(X, Y, nw, lth, lth, n2c, n2p, shift, 1.0f / n2c);
+      __syncthreads();
+      float2* Z = fft_lines(Y, X, nw, lth, ftp, 1);
+      // parent slot s = i * npp + col owned by thread s mod 256
+#pragma unroll
+      for (int m = 0; m < MAXS; ++m) {
+        const int s = threadIdx.x + 256 * m;
+        if (s < npar) {
+          const int i = s / npp, col = s - i * npp;
+          int w = -1, e = 0;
+          if (col >= k0 && col < k0 + nw) {
+            w = col - k0;
+            e = i;
+          } else if (col >= k0 + half && col < k0 + half + nw) {
+            w = col - k0 - half;
+            e = (mirror_base - i) % n2p;
+            if (e < 0) e += n2p;
+          }
+          if (w >= 0) acc[m] = cadd(acc[m], cmul(Z[w * lth + e], Tt[s]));
+        }
+      }
+    }
+  }
+  float2* dst = parent + ((int64_t)p * F + f) * (int64_t)npar;
+#pragma unroll
+  for (int m = 0; m < MAXS; ++m) {
+    const int s = threadIdx.x + 256 * m;
+    if (s < npar) dst[s] = acc[m];
   }
 }
 
@@ -172,14 +273,15 @@ __global__ __launch_bounds__(256) void trans_table_kernel(float2* __restrict__ T
 
 namespace wf {
 
-// Dynamic LDS budget per block (bytes).
+// Dynamic LDS budget per block (bytes) and line elements per batch.
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr int kBatchElems = 3584;  // two ping-pong buffers of this many float2 = 56 KB
 
 hipError_t launch_phi_pass(const float2* in, float2* out, int64_t planes, int ntc, int npc, int npp,
                            const FftPlanDev& pc, const FftPlanDev& pp, hipStream_t st) {
   const int lmax = npc > npp ? npc : npp;
-  int rows = 8;
-  while (rows > 1 && (size_t)2 * rows * lmax * sizeof(float2) > kLdsBudget) rows >>= 1;
+  int rows = kBatchElems / lmax;
+  if (rows < 1) rows = 1;
   if (rows > ntc) rows = ntc;
   const size_t sh = (size_t)2 * rows * lmax * sizeof(float2);
   if (sh > kLdsBudget) return hipErrorInvalidValue;
@@ -193,6 +295,8 @@ hipError_t launch_phi_pass(const float2* in, float2* out, int64_t planes, int nt
   return hipGetLastError();
 }
 
+constexpr int kThetaSlots = 16;
+
 hipError_t launch_theta_pass(const float2* G, int F, int ntc, int npp, int ntp, int nparents,
                              const int32_t* cb, const int32_t* ce, const uint32_t* child_key,
                              const float2* T, double shift, int mirror_base,
@@ -200,27 +304,62 @@ hipError_t launch_theta_pass(const float2* G, int F, int ntc, int npp, int ntp, 
                              double2* out_root, hipStream_t st) {
   const int n2c = 2 * ntc, n2p = 2 * ntp;
   const int lmax = n2c > n2p ? n2c : n2p;
-  const size_t acc_elem = out_root ? sizeof(double2) : sizeof(float2);
-  int W = 16;
-  auto lds = [&](int w) { return (size_t)2 * w * lmax * sizeof(float2) + (size_t)w * n2p * acc_elem; };
-  while (W > 1 && lds(W) > kLdsBudget) W >>= 1;
+  int W = std::min(kBatchElems / lmax, (256 * kThetaSlots) / n2p);
+  if (W < 1) W = 1;
   if (W > npp / 2) W = npp / 2 > 0 ? npp / 2 : 1;
-  const size_t sh = lds(W);
+  if (W * n2p > 256 * kThetaSlots) return hipErrorInvalidValue;  // 2 ntp above 4096
+  const size_t sh = (size_t)2 * W * lmax * sizeof(float2);
   if (sh > kLdsBudget) return hipErrorInvalidValue;
   const int64_t nblk = (int64_t)((npp / 2 + W - 1) / W) * F * nparents;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk);
   if (out_root) {
-    hipFuncSetAttribute((const void*)theta_pass_kernel<true>,
+    hipFuncSetAttribute((const void*)theta_pass_kernel<true, kThetaSlots>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    hipLaunchKernelGGL(theta_pass_kernel<true>, grid, dim3(256), sh, st, G, F, ntc, npp, ntp, W, cb,
-                       ce, child_key, T, (float)shift, mirror_base, pc, pp, out, out_root);
+    hipLaunchKernelGGL((theta_pass_kernel<true, kThetaSlots>), grid, dim3(256), sh, st, G, F, ntc,
+                       npp, ntp, W, cb, ce, child_key, T, (float)shift, mirror_base, pc, pp, out,
+                       out_root);
   } else {
-    hipFuncSetAttribute((const void*)theta_pass_kernel<false>,
+    hipFuncSetAttribute((const void*)theta_pass_kernel<false, kThetaSlots>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    hipLaunchKernelGGL(theta_pass_kernel<false>, grid, dim3(256), sh, st, G, F, ntc, npp, ntp, W,
-                       cb, ce, child_key, T, (float)shift, mirror_base, pc, pp, out, out_root);
+    hipLaunchKernelGGL((theta_pass_kernel<false, kThetaSlots>), grid, dim3(256), sh, st, G, F,
+                       ntc, npp, ntp, W, cb, ce, child_key, T, (float)shift, mirror_base, pc, pp,
+                       out, out_root);
   }
+  return hipGetLastError();
+}
+
+constexpr int kFusedSlots = 32;
+
+bool fused_fits(int ntc, int npc, int ntp, int npp) {
+  if ((int64_t)ntp * npp > 256 * kFusedSlots) return false;
+  const int lphi = npc > npp ? npc : npp;
+  const int lth = 2 * ntc > 2 * ntp ? 2 * ntc : 2 * ntp;
+  const int RB = std::min(ntc, std::max(1, 2048 / lphi));
+  const int CB = std::min(npp / 2, std::max(1, 2048 / lth));
+  const size_t buf = (size_t)std::max(RB * lphi, CB * lth);
+  return ((size_t)ntc * npp + 2 * buf) * sizeof(float2) <= kLdsBudget;
+}
+
+hipError_t launch_agg_fused(const float2* child, float2* parent, int F, int ntc, int npc, int ntp,
+                            int npp, int nparents, const int32_t* cb, const int32_t* ce,
+                            const uint32_t* child_key, const float2* T, double shift,
+                            int mirror_base, const FftPlanDev& fpc, const FftPlanDev& fpp,
+                            const FftPlanDev& ftc, const FftPlanDev& ftp, hipStream_t st) {
+  if (!fused_fits(ntc, npc, ntp, npp)) return hipErrorInvalidValue;
+  const int lphi = npc > npp ? npc : npp;
+  const int lth = 2 * ntc > 2 * ntp ? 2 * ntc : 2 * ntp;
+  const int RB = std::min(ntc, std::max(1, 2048 / lphi));
+  const int CB = std::min(npp / 2, std::max(1, 2048 / lth));
+  const size_t buf = (size_t)std::max(RB * lphi, CB * lth);
+  const size_t sh = ((size_t)ntc * npp + 2 * buf) * sizeof(float2);
+  const int64_t nblk = (int64_t)nparents * F;
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipFuncSetAttribute((const void*)agg_fused_kernel<kFusedSlots>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  hipLaunchKernelGGL((agg_fused_kernel<kFusedSlots>), dim3((unsigned)nblk), dim3(256), sh, st,
+                     child, parent, F, ntc, npc, ntp, npp, RB, CB, cb, ce, child_key, T,
+                     (float)shift, mirror_base, fpc, fpp, ftc, ftp);
   return hipGetLastError();
 }
 

@@ -184,6 +184,150 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_kernel(wf::LeafArgs a, i
   }
 }
 
+// ------------------------------------------------------- BF16 x3, quads --
+// The half-sample theta grid is symmetric (theta_{nt-1-j} = -theta_j) and
+// np is even (phi_{k+np/2} = phi_k + pi), so directions come in quadruples
+//   r0 = (j, k), r1 = (j, k + np/2), r2 = (nt-1-j, k), r3 = (nt-1-j, k + np/2)
+// with phases A + B, -A + B, A - B, -A - B, A = k cos(theta)(x cos phi + y sin phi),
+// B = k z sin(theta).  Two exponentials per point, e^{i(A+B)} and e^{i(A-B)},
+// give all four (the other two are their conjugates).  Lane l owns quad
+// tile*16 + l%16; the four directions are its four MFMA B operands.
+template <int MT>
+__global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs a, int KS) {
+  extern __shared__ float4 smem4[];
+  float4* spt = smem4;
+  bf16x8* scur = reinterpret_cast<bf16x8*>(smem4 + KS * 16);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+  const int D = a.nt * a.np;
+  const int nth = (a.nt + 1) / 2, nph = a.np / 2;
+  const int nquad = nth * nph;
+  const int qtiles = (nquad + 15) / 16;
+  const int nsuper = (qtiles + 3) / 4;
+  const int leaf = (int)(blockIdx.x / nsuper), sup = (int)(blockIdx.x % nsuper);
+  const int b = a.leaf_begin[leaf], e = a.leaf_end[leaf];
+  double c[3];
+  leaf_centre(a.leaf_key[leaf], a.depth, a.box_lo, a.leaf_size, c);
+  const double k_rev = a.k0 / WF_TWO_PI;
+  const int quad = (sup * 4 + wave) * 16 + col;
+  const bool qvalid = quad < nquad;
+  const int j = qvalid ? quad / nph : 0, k = qvalid ? quad - (quad / nph) * nph : 0;
+  const bool mirror_row = (a.nt - 1 - j) != j;
+  int dd[4];
+  dd[0] = j * a.np + k;
+  dd[1] = j * a.np + k + nph;
+  dd[2] = (a.nt - 1 - j) * a.np + k;
+  dd[3] = (a.nt - 1 - j) * a.np + k + nph;
+  float ax = 0.f, ay = 0.f, bz = 0.f;  // A = ax x + ay y, B = bz z (revolutions per metre scaled)
+  if (qvalid) {
+    const double th = -WF_PI / 2 + WF_PI * (j + 0.5) / a.nt;
+    const double ph = WF_TWO_PI * k / a.np;
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    ax = (float)(ct * cp);
+    ay = (float)(ct * sp);
+    bz = (float)st;
+  }
+  floatx4 acc[4][MT];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[r][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int chunk = KS * 16;
+  for (int base = b; base < e; base += chunk) {
+    const int cnt = min(chunk, e - base);
+    const int ksteps = (cnt + 15) / 16;
+    __syncthreads();
+    stage_points(a, base, cnt, ksteps * 16, c, k_rev, spt);
+    for (int idx = threadIdx.x; idx < ksteps * MT * 64; idx += kThreads) {
+      const int ln = idx & 63, mt = (idx >> 6) % MT, ks = idx / (64 * MT);
+      const int cc = ln & 15, gg = ln >> 4;
+      const int f = mt * 8 + (cc >> 1), part = cc & 1;
+      bf16x8 hi, lo;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int p = ks * 16 + 4 * gg + q;
+        float2 C = make_float2(0.f, 0.f);
+        if (p < cnt && f < a.F) C = a.C[(int64_t)(base + p) * a.Ftot + a.f0 + f];
+        const float vr = part ? C.y : C.x;
+        const float vi = part ? C.x : -C.y;
+        const __bf16 hr = (__bf16)vr, hi_ = (__bf16)vi;
+        hi[2 * q] = hr;
+        hi[2 * q + 1] = hi_;
+        lo[2 * q] = (__bf16)(vr - (float)hr);
+        lo[2 * q + 1] = (__bf16)(vi - (float)hi_);
+      }
+      scur[(ks * MT + mt) * 128 + ln] = hi;
+      scur[(ks * MT + mt) * 128 + 64 + ln] = lo;
+    }
+    __syncthreads();
+    for (int ks = 0; ks < ksteps; ++ks) {
+      bf16x8 h0, l0, h2, l2;  // phases of r0 (A+B) and r2 (A-B)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 pt = spt[ks * 16 + 4 * g + q];
+        const float fa = rev_reduce(ax * pt.x + ay * pt.y);
+        const float fb = rev_reduce(bz * pt.z);
+        const float cA = __builtin_amdgcn_cosf(fa), sA = __builtin_amdgcn_sinf(fa);
+        const float cB = __builtin_amdgcn_cosf(fb), sB = __builtin_amdgcn_sinf(fb);
+        const float cc_ = cA * cB, ss = sA * sB, sc = sA * cB, cs = cA * sB;
+        const float p0c = cc_ - ss, p0s = sc + cs;  // e^{i(A+B)}
+        const float p2c = cc_ + ss, p2s = sc - cs;  // e^{i(A-B)}
+        const __bf16 h0c = (__bf16)p0c, h0s = (__bf16)p0s, h2c = (__bf16)p2c, h2s = (__bf16)p2s;
+        h0[2 * q] = h0c;
+        h0[2 * q + 1] = h0s;
+        h2[2 * q] = h2c;
+        h2[2 * q + 1] = h2s;
+        l0[2 * q] = (__bf16)(p0c - (float)h0c);
+        l0[2 * q + 1] = (__bf16)(p0s - (float)h0s);
+        l2[2 * q] = (__bf16)(p2c - (float)h2c);
+        l2[2 * q + 1] = (__bf16)(p2s - (float)h2s);
+      }
+      // r1 = conj(r2), r3 = conj(r0): negate the sine entries (exact in bf16)
+      bf16x8 h1 = h2, l1 = l2, h3 = h0, l3 = l0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        h1[2 * q + 1] = -h2[2 * q + 1];
+        l1[2 * q + 1] = -l2[2 * q + 1];
+        h3[2 * q + 1] = -h0[2 * q + 1];
+        l3[2 * q + 1] = -l0[2 * q + 1];
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const bf16x8 ahi = scur[(ks * MT + m) * 128 + lane];
+        const bf16x8 alo = scur[(ks * MT + m) * 128 + 64 + lane];
+        acc[0][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, h0, acc[0][m], 0, 0, 0);
+        acc[0][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, l0, acc[0][m], 0, 0, 0);
+        acc[0][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, h0, acc[0][m], 0, 0, 0);
+        acc[1][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, h1, acc[1][m], 0, 0, 0);
+        acc[1][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, l1, acc[1][m], 0, 0, 0);
+        acc[1][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, h1, acc[1][m], 0, 0, 0);
+        acc[2][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, h2, acc[2][m], 0, 0, 0);
+        acc[2][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, l2, acc[2][m], 0, 0, 0);
+        acc[2][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, h2, acc[2][m], 0, 0, 0);
+        acc[3][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, h3, acc[3][m], 0, 0, 0);
+        acc[3][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, l3, acc[3][m], 0, 0, 0);
+        acc[3][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, h3, acc[3][m], 0, 0, 0);
+      }
+    }
+  }
+  if (!qvalid) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (r >= 2 && !mirror_row) break;  // middle row of an odd grid: r2, r3 repeat r0, r1
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int f = m * 8 + 2 * g + hh;
+        if (f < a.F)
+          a.out[((int64_t)leaf * a.F + f) * D + dd[r]] =
+              make_float2(acc[r][m][2 * hh], acc[r][m][2 * hh + 1]);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------- FP32 --
 template <int MT, int R>
 __global__ __launch_bounds__(kThreads) void leaf_f32_kernel(wf::LeafArgs a, int KS) {
@@ -268,6 +412,19 @@ template <int MT>
 hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
   constexpr int R = (MT <= 2) ? 4 : (MT <= 4 ? 2 : 1);
   const int D = a.nt * a.np;
+  if (bf16 && MT <= 3 && (a.np % 2) == 0) {
+    const int nquad = ((a.nt + 1) / 2) * (a.np / 2);
+    const int nsuper = ((nquad + 15) / 16 + 3) / 4;
+    const int64_t nblk = (int64_t)nsuper * a.nleaves;
+    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    const size_t per_ks = (size_t)MT * 128 * sizeof(bf16x8);
+    int KS = (int)((48 * 1024) / per_ks);
+    KS = KS < 1 ? 1 : (KS > 8 ? 8 : KS);
+    const size_t sh = (size_t)KS * 16 * sizeof(float4) + (size_t)KS * per_ks;
+    hipLaunchKernelGGL((leaf_bf16x3_quad_kernel<MT>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
+                       a, KS);
+    return hipGetLastError();
+  }
   const int dtiles = (D + 15) / 16;
   const int nsuper = (dtiles + 4 * R - 1) / (4 * R);
   const int64_t nblk = (int64_t)nsuper * a.nleaves;

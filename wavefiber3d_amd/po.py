@@ -101,7 +101,8 @@ class POSolver:
     def __init__(self, device: int = -1, precision: str = "bf16x3", beta: float = 4.0,
                  box=None, box_enlarge: float = 1.1, lambda_sample_m: float = 0.0,
                  strict_grid: bool = False, workspace_cap_bytes: int = 0,
-                 field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None):
+                 field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None,
+                 use_graph: bool = False):
         L = lib()
         o = _lib.Opts()
         L.wf_default_opts(ct.byref(o))
@@ -114,6 +115,7 @@ class POSolver:
         o.strict_grid = int(strict_grid)
         o.workspace_cap_bytes = workspace_cap_bytes
         o.field_chunk = field_chunk
+        o.use_graph = int(use_graph)
         if box is not None:
             o.has_box = 1
             for i in range(6):
