@@ -36,7 +36,12 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 1
+#define WF_ABI_VERSION 2
+
+/* depth argument of wf_set_geometry: build the octree to the deepest useful
+ * depth (about log8 n) and let each wavelength group of a solve use the
+ * depth that the cost model (wf_tune) rates fastest. */
+#define WF_DEPTH_AUTO (-1)
 
 typedef enum wf_status {
   WF_OK = 0,
@@ -144,7 +149,8 @@ void wf_destroy(wf_handle* h);
  * build_octree_kernel<256> at util/tree.h:289-715, the active lists and
  * per-point centres).  The points are validated (finite, unit-ish normals,
  * non-negative area), Morton-keyed with the reference's float split planes,
- * stably sorted and cut into per-level active-node lists.  depth in [0, 10]. */
+ * stably sorted and cut into per-level active-node lists.  depth in [0, 10]
+ * or WF_DEPTH_AUTO. */
 wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void* stream);
 
 /* PO far-field solve on the current geometry (the treePO.cu:501-961 loop body:
@@ -207,8 +213,79 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
                     const float* grad_I, float* intensity, float* grad_x, float* grad_n,
                     float* grad_a, int32_t fp64, int32_t memspace, void* stream);
 
+/* Current geometry: point count, built octree depth (depth_max of a
+ * WF_DEPTH_AUTO build) and root box {min x, y, z, max x, y, z}; any may be null. */
+wf_status wf_geometry_info(wf_handle* h, int64_t* n, int32_t* depth_built, double* box6);
+
 /* Statistics of the last wf_solve on this handle. */
 wf_status wf_get_stats(wf_handle* h, wf_stats* out);
+
+/* ---------------------------------------------------------------------------
+ * Depth and kernel autotuning (no reference counterpart: the reference takes
+ * the depth from its command line, treePO.cu:27-42, and has one leaf kernel).
+ *
+ * A WF_DEPTH_AUTO geometry is built once to depth_max; a solve of one
+ * wavelength group on depth d < depth_max uses levels 0..d of the same tree
+ * (the level lists and keys of a depth-d build are prefixes of the deeper
+ * one).  The depth minimises
+ *   leaf_s_per_pair * n * Nt_d * Np_d
+ *   + agg_s_per_byte * sum_{l<d} 48 (nodes_{l+1} Nt_{l+1} Np_{l+1} + nodes_l Nt_l Np_l)
+ *   + level_s * d
+ * per solve.  wf_autotune fits the three coefficients from timed solves of
+ * the current geometry at the model's best candidate depths, picks the faster
+ * of (quad-symmetric BF16x3 leaf kernel, 4 solves per pass) and (plain
+ * kernel, 8 solves per pass), and stores the result in the handle;
+ * wf_get_tune/wf_set_tune move a calibration between handles.  On a geometry
+ * of fixed depth the solves keep that depth (best_depth is advisory). */
+typedef struct wf_tune {
+  double leaf_s_per_pair; /* leaf sum, per (point, leaf-grid direction, solve) */
+  double agg_s_per_byte;  /* aggregation, per compulsory HBM byte (complex64 patterns) */
+  double level_s;         /* fixed cost per level transition and solve */
+  int32_t leaf_quad;      /* 1: quad-symmetric BF16x3 leaf kernel where eligible (<= 4
+                             solves per tree pass); 0: plain */
+  int32_t chunk_solves;   /* solves per tree pass (1..8); opts.field_chunk overrides */
+  int32_t calibrated;     /* 1 after wf_autotune (or a wf_set_tune of a calibrated tune) */
+} wf_tune;
+
+wf_status wf_get_tune(wf_handle* h, wf_tune* out);
+wf_status wf_set_tune(wf_handle* h, const wf_tune* t);
+
+/* Times solves of `inc` (its first wavelength group, at most 8 solves) on the
+ * current geometry at up to three candidate depths <= the built depth, fits
+ * the cost model and the leaf kernel choice, and returns the measured fastest
+ * depth in *best_depth (nullable).  Synchronous. */
+wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                      int32_t* best_depth, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched solves over many surfaces (no reference counterpart: the reference
+ * runs one fiber segment per process, treePO.cu:21-992).  Every segment keeps
+ * its own cached octree; solves run on n_streams worker streams, each with
+ * its own workspace, and each (segment, stream) pair keeps its captured
+ * hipGraph when opts.use_graph is set, so repeated wf_batch_solve calls with
+ * the same incidences replay.  Segments are solved largest first. */
+typedef struct wf_batch wf_batch;
+
+wf_status wf_batch_create(wf_batch** out, const wf_opts* opts, int32_t n_streams);
+void wf_batch_destroy(wf_batch* b);
+
+/* Uploads a segment and builds its octree (depth in [0, 10] or WF_DEPTH_AUTO);
+ * *index (nullable) receives its position in the batch output. */
+wf_status wf_batch_add(wf_batch* b, const wf_surface* s, int32_t depth, int32_t* index);
+
+/* Solves every segment for the same incidences and grid.  intensity is
+ * [n_segments][n_inc][Nt][Np] float32 in out_memspace.  WF_DEPTH_AUTO
+ * segments are tuned first (wf_autotune on the largest one) unless the batch
+ * already holds a calibration.  WF_MEM_HOST: returns with the results in host
+ * memory.  WF_MEM_DEVICE: asynchronous; `stream` is made to wait for the
+ * worker streams, which first wait for `stream`'s prior work. */
+wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                         float* intensity, int32_t out_memspace, void* stream);
+
+/* Per-segment chosen depth (last wavelength group) and solve time (ms) of the
+ * last wf_batch_solve; arrays of length n_segments, nullable. */
+wf_status wf_batch_info(wf_batch* b, int32_t* n_segments, int32_t* depth, double* ms,
+                        int64_t* n_points);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_default_opts():
     L = wf.lib()
-    assert L.wf_abi_version() == 1
+    assert L.wf_abi_version() == 2
     o = _lib.Opts()
     L.wf_default_opts(ct.byref(o))
     assert o.struct_size == ct.sizeof(_lib.Opts)
@@ -72,6 +72,27 @@ def test_null_handle_paths():
     assert L.wf_set_geometry(None, None, 0, None) == 1
     assert L.wf_get_stats(None, None) == 1
     L.wf_destroy(None)  # no-op
+    t = _lib.Tune()
+    assert L.wf_get_tune(None, ct.byref(t)) == 1
+    assert L.wf_set_tune(None, ct.byref(t)) == 1
+    assert L.wf_autotune(None, None, 0, g, None, None) == 1
+    assert L.wf_geometry_info(None, None, None, None) == 1
+    assert L.wf_batch_add(None, None, 0, None) == 1
+    assert L.wf_batch_solve(None, None, 0, g, None, 0, None) == 1
+    assert L.wf_batch_info(None, None, None, None, None) == 1
+    L.wf_batch_destroy(None)  # no-op
+
+
+def test_batch_create_arguments():
+    L = wf.lib()
+    b = ct.c_void_p()
+    assert L.wf_batch_create(None, None, 1) == 1
+    assert L.wf_batch_create(ct.byref(b), None, 0) == 1
+    assert b"n_streams" in L.wf_last_error()
+    assert L.wf_batch_create(ct.byref(b), None, 17) == 1
+    if _no_gpu():
+        assert L.wf_batch_create(ct.byref(b), None, 2) == 4
+        assert not b.value
 
 
 def test_reference_parameter_grids():

@@ -7,6 +7,7 @@ gfx950 behind the C-ABI in include/wf_po.h (libwfpo.so).
 from ._lib import WfError, lib  # noqa: F401
 from .geometry import SurfaceSamples, fibonacci_sphere, fiber_segment, read_soa_mesh  # noqa: F401
 from .po import (  # noqa: F401
+    Batch,
     Incidence,
     POSolver,
     output_directions,
@@ -17,6 +18,6 @@ from .po import (  # noqa: F401
 
 __all__ = [
     "WfError", "lib", "SurfaceSamples", "fibonacci_sphere", "fiber_segment", "read_soa_mesh",
-    "Incidence", "POSolver", "output_directions", "reference_incidence_phi",
+    "Batch", "Incidence", "POSolver", "output_directions", "reference_incidence_phi",
     "reference_incidence_theta", "reference_lambda",
 ]

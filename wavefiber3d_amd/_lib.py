@@ -28,8 +28,11 @@ WF_PREC_FP32, WF_PREC_BF16X3, WF_PREC_FP32_VALU = 0, 1, 2
 EXPORTS = (
     "wf_abi_version", "wf_last_error", "wf_default_opts", "wf_create", "wf_destroy",
     "wf_set_geometry", "wf_solve", "wf_po_solve", "wf_po_solve_fp64_direct", "wf_currents",
-    "wf_octree_nodes", "wf_interp", "wf_get_stats", "wf_po_vjp",
+    "wf_octree_nodes", "wf_interp", "wf_get_stats", "wf_po_vjp", "wf_geometry_info",
+    "wf_get_tune", "wf_set_tune", "wf_autotune", "wf_batch_create", "wf_batch_destroy",
+    "wf_batch_add", "wf_batch_solve", "wf_batch_info",
 )
+WF_DEPTH_AUTO = -1
 
 
 class WfError(RuntimeError):
@@ -97,6 +100,17 @@ class Stats(ct.Structure):
     ]
 
 
+class Tune(ct.Structure):
+    _fields_ = [
+        ("leaf_s_per_pair", ct.c_double),
+        ("agg_s_per_byte", ct.c_double),
+        ("level_s", ct.c_double),
+        ("leaf_quad", ct.c_int32),
+        ("chunk_solves", ct.c_int32),
+        ("calibrated", ct.c_int32),
+    ]
+
+
 _lib = None
 
 
@@ -120,6 +134,7 @@ def lib(path: str | None = None):
     L.wf_default_opts.argtypes = [ct.POINTER(Opts)]
     L.wf_create.argtypes = [ct.POINTER(vp), ct.POINTER(Opts)]
     L.wf_destroy.argtypes = [vp]
+    L.wf_destroy.restype = None
     L.wf_set_geometry.argtypes = [vp, ct.POINTER(Surface), ct.c_int32, vp]
     L.wf_solve.argtypes = [vp, ct.POINTER(Incidence), ct.c_int32, Grid, vp, vp, ct.c_int32, vp]
     L.wf_po_solve.argtypes = [vp, ct.POINTER(Surface), ct.POINTER(Incidence), ct.c_int32, Grid,
@@ -134,8 +149,21 @@ def lib(path: str | None = None):
     L.wf_get_stats.argtypes = [vp, ct.POINTER(Stats)]
     L.wf_po_vjp.argtypes = [vp, ct.POINTER(Incidence), ct.c_int32, Grid, vp, vp, vp, vp, vp,
                             ct.c_int32, ct.c_int32, vp]
+    L.wf_geometry_info.argtypes = [vp, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int32),
+                                   ct.POINTER(ct.c_double)]
+    L.wf_get_tune.argtypes = [vp, ct.POINTER(Tune)]
+    L.wf_set_tune.argtypes = [vp, ct.POINTER(Tune)]
+    L.wf_autotune.argtypes = [vp, ct.POINTER(Incidence), ct.c_int32, Grid, ct.POINTER(ct.c_int32),
+                              vp]
+    L.wf_batch_create.argtypes = [ct.POINTER(vp), ct.POINTER(Opts), ct.c_int32]
+    L.wf_batch_destroy.argtypes = [vp]
+    L.wf_batch_destroy.restype = None
+    L.wf_batch_add.argtypes = [vp, ct.POINTER(Surface), ct.c_int32, ct.POINTER(ct.c_int32)]
+    L.wf_batch_solve.argtypes = [vp, ct.POINTER(Incidence), ct.c_int32, Grid, vp, ct.c_int32, vp]
+    L.wf_batch_info.argtypes = [vp, ct.POINTER(ct.c_int32), vp, vp, vp]
     for name in EXPORTS:
-        if name not in ("wf_abi_version", "wf_last_error", "wf_default_opts", "wf_destroy"):
+        if name not in ("wf_abi_version", "wf_last_error", "wf_default_opts", "wf_destroy",
+                        "wf_batch_destroy"):
             getattr(L, name).restype = ct.c_int
     if path is None:
         _lib = L

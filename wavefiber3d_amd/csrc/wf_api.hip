@@ -64,6 +64,23 @@ struct FftEntry {
   float2* tw = nullptr;
 };
 
+// A captured solve; valid while the workspace it was recorded on has not
+// moved (ws_gen) and the call signature matches.
+struct GraphCache {
+  bool valid = false;
+  uint64_t sig = 0;
+  uint64_t ws_gen = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+};
+
 }  // namespace
 
 struct wf_handle {
@@ -75,7 +92,11 @@ struct wf_handle {
   // geometry
   bool has_geom = false;
   int64_t n = 0;
-  int depth = 0;
+  int depth = 0;           // depth used by the current solve (<= depth_max)
+  int depth_max = 0;       // depth the octree was built to
+  bool auto_depth = false; // WF_DEPTH_AUTO: per-group depth from the cost model
+  int force_depth = -1;    // autotuning: depth forced for every group
+  wf_tune tune;
   float boxf[6] = {0, 0, 0, 0, 0, 0};
   double box[6] = {0, 0, 0, 0, 0, 0};
   float* pts = nullptr;     // sorted SoA: x y z nx ny nz area (7 n)
@@ -100,13 +121,8 @@ struct wf_handle {
   // hipGraph replay of wf_solve (opts.use_graph)
   bool capturing = false;
   uint64_t geom_gen = 0;
-  struct GraphCache {
-    bool valid = false;
-    uint64_t sig = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-  } gcache;
-  int32_t leaf_variant = -1;  // autotuned leaf kernel variant (-1: not tuned)
+  uint64_t ws_gen = 1;  // bumped whenever a workspace buffer moves
+  GraphCache gcache;
 };
 
 namespace {
@@ -118,7 +134,7 @@ wf_status dalloc(wf_handle* h, const char* name, size_t bytes, void** out) {
     *out = b.p;
     return WF_OK;
   }
-  h->gcache.valid = false;  // a captured graph may hold the old address
+  h->ws_gen++;  // captured graphs may hold the old address
   if (b.p) {
     hipFree(b.p);
     h->bytes_in_use -= b.cap;
@@ -314,6 +330,29 @@ Grid level_grid(const wf_handle* h, int level, double lambda_s) {
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Cost-model defaults (measured on MI355X with the BF16x3 quad leaf kernel
+// and the fused aggregation; wf_autotune refits them per geometry).
+void default_tune(wf_tune* t) {
+  t->leaf_s_per_pair = 6.0e-14;
+  t->agg_s_per_byte = 4.0e-13;
+  t->level_s = 2.0e-6;
+  t->leaf_quad = 1;
+  t->chunk_solves = 4;
+  t->calibrated = 0;
+}
+
+// Deepest useful level of a WF_DEPTH_AUTO build: about one octant digit per
+// factor 8 of points (ceil(log8 n)), at least 1, at most 10 (30-bit keys).
+int auto_depth_max(int64_t n) {
+  int d = 0;
+  double m = 1.0;
+  while (m < (double)n && d < 10) {
+    m *= 8.0;
+    ++d;
+  }
+  return std::max(1, d);
+}
+
 wf_status set_device(wf_handle* h) {
   if (h->device >= 0) WF_HIP(hipSetDevice(h->device));
   return WF_OK;
@@ -450,6 +489,7 @@ wf_status wf_create(wf_handle** out, const wf_opts* opts) {
     return set_err(WF_ERR_HIP, "hipEventCreate failed");
   }
   memset(&h->stats, 0, sizeof h->stats);
+  default_tune(&h->tune);
   *out = h;
   return WF_OK;
 }
@@ -465,8 +505,7 @@ void wf_destroy(wf_handle* h) {
   if (h->inc_host) hipHostFree(h->inc_host);
   if (h->inc_evt) hipEventDestroy(h->inc_evt);
   for (auto ev : h->evpool) hipEventDestroy(ev);
-  if (h->gcache.exec) hipGraphExecDestroy(h->gcache.exec);
-  if (h->gcache.graph) hipGraphDestroy(h->gcache.graph);
+  h->gcache.reset();
   delete h;
 }
 
@@ -475,7 +514,8 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
   if (!s) return set_err(WF_ERR_INVALID_ARG, "surface is null");
   if (s->n <= 0) return set_err(WF_ERR_INVALID_ARG, "surface.n must be > 0 (got %lld)", (long long)s->n);
   if (s->n > 0x7fffffffLL) return set_err(WF_ERR_INVALID_ARG, "surface.n too large");
-  if (depth < 0 || depth > 10) return set_err(WF_ERR_INVALID_ARG, "depth must be in [0, 10]");
+  if ((depth < 0 || depth > 10) && depth != WF_DEPTH_AUTO)
+    return set_err(WF_ERR_INVALID_ARG, "depth must be in [0, 10] or WF_DEPTH_AUTO");
   if (!s->x || !s->y || !s->z || !s->nx || !s->ny || !s->nz || !s->area)
     return set_err(WF_ERR_INVALID_ARG, "surface: null array pointer");
   if (s->memspace != WF_MEM_HOST && s->memspace != WF_MEM_DEVICE)
@@ -484,6 +524,8 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
   hipStream_t st = as_stream(stream);
   h->has_geom = false;
   const int64_t n = s->n;
+  const bool auto_depth = depth == WF_DEPTH_AUTO;
+  if (auto_depth) depth = auto_depth_max(n);
   // raw device copies (host input) or the caller's device arrays
   const float* src[7] = {s->x, s->y, s->z, s->nx, s->ny, s->nz, s->area};
   if (s->memspace == WF_MEM_HOST) {
@@ -601,6 +643,8 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
                             h->levels[l].cb, h->levels[l].ce, st));
   h->n = n;
   h->depth = depth;
+  h->depth_max = depth;
+  h->auto_depth = auto_depth;
   h->geom_gen++;
   h->pts = pts;
   h->perm = idx;
@@ -618,7 +662,7 @@ wf_status wf_octree_nodes(wf_handle* h, int32_t* perm, int32_t* node_begin, int3
   if (perm)
     WF_HIP(hipMemcpyAsync(perm, h->perm, sizeof(int32_t) * h->n, hipMemcpyDeviceToDevice, st));
   if (node_begin && node_end)
-    WF_HIP(wf::octree_node_ranges(h->keys, h->n, h->depth, node_begin, node_end, st));
+    WF_HIP(wf::octree_node_ranges(h->keys, h->n, h->depth_max, node_begin, node_end, st));
   return WF_OK;
 }
 
@@ -740,11 +784,55 @@ struct GroupCtx {
   std::vector<int64_t> toff;
 };
 
+// Output grids of a tree solve: even n_phi (theta extension), FFT lengths.
+wf_status check_tree_grid(wf_grid grid) {
+  const int NT = grid.n_theta, NP = grid.n_phi;
+  if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
+  if (!is_smooth(NP) || !is_smooth(2 * NT))
+    return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
+                   NT, NP);
+  return WF_OK;
+}
+
+double sample_lambda(const wf_handle* h, double lam) {
+  return h->opts.lambda_sample_m > 0 ? h->opts.lambda_sample_m : lam;
+}
+
+// Modelled seconds per solve of a depth-d tree pass (see wf_tune).
+double model_cost(const wf_handle* h, const wf_tune& t, int d, double lam_s, int NT, int NP) {
+  std::vector<Grid> g(d + 1, Grid{NT, NP});
+  for (int l = 1; l <= d; ++l) g[l] = level_grid(h, l, lam_s);
+  double bytes = 0;
+  for (int l = 0; l < d; ++l)
+    bytes += 48.0 * ((double)h->levels[l + 1].count * g[l + 1].nt * g[l + 1].np +
+                     (double)h->levels[l].count * g[l].nt * g[l].np);
+  return t.leaf_s_per_pair * (double)h->n * g[d].nt * g[d].np + t.agg_s_per_byte * bytes +
+         t.level_s * d;
+}
+
+// Depth of a wavelength group: the built depth, or for WF_DEPTH_AUTO
+// geometries the model's cheapest depth in [1, depth_max].
+int choose_depth(const wf_handle* h, double lam, int NT, int NP) {
+  if (h->force_depth >= 0) return h->force_depth;
+  if (!h->auto_depth) return h->depth_max;
+  const double lam_s = sample_lambda(h, lam);
+  int best = h->depth_max;
+  double cbest = 0;
+  for (int d = 1; d <= h->depth_max; ++d) {
+    const double c = model_cost(h, h->tune, d, lam_s, NT, NP);
+    if (d == 1 || c < cbest) {
+      cbest = c;
+      best = d;
+    }
+  }
+  return best;
+}
+
 wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP, GroupCtx& gc) {
   const int depth = h->depth;
   gc.lam = lam;
   gc.k0 = 2.0 * WF_PI / lam;
-  const double lam_s = h->opts.lambda_sample_m > 0 ? h->opts.lambda_sample_m : lam;
+  const double lam_s = sample_lambda(h, lam);
   gc.g.assign(depth + 1, Grid{NT, NP});
   gc.toff.assign(depth + 1, 0);
   if (depth == 0) return WF_OK;
@@ -812,6 +900,7 @@ wf::LeafArgs leaf_args(wf_handle* h, const GroupCtx& gc, const float2* C, int F,
   la.np = gc.g[depth].np;
   la.out = out;
   la.precision = h->opts.precision;
+  la.quad = h->tune.leaf_quad;
   return la;
 }
 
@@ -975,18 +1064,21 @@ wf_status backward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const
 wf_status solve_core(wf_handle* h, hipStream_t st, const wf_incidence* inc, int n_inc, int NT,
                      int NP, const WfIncDev* dinc, const int* order, float* I_dev, float* E_dev) {
   const int64_t n = h->n;
-  const int depth = h->depth;
   const int64_t D = (int64_t)NT * NP;
   const float* px = h->pts;
   const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
                             0.5 * (h->box[2] + h->box[5])};
-  int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6) : 8;
-  max_solves = std::min(max_solves, 8);
-  if (depth == 0) max_solves = std::min(max_solves, 4);
   const double epi_bytes = (96.0 + 4.0 + (E_dev ? 24.0 : 0.0)) * D;
   for (int i0 = 0; i0 < n_inc;) {
     int i1 = i0 + 1;
     while (i1 < n_inc && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
+    h->depth = choose_depth(h, inc[order[i0]].lambda_m, NT, NP);
+    h->stats.depth = h->depth;
+    const int depth = h->depth;
+    int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6)
+                     : (h->tune.chunk_solves > 0 ? h->tune.chunk_solves : 8);
+    max_solves = std::min(max_solves, 8);
+    if (depth == 0) max_solves = std::min(max_solves, 4);
     GroupCtx gc;
     WF_TRY(prepare_group(h, st, inc[order[i0]].lambda_m, NT, NP, gc));
     for (int c0 = i0; c0 < i1; c0 += max_solves) {
@@ -1044,15 +1136,9 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
   WF_TRY(check_inc(inc, n_inc));
   WF_TRY(set_device(h));
   hipStream_t st = as_stream(stream);
-  const int depth = h->depth;
   const int NT = grid.n_theta, NP = grid.n_phi;
   const int64_t D = (int64_t)NT * NP;
-  if (depth > 0) {
-    if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
-    if (!is_smooth(NP) || !is_smooth(2 * NT))
-      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
-                     NT, NP);
-  }
+  if (h->depth_max > 0) WF_TRY(check_tree_grid(grid));
   float* I_dev = intensity;
   float* E_dev = E_cplx;
   if (out_memspace == WF_MEM_HOST) {
@@ -1073,11 +1159,15 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
     mix(inc, sizeof(wf_incidence) * n_inc);
     mix(&I_dev, sizeof I_dev);
     mix(&E_dev, sizeof E_dev);
-    if (h->gcache.valid && h->gcache.sig == sig) {
+    mix(&h->tune, sizeof h->tune);
+    mix(&h->auto_depth, sizeof h->auto_depth);
+    if (h->gcache.valid && h->gcache.sig == sig && h->gcache.ws_gen == h->ws_gen) {
       WfIncDev* gdinc = nullptr;
       WF_TRY(upload_incs(h, inc, n_inc, &gdinc, st, "inc_graph"));
-      if (h->gcache.valid) {  // the upload did not reallocate
+      if (h->gcache.ws_gen == h->ws_gen) {  // the upload did not reallocate
+        const int d_last = h->stats.depth;
         prof_reset(h);
+        h->stats.depth = d_last;
         h->stats_final = true;
         WF_HIP(hipGraphLaunch(h->gcache.exec, st));
         if (out_memspace == WF_MEM_HOST) {
@@ -1097,21 +1187,13 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
   prof_reset(h);
-  h->stats.depth = depth;
   WF_TRY(prof_event(h, st, &h->ev_t0));
   WF_TRY(solve_core(h, st, inc, n_inc, NT, NP, dinc, order.data(), I_dev, E_dev));
   WF_TRY(prof_event(h, st, &h->ev_t1));
   if (graph_ok) {
     // capture the same launch sequence (every buffer now exists) for replays
-    if (h->gcache.exec) {
-      hipGraphExecDestroy(h->gcache.exec);
-      h->gcache.exec = nullptr;
-    }
-    if (h->gcache.graph) {
-      hipGraphDestroy(h->gcache.graph);
-      h->gcache.graph = nullptr;
-    }
-    h->gcache.valid = false;
+    h->gcache.reset();
+    const uint64_t ws_before = h->ws_gen;
     WF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     h->capturing = true;
     const wf_status cs = solve_core(h, st, inc, n_inc, NT, NP, dinc, order.data(), I_dev, E_dev);
@@ -1123,16 +1205,25 @@ wf_status wf_solve(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid
       return cs;
     }
     if (ce != hipSuccess) return set_err(WF_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    if (h->ws_gen != ws_before) {
+      // a buffer moved while recording: earlier nodes may hold its old
+      // address, so the graph is dropped (the eager results stand)
+      hipGraphDestroy(g);
+      g = nullptr;
+    }
     hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    const hipError_t ie = g ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) : hipSuccess;
     if (ie != hipSuccess) {
       hipGraphDestroy(g);
       return set_err(WF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
     }
-    h->gcache.graph = g;
-    h->gcache.exec = ex;
-    h->gcache.sig = sig;
-    h->gcache.valid = true;
+    if (g) {
+      h->gcache.graph = g;
+      h->gcache.exec = ex;
+      h->gcache.sig = sig;
+      h->gcache.ws_gen = h->ws_gen;
+      h->gcache.valid = true;
+    }
   }
   if (out_memspace == WF_MEM_HOST) {
     WF_HIP(hipMemcpyAsync(intensity, I_dev, sizeof(float) * n_inc * D, hipMemcpyDeviceToHost, st));
@@ -1156,16 +1247,10 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
   WF_TRY(set_device(h));
   hipStream_t st = as_stream(stream);
   const int64_t n = h->n;
-  const int depth = h->depth;
-  const bool direct = fp64 || depth == 0;
+  const bool direct = fp64 || h->depth_max == 0;
   const int NT = grid.n_theta, NP = grid.n_phi;
   const int64_t D = (int64_t)NT * NP;
-  if (!direct) {
-    if (NP % 2) return set_err(WF_ERR_INVALID_ARG, "tree solve needs an even n_phi (theta extension)");
-    if (!is_smooth(NP) || !is_smooth(2 * NT))
-      return set_err(WF_ERR_UNSUPPORTED, "output grid %dx%d: n_phi and 2 n_theta must be 13-smooth",
-                     NT, NP);
-  }
+  if (!direct) WF_TRY(check_tree_grid(grid));
   WfIncDev* dinc = nullptr;
   WF_TRY(upload_incs(h, inc, n_inc, &dinc, st));
   const float* gI = grad_I;
@@ -1191,7 +1276,6 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
   const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
                             0.5 * (h->box[2] + h->box[5])};
   prof_reset(h);
-  h->stats.depth = direct ? 0 : depth;
   WF_TRY(prof_event(h, st, &h->ev_t0));
   const float* px = h->pts;
   const int max_solves = direct ? 1 : 8;
@@ -1199,6 +1283,8 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
     int i1 = i0 + 1;
     while (i1 < n_inc && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
     GroupCtx gc;
+    h->depth = direct ? 0 : choose_depth(h, inc[order[i0]].lambda_m, NT, NP);
+    h->stats.depth = h->depth;
     if (direct) {
       gc.lam = inc[order[i0]].lambda_m;
       gc.k0 = 2.0 * WF_PI / gc.lam;
@@ -1278,6 +1364,452 @@ wf_status wf_get_stats(wf_handle* h, wf_stats* out) {
   WF_TRY(set_device(h));
   WF_TRY(prof_finalize(h));
   *out = h->stats;
+  return WF_OK;
+}
+
+wf_status wf_geometry_info(wf_handle* h, int64_t* n, int32_t* depth_built, double* box6) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "no geometry set");
+  if (n) *n = h->n;
+  if (depth_built) *depth_built = h->depth_max;
+  if (box6)
+    for (int q = 0; q < 6; ++q) box6[q] = h->box[q];
+  return WF_OK;
+}
+
+wf_status wf_get_tune(wf_handle* h, wf_tune* out) {
+  if (!h || !out) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  *out = h->tune;
+  return WF_OK;
+}
+
+wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
+  if (!h || !t) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  if (!(std::isfinite(t->leaf_s_per_pair) && t->leaf_s_per_pair >= 0 &&
+        std::isfinite(t->agg_s_per_byte) && t->agg_s_per_byte >= 0 && std::isfinite(t->level_s) &&
+        t->level_s >= 0))
+    return set_err(WF_ERR_INVALID_ARG, "wf_tune: coefficients must be finite and >= 0");
+  if (t->chunk_solves < 0 || t->chunk_solves > 8)
+    return set_err(WF_ERR_INVALID_ARG, "wf_tune: chunk_solves must be in [0, 8]");
+  h->tune = *t;
+  h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
+  return WF_OK;
+}
+
+wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                      int32_t* best_depth, void* stream) {
+  if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
+  if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "wf_autotune: no geometry");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_TRY(set_device(h));
+  if (h->depth_max == 0) {
+    if (best_depth) *best_depth = 0;
+    return WF_OK;
+  }
+  WF_TRY(check_tree_grid(grid));
+  hipStream_t st = as_stream(stream);
+  const int NT = grid.n_theta, NP = grid.n_phi;
+  const int64_t D = (int64_t)NT * NP;
+  // the first wavelength group, at most 8 solves
+  std::vector<wf_incidence> sub;
+  for (int i = 0; i < n_inc && sub.size() < 8; ++i)
+    if (inc[i].lambda_m == inc[0].lambda_m) sub.push_back(inc[i]);
+  const int S = (int)sub.size();
+  std::vector<int> order(S);
+  for (int i = 0; i < S; ++i) order[i] = i;
+  WfIncDev* dinc = nullptr;
+  WF_TRY(upload_incs(h, sub.data(), S, &dinc, st, "inc_tune"));
+  float* I_dev = nullptr;
+  WF_TRY(dalloc_t(h, "tune_I", (size_t)S * D, &I_dev));
+  const double lam_s = sample_lambda(h, sub[0].lambda_m);
+  const wf_tune saved = h->tune;
+  struct Run {
+    int d;
+    double total, leaf, agg, other;
+  };
+  // eager solve at a forced depth: one untimed pass (allocations), one timed
+  auto run = [&](int d, Run& r) -> wf_status {
+    h->force_depth = d;
+    for (int rep = 0; rep < 2; ++rep) {
+      prof_reset(h);
+      wf_status e = prof_event(h, st, &h->ev_t0);
+      if (e == WF_OK) e = solve_core(h, st, sub.data(), S, NT, NP, dinc, order.data(), I_dev, nullptr);
+      if (e == WF_OK) e = prof_event(h, st, &h->ev_t1);
+      if (e == WF_OK) e = prof_finalize(h);
+      if (e != WF_OK) {
+        h->force_depth = -1;
+        return e;
+      }
+    }
+    h->force_depth = -1;
+    const double* k = h->stats.kern_ms;
+    r.d = d;
+    r.total = 1e-3 * h->stats.ms_total;
+    r.leaf = 1e-3 * k[WF_K_LEAF];
+    r.agg = 1e-3 * (k[WF_K_PHI] + k[WF_K_THETA] + k[WF_K_TRANS]);
+    r.other = 1e-3 * (k[WF_K_CURRENTS] + k[WF_K_EPILOGUE]);
+    return WF_OK;
+  };
+  // candidate depths: the model's three cheapest
+  std::vector<std::pair<double, int>> rank;
+  for (int d = 1; d <= h->depth_max; ++d)
+    rank.push_back({model_cost(h, h->tune, d, lam_s, NT, NP), d});
+  std::sort(rank.begin(), rank.end());
+  if (rank.size() > 3) rank.resize(3);
+  // kernel variant at the model's best depth (only a BF16x3 choice; groups of
+  // <= 4 solves run the quad kernel in either setting)
+  wf_status e = WF_OK;
+  if (h->opts.precision == WF_PREC_BF16X3 && S > 4 && h->opts.field_chunk <= 0) {
+    Run qa, qb;
+    h->tune.leaf_quad = 1;
+    h->tune.chunk_solves = 4;
+    e = run(rank[0].second, qa);
+    if (e == WF_OK) {
+      h->tune.leaf_quad = 0;
+      h->tune.chunk_solves = 8;
+      e = run(rank[0].second, qb);
+    }
+    if (e != WF_OK) {
+      h->tune = saved;
+      return e;
+    }
+    const bool quad = qa.total <= qb.total;
+    h->tune.leaf_quad = quad ? 1 : 0;
+    h->tune.chunk_solves = quad ? 4 : 8;
+  }
+  std::vector<Run> runs;
+  for (const auto& c : rank) {
+    Run r;
+    e = run(c.second, r);
+    if (e != WF_OK) {
+      h->tune = saved;
+      return e;
+    }
+    runs.push_back(r);
+  }
+  // fit: leaf seconds per pair, aggregation seconds per byte, per-level rest
+  double sl = 0, pl = 0, sa = 0, ba = 0, rest = 0;
+  int nrest = 0;
+  const Run* best = &runs[0];
+  for (const Run& r : runs) {
+    const Grid gd = level_grid(h, r.d, lam_s);
+    sl += r.leaf;
+    pl += (double)h->n * gd.nt * gd.np * S;
+    wf_tune unit = {0.0, 1.0, 0.0, 0, 0, 0};
+    sa += r.agg;
+    ba += model_cost(h, unit, r.d, lam_s, NT, NP) * S;
+    rest += std::max(0.0, r.total - r.leaf - r.agg - r.other) / ((double)r.d * S);
+    ++nrest;
+    if (r.total < best->total) best = &r;
+  }
+  if (pl > 0) h->tune.leaf_s_per_pair = sl / pl;
+  if (ba > 0) h->tune.agg_s_per_byte = sa / ba;
+  if (nrest) h->tune.level_s = rest / nrest;
+  h->tune.calibrated = 1;
+  if (best_depth) *best_depth = best->d;
+  return WF_OK;
+}
+
+}  // extern "C"
+
+// ============================================================ batches =====
+namespace {
+
+// A batch segment's geometry: the sorted points and the per-level node lists
+// of its octree, compacted into one device allocation.
+struct SegGeom {
+  int64_t n = 0;
+  int depth_max = 0;
+  bool auto_depth = false;
+  float boxf[6];
+  double box[6];
+  float* pts = nullptr;
+  int32_t* perm = nullptr;
+  uint32_t* keys = nullptr;
+  std::vector<LevelDev> levels;
+  void* mem = nullptr;
+  uint64_t gen = 0;
+  std::vector<GraphCache> graphs;  // one per worker
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int last_depth = 0;
+  bool timed = false;
+};
+
+uint64_t g_seg_gen = 0;
+
+}  // namespace
+
+struct wf_batch {
+  wf_opts opts;
+  int device = 0;
+  std::vector<wf_handle*> workers;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> done;
+  std::vector<SegGeom> segs;
+  bool tuned = false;
+  void* out = nullptr;  // device staging of host outputs
+  size_t out_cap = 0;
+};
+
+namespace {
+
+// Points the handle at a segment's geometry (no copies).
+void bind_segment(wf_handle* h, const SegGeom& g) {
+  h->n = g.n;
+  h->depth = g.depth_max;
+  h->depth_max = g.depth_max;
+  h->auto_depth = g.auto_depth;
+  for (int q = 0; q < 6; ++q) {
+    h->boxf[q] = g.boxf[q];
+    h->box[q] = g.box[q];
+  }
+  h->pts = g.pts;
+  h->perm = g.perm;
+  h->keys = g.keys;
+  h->levels = g.levels;
+  h->geom_gen = g.gen;
+  h->has_geom = true;
+}
+
+void free_segment(SegGeom& g) {
+  for (auto& c : g.graphs) c.reset();
+  if (g.ev0) hipEventDestroy(g.ev0);
+  if (g.ev1) hipEventDestroy(g.ev1);
+  if (g.mem) hipFree(g.mem);
+  g.mem = nullptr;
+  g.ev0 = g.ev1 = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+wf_status wf_batch_create(wf_batch** out, const wf_opts* opts, int32_t n_streams) {
+  if (!out) return set_err(WF_ERR_INVALID_ARG, "wf_batch_create: out is null");
+  *out = nullptr;
+  if (n_streams < 1 || n_streams > 16)
+    return set_err(WF_ERR_INVALID_ARG, "wf_batch_create: n_streams must be in [1, 16]");
+  wf_batch* b = new wf_batch();
+  for (int w = 0; w < n_streams; ++w) {
+    wf_handle* h = nullptr;
+    const wf_status e = wf_create(&h, opts);
+    if (e != WF_OK) {
+      wf_batch_destroy(b);
+      return e;
+    }
+    b->workers.push_back(h);
+    if (w == 0) {
+      b->opts = h->opts;
+      b->device = h->device;
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      if (s) hipStreamDestroy(s);
+      wf_batch_destroy(b);
+      return set_err(WF_ERR_HIP, "wf_batch_create: stream/event creation failed");
+    }
+    b->streams.push_back(s);
+    b->done.push_back(ev);
+  }
+  *out = b;
+  return WF_OK;
+}
+
+void wf_batch_destroy(wf_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  for (auto s : b->streams) hipStreamSynchronize(s);
+  for (auto& g : b->segs) free_segment(g);
+  if (b->out) hipFree(b->out);
+  for (auto ev : b->done) hipEventDestroy(ev);
+  for (auto s : b->streams) hipStreamDestroy(s);
+  for (auto h : b->workers) wf_destroy(h);
+  delete b;
+}
+
+wf_status wf_batch_add(wf_batch* b, const wf_surface* s, int32_t depth, int32_t* index) {
+  if (!b) return set_err(WF_ERR_INVALID_ARG, "batch is null");
+  wf_handle* h = b->workers[0];
+  hipStream_t st = b->streams[0];
+  WF_TRY(wf_set_geometry(h, s, depth, st));
+  SegGeom g;
+  g.n = h->n;
+  g.depth_max = h->depth_max;
+  g.auto_depth = h->auto_depth;
+  for (int q = 0; q < 6; ++q) {
+    g.boxf[q] = h->boxf[q];
+    g.box[q] = h->box[q];
+  }
+  const int64_t n = h->n;
+  size_t nodes = 0;
+  for (const LevelDev& L : h->levels) nodes += (size_t)L.count;
+  const size_t bytes = sizeof(float) * 7 * n + 2 * sizeof(int32_t) * n + 6 * sizeof(int32_t) * nodes;
+  WF_HIP(hipMalloc(&g.mem, bytes));
+  char* p = static_cast<char*>(g.mem);
+  auto take = [&](size_t nb) {
+    char* q = p;
+    p += nb;
+    return q;
+  };
+  g.pts = reinterpret_cast<float*>(take(sizeof(float) * 7 * n));
+  g.perm = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * n));
+  g.keys = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n));
+  wf_status e = WF_OK;
+  auto cp = [&](void* dst, const void* src, size_t nb) {
+    if (e == WF_OK && nb && hipMemcpyAsync(dst, src, nb, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      e = set_err(WF_ERR_HIP, "wf_batch_add: device copy failed");
+  };
+  cp(g.pts, h->pts, sizeof(float) * 7 * n);
+  cp(g.perm, h->perm, sizeof(int32_t) * n);
+  cp(g.keys, h->keys, sizeof(uint32_t) * n);
+  g.levels.resize(h->levels.size());
+  for (size_t l = 0; l < h->levels.size(); ++l) {
+    const LevelDev& L = h->levels[l];
+    LevelDev& G = g.levels[l];
+    const size_t c = (size_t)L.count;
+    G.count = L.count;
+    G.key = reinterpret_cast<uint32_t*>(take(4 * c));
+    G.begin = reinterpret_cast<int32_t*>(take(4 * c));
+    G.end = reinterpret_cast<int32_t*>(take(4 * c));
+    G.cb = reinterpret_cast<int32_t*>(take(4 * c));
+    G.ce = reinterpret_cast<int32_t*>(take(4 * c));
+    G.parent = reinterpret_cast<int32_t*>(take(4 * c));
+    cp(G.key, L.key, 4 * c);
+    cp(G.begin, L.begin, 4 * c);
+    cp(G.end, L.end, 4 * c);
+    cp(G.cb, L.cb, 4 * c);
+    cp(G.ce, L.ce, 4 * c);
+    cp(G.parent, L.parent, 4 * c);
+  }
+  if (e == WF_OK && hipStreamSynchronize(st) != hipSuccess)
+    e = set_err(WF_ERR_HIP, "wf_batch_add: stream synchronize failed");
+  h->has_geom = false;
+  if (e != WF_OK) {
+    hipFree(g.mem);
+    return e;
+  }
+  g.gen = (1ULL << 63) | ++g_seg_gen;
+  g.graphs.resize(b->workers.size());
+  if (hipEventCreate(&g.ev0) != hipSuccess || hipEventCreate(&g.ev1) != hipSuccess) {
+    free_segment(g);
+    return set_err(WF_ERR_HIP, "wf_batch_add: event creation failed");
+  }
+  if (index) *index = (int32_t)b->segs.size();
+  b->segs.push_back(std::move(g));
+  return WF_OK;
+}
+
+wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                         float* intensity, int32_t out_memspace, void* stream) {
+  if (!b) return set_err(WF_ERR_INVALID_ARG, "batch is null");
+  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
+  if (b->segs.empty()) return set_err(WF_ERR_NO_GEOMETRY, "wf_batch_solve: no segments");
+  if (out_memspace != WF_MEM_HOST && out_memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "out_memspace unknown");
+  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_HIP(hipSetDevice(b->device));
+  const int nseg = (int)b->segs.size();
+  const int nw = (int)b->workers.size();
+  const int64_t per_seg = (int64_t)n_inc * grid.n_theta * grid.n_phi;
+  for (const SegGeom& g : b->segs)
+    if (g.depth_max > 0) WF_TRY(check_tree_grid(grid));
+  float* out = intensity;
+  if (out_memspace == WF_MEM_HOST) {
+    const size_t need = sizeof(float) * (size_t)per_seg * nseg;
+    if (b->out_cap < need) {
+      if (b->out) {
+        for (auto s : b->streams) WF_HIP(hipStreamSynchronize(s));
+        hipFree(b->out);
+      }
+      b->out = nullptr;
+      b->out_cap = 0;
+      WF_HIP(hipMalloc(&b->out, need));
+      b->out_cap = need;
+    }
+    out = static_cast<float*>(b->out);
+  }
+  hipStream_t caller = as_stream(stream);
+  // the workers start after the caller's prior work (device outputs)
+  WF_HIP(hipEventRecord(b->done[0], caller));
+  for (int w = 0; w < nw; ++w) WF_HIP(hipStreamWaitEvent(b->streams[w], b->done[0], 0));
+  // largest segments first; each goes to the worker with the least queued
+  // points (deterministic, so a segment keeps its worker and its graph)
+  std::vector<int> order(nseg);
+  for (int i = 0; i < nseg; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return b->segs[a].n > b->segs[c].n; });
+  // cost-model calibration on the largest auto-depth segment
+  if (!b->tuned) {
+    for (int i : order) {
+      if (!b->segs[i].auto_depth) continue;
+      wf_handle* h0 = b->workers[0];
+      bind_segment(h0, b->segs[i]);
+      std::swap(h0->gcache, b->segs[i].graphs[0]);
+      const wf_status e = wf_autotune(h0, inc, n_inc, grid, nullptr, b->streams[0]);
+      std::swap(h0->gcache, b->segs[i].graphs[0]);
+      h0->has_geom = false;
+      WF_TRY(e);
+      wf_tune t;
+      WF_TRY(wf_get_tune(h0, &t));
+      for (int w = 1; w < nw; ++w) WF_TRY(wf_set_tune(b->workers[w], &t));
+      b->tuned = true;
+      break;
+    }
+  }
+  std::vector<double> load(nw, 0.0);
+  double lam_max = inc[0].lambda_m;
+  for (int i = 1; i < n_inc; ++i) lam_max = std::max(lam_max, inc[i].lambda_m);
+  for (int i : order) {
+    SegGeom& g = b->segs[i];
+    const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[w] += (double)g.n;
+    wf_handle* h = b->workers[w];
+    hipStream_t st = b->streams[w];
+    bind_segment(h, g);
+    std::swap(h->gcache, g.graphs[w]);
+    WF_HIP(hipEventRecord(g.ev0, st));
+    const wf_status e =
+        wf_solve(h, inc, n_inc, grid, out + (int64_t)i * per_seg, nullptr, WF_MEM_DEVICE, st);
+    std::swap(h->gcache, g.graphs[w]);
+    g.last_depth = choose_depth(h, lam_max, grid.n_theta, grid.n_phi);
+    h->has_geom = false;
+    WF_TRY(e);
+    WF_HIP(hipEventRecord(g.ev1, st));
+    g.timed = true;
+  }
+  for (int w = 0; w < nw; ++w) {
+    WF_HIP(hipEventRecord(b->done[w], b->streams[w]));
+    WF_HIP(hipStreamWaitEvent(caller, b->done[w], 0));
+  }
+  if (out_memspace == WF_MEM_HOST) {
+    WF_HIP(hipMemcpyAsync(intensity, out, sizeof(float) * (size_t)per_seg * nseg,
+                          hipMemcpyDeviceToHost, caller));
+    WF_HIP(hipStreamSynchronize(caller));
+  }
+  return WF_OK;
+}
+
+wf_status wf_batch_info(wf_batch* b, int32_t* n_segments, int32_t* depth, double* ms,
+                        int64_t* n_points) {
+  if (!b) return set_err(WF_ERR_INVALID_ARG, "batch is null");
+  WF_HIP(hipSetDevice(b->device));
+  if (n_segments) *n_segments = (int32_t)b->segs.size();
+  for (size_t i = 0; i < b->segs.size(); ++i) {
+    const SegGeom& g = b->segs[i];
+    if (depth) depth[i] = g.last_depth;
+    if (n_points) n_points[i] = g.n;
+    if (ms) {
+      float t = 0;
+      if (g.timed) {
+        WF_HIP(hipEventSynchronize(g.ev1));
+        WF_HIP(hipEventElapsedTime(&t, g.ev0, g.ev1));
+      }
+      ms[i] = t;
+    }
+  }
   return WF_OK;
 }
 

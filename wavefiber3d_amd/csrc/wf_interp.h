@@ -54,6 +54,7 @@ struct LeafArgs {
   int nt, np;
   float2* out;
   int precision;
+  int quad;  // 1: quad-symmetric BF16x3 kernel where eligible (np even, F <= 24)
 };
 hipError_t launch_leaf(const LeafArgs& a, hipStream_t st);
 // MFMA variants (F <= 48 fields per launch): split BF16 x3 or FP32 MFMA.

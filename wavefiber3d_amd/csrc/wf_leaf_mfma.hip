@@ -412,7 +412,7 @@ template <int MT>
 hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
   constexpr int R = (MT <= 2) ? 4 : (MT <= 4 ? 2 : 1);
   const int D = a.nt * a.np;
-  if (bf16 && MT <= 3 && (a.np % 2) == 0) {
+  if (bf16 && a.quad && MT <= 3 && (a.np % 2) == 0) {
     const int nquad = ((a.nt + 1) / 2) * (a.np / 2);
     const int nsuper = ((nquad + 15) / 16 + 3) / 4;
     const int64_t nblk = (int64_t)nsuper * a.nleaves;

@@ -95,6 +95,57 @@ def _stream_ptr(arrays) -> int | None:
     return None
 
 
+def _make_opts(device: int = -1, precision: str = "bf16x3", beta: float = 4.0, box=None,
+               box_enlarge: float = 1.1, lambda_sample_m: float = 0.0, strict_grid: bool = False,
+               workspace_cap_bytes: int = 0, field_chunk: int = 0,
+               cancel_flag: ct.c_int32 | None = None, use_graph: bool = False) -> _lib.Opts:
+    o = _lib.Opts()
+    lib().wf_default_opts(ct.byref(o))
+    o.device = device
+    o.precision = {"fp32": _lib.WF_PREC_FP32, "bf16x3": _lib.WF_PREC_BF16X3,
+                   "fp32_valu": _lib.WF_PREC_FP32_VALU}[precision]
+    o.beta = beta
+    o.box_enlarge = box_enlarge
+    o.lambda_sample_m = lambda_sample_m
+    o.strict_grid = int(strict_grid)
+    o.workspace_cap_bytes = workspace_cap_bytes
+    o.field_chunk = field_chunk
+    o.use_graph = int(use_graph)
+    if box is not None:
+        o.has_box = 1
+        for i in range(6):
+            o.box[i] = float(box[i])
+    if cancel_flag is not None:
+        o.cancel_flag = ct.pointer(cancel_flag)
+    return o
+
+
+def _depth_arg(depth) -> int:
+    if isinstance(depth, str):
+        if depth != "auto":
+            raise ValueError("depth must be an int in [0, 10] or 'auto'")
+        return _lib.WF_DEPTH_AUTO
+    return int(depth)
+
+
+def _surface_struct(s):
+    arrs = s.arrays() if isinstance(s, SurfaceSamples) else list(s)
+    if len(arrs) != 7:
+        raise ValueError("surface needs x, y, z, nx, ny, nz, area")
+    dev = _is_torch(arrs[0]) and arrs[0].is_cuda
+    if dev:
+        import torch
+
+        arrs = [a.to(torch.float32).contiguous() for a in arrs]
+    else:
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in arrs]
+    n = int(arrs[0].shape[0])
+    if any(int(a.shape[0]) != n for a in arrs):
+        raise ValueError("surface arrays differ in length")
+    c = _lib.Surface(*[_ptr(a) for a in arrs], n, WF_MEM_DEVICE if dev else WF_MEM_HOST)
+    return c, arrs, dev
+
+
 class POSolver:
     """Handle over the C-ABI (one per host thread)."""
 
@@ -103,28 +154,11 @@ class POSolver:
                  strict_grid: bool = False, workspace_cap_bytes: int = 0,
                  field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None,
                  use_graph: bool = False):
-        L = lib()
-        o = _lib.Opts()
-        L.wf_default_opts(ct.byref(o))
-        o.device = device
-        o.precision = {"fp32": _lib.WF_PREC_FP32, "bf16x3": _lib.WF_PREC_BF16X3,
-                       "fp32_valu": _lib.WF_PREC_FP32_VALU}[precision]
-        o.beta = beta
-        o.box_enlarge = box_enlarge
-        o.lambda_sample_m = lambda_sample_m
-        o.strict_grid = int(strict_grid)
-        o.workspace_cap_bytes = workspace_cap_bytes
-        o.field_chunk = field_chunk
-        o.use_graph = int(use_graph)
-        if box is not None:
-            o.has_box = 1
-            for i in range(6):
-                o.box[i] = float(box[i])
+        o = _make_opts(device, precision, beta, box, box_enlarge, lambda_sample_m, strict_grid,
+                       workspace_cap_bytes, field_chunk, cancel_flag, use_graph)
         self._cancel = cancel_flag
-        if cancel_flag is not None:
-            o.cancel_flag = ct.pointer(cancel_flag)
         self._h = ct.c_void_p()
-        check(L.wf_create(ct.byref(self._h), ct.byref(o)))
+        check(lib().wf_create(ct.byref(self._h), ct.byref(o)))
         self._keep = []
 
     def close(self):
@@ -146,30 +180,50 @@ class POSolver:
 
     # -------------------------------------------------------------- geometry
     def _surface(self, s):
-        arrs = s.arrays() if isinstance(s, SurfaceSamples) else list(s)
-        if len(arrs) != 7:
-            raise ValueError("surface needs x, y, z, nx, ny, nz, area")
-        dev = _is_torch(arrs[0]) and arrs[0].is_cuda
-        if dev:
-            import torch
+        return _surface_struct(s)
 
-            arrs = [a.to(torch.float32).contiguous() for a in arrs]
-        else:
-            arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in arrs]
-        n = int(arrs[0].shape[0])
-        if any(int(a.shape[0]) != n for a in arrs):
-            raise ValueError("surface arrays differ in length")
-        c = _lib.Surface(*[_ptr(a) for a in arrs], n, WF_MEM_DEVICE if dev else WF_MEM_HOST)
-        return c, arrs, dev
-
-    def set_geometry(self, surface, depth: int):
-        """Upload + octree build (tree.h:289-715; treePO.cu:137-337)."""
+    def set_geometry(self, surface, depth):
+        """Upload + octree build (tree.h:289-715; treePO.cu:137-337).  depth: int in
+        [0, 10] or "auto" (built to ~log8 n; each wavelength group then uses the
+        cost model's depth)."""
         c, arrs, dev = self._surface(surface)
         self._keep = arrs
-        check(lib().wf_set_geometry(self._h, ct.byref(c), depth, _stream_ptr(arrs)))
-        self.depth = depth
-        self.n = c.n
+        check(lib().wf_set_geometry(self._h, ct.byref(c), _depth_arg(depth), _stream_ptr(arrs)))
+        n = ct.c_int64()
+        d = ct.c_int32()
+        check(lib().wf_geometry_info(self._h, ct.byref(n), ct.byref(d), None))
+        self.depth = d.value  # built depth
+        self.auto_depth = depth == "auto"
+        self.n = n.value
         self._dev_geom = dev
+
+    # ------------------------------------------------------------- autotuning
+    @property
+    def tune(self) -> dict:
+        t = _lib.Tune()
+        check(lib().wf_get_tune(self._h, ct.byref(t)))
+        return {k: getattr(t, k) for k, _ in _lib.Tune._fields_}
+
+    @tune.setter
+    def tune(self, d: dict):
+        t = _lib.Tune()
+        check(lib().wf_get_tune(self._h, ct.byref(t)))
+        for k, v in d.items():
+            setattr(t, k, v)
+        check(lib().wf_set_tune(self._h, ct.byref(t)))
+
+    def autotune(self, incidences, grid) -> int:
+        """Fit the depth cost model and leaf-kernel choice on the current geometry
+        (timed solves of the first wavelength group); returns the fastest depth."""
+        arr, k = _incs(incidences)
+        best = ct.c_int32()
+        st = None
+        if self._dev_geom:
+            import torch
+
+            st = torch.cuda.current_stream().cuda_stream
+        check(lib().wf_autotune(self._h, arr, k, _lib.Grid(*grid), ct.byref(best), st))
+        return best.value
 
     # ----------------------------------------------------------------- solves
     def solve(self, incidences, grid, return_E: bool = False, out: str = "numpy"):
@@ -317,3 +371,77 @@ class POSolver:
             "ms_total": s.ms_total, "kernels": kern, "leaf_pairs": s.leaf_pairs, "depth": s.depth,
             "levels": [(s.level_nt[i], s.level_np[i], s.level_nodes[i]) for i in range(nl)],
         }
+
+
+class Batch:
+    """Many surfaces solved for the same incidences and grid (wf_batch_*).
+
+    Each segment keeps its own octree; solves run on ``n_streams`` worker
+    streams with separate workspaces.  ``depth="auto"`` segments get their
+    depth per wavelength group from the cost model, calibrated on the largest
+    one at the first solve."""
+
+    def __init__(self, n_streams: int = 4, **opts):
+        o = _make_opts(**opts)
+        self._b = ct.c_void_p()
+        check(lib().wf_batch_create(ct.byref(self._b), ct.byref(o), int(n_streams)))
+        self.n_points = []
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            lib().wf_batch_destroy(self._b)
+            self._b = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, surface, depth="auto") -> int:
+        c, arrs, dev = _surface_struct(surface)
+        idx = ct.c_int32()
+        if dev:
+            import torch
+
+            torch.cuda.current_stream().synchronize()
+        check(lib().wf_batch_add(self._b, ct.byref(c), _depth_arg(depth), ct.byref(idx)))
+        self.n_points.append(c.n)
+        return idx.value
+
+    def __len__(self):
+        return len(self.n_points)
+
+    def solve(self, incidences, grid, out: str = "numpy"):
+        """Intensity [n_segments, n_inc, Nt, Np] float32."""
+        arr, k = _incs(incidences)
+        nt, np_ = grid
+        g = _lib.Grid(nt, np_)
+        shape = (len(self), k, nt, np_)
+        if out == "torch":
+            import torch
+
+            I = torch.empty(shape, dtype=torch.float32, device="cuda")
+            check(lib().wf_batch_solve(self._b, arr, k, g, I.data_ptr(), WF_MEM_DEVICE,
+                                       torch.cuda.current_stream().cuda_stream))
+            return I
+        I = np.empty(shape, dtype=np.float32)
+        check(lib().wf_batch_solve(self._b, arr, k, g, I.ctypes.data, WF_MEM_HOST, None))
+        return I
+
+    def info(self) -> dict:
+        """Per segment: depth of the last wavelength group, solve time (ms), points."""
+        n = len(self)
+        cnt = ct.c_int32()
+        d = np.zeros(n, np.int32)
+        ms = np.zeros(n, np.float64)
+        npts = np.zeros(n, np.int64)
+        check(lib().wf_batch_info(self._b, ct.byref(cnt), d.ctypes.data, ms.ctypes.data,
+                                  npts.ctypes.data))
+        return {"depth": d, "ms": ms, "n_points": npts}
