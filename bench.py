@@ -58,35 +58,48 @@ def workload(name: str):
                     desc="fiber segment 40x30um ellipse, length 30um, cuticle 5um/3deg, noise "
                          "0.05um/0.5um seed 0, 0.1um spacing, IOR 1.55+0.02i, depth 6, "
                          "32 incidences TM, lambda 0.4um, far field 360x720")
+    if name == "fibers256":
+        # 8 incidences (theta idx {60, 90, 105, 120} x phi idx {0, 180}) x 25 wavelengths
+        incs = [wf.Incidence(wf.reference_incidence_theta(t), wf.reference_incidence_phi(p),
+                             wf.reference_lambda(l), "M", 1.55 + 0.02j)
+                for l in range(25) for t in (60, 90, 105, 120) for p in (0, 180)]
+        return dict(batch=True, incs=incs, grid=(360, 720), depth="auto",
+                    desc="uneven fiber segments (seeds 0..K-1, r1 U[20,50]um, r2 0.75 r1, length "
+                         "U[5,40]um, 20k-400k points), 8 incidences x 25 lambdas TM, "
+                         "IOR 1.55+0.02i, far field 360x720, auto depth, batched on worker streams")
     raise SystemExit(f"unknown workload {name}")
 
 
 def cpu_baseline(surf, incs, grid, budget_s: float):
     """FP64 oracle direct sum on the host cores over a sample of output
-    directions (every k-th), both solves of the first wavelength."""
+    directions (every k-th), both solves of the first wavelength.  The sample
+    grows geometrically until one run takes about budget_s."""
     import oracle
 
     dirs_all = oracle.output_dirs(*grid)
     lam0 = incs[0].lambda_m
     sample_incs = [t.as_tuple() for t in incs if t.lambda_m == lam0][:2]
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    # calibrate on a few directions, then size the sample to ~budget_s
-    probe = dirs_all[:: max(1, len(dirs_all) // 16)][:16]
-    t0 = time.perf_counter()
-    oracle.solve_direct(surf.arrays(), sample_incs, 0, 0, dirs=probe, nthreads=nthreads)
-    dt = max(time.perf_counter() - t0, 1e-3)
-    per_dir = dt / len(probe)
-    nd = int(min(len(dirs_all), max(16, budget_s / per_dir)))
-    step = max(1, len(dirs_all) // nd)
-    dirs = dirs_all[::step][:nd]
-    t0 = time.perf_counter()
-    oracle.solve_direct(surf.arrays(), sample_incs, 0, 0, dirs=dirs, nthreads=nthreads)
-    dt = time.perf_counter() - t0
+    arrays = surf.arrays()
+
+    def run(nd):
+        step = max(1, len(dirs_all) // nd)
+        dirs = dirs_all[::step][:nd]
+        t0 = time.perf_counter()
+        oracle.solve_direct(arrays, sample_incs, 0, 0, dirs=dirs, nthreads=nthreads)
+        return dirs, step, time.perf_counter() - t0
+
+    nd = min(len(dirs_all), 4 * nthreads)
+    dirs, step, dt = run(nd)
+    while dt < 0.6 * budget_s and nd < len(dirs_all):
+        grow = budget_s / max(dt, 1e-3)
+        nd = min(len(dirs_all), max(nd + 1, int(nd * min(grow, 8.0))))
+        dirs, step, dt = run(nd)
     pairs = surf.n * len(dirs) * len(sample_incs)
     return {"value": pairs / dt, "unit": "pairs/s", "cores": nthreads, "kind": "port",
-            "sample": f"FP64 oracle direct sum, {len(dirs)} of {len(dirs_all)} output directions "
-                      f"(every {step}th), {len(sample_incs)} solves at lambda {lam0 * 1e9:.0f} nm, "
-                      f"N={surf.n}, {dt:.1f} s"}
+            "sample": f"FP64 oracle direct sum (C, OpenMP), {len(dirs)} of {len(dirs_all)} output "
+                      f"directions (every {step}th), {len(sample_incs)} solves at lambda "
+                      f"{lam0 * 1e9:.0f} nm, N={surf.n}, {dt:.1f} s"}
 
 
 def main():
@@ -100,6 +113,11 @@ def main():
     ap.add_argument("--no-direct", action="store_true", help="skip the FP64 direct-sum gate")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--depth", default=None, help="override the workload's depth (int or 'auto')")
+    ap.add_argument("--autotune", action="store_true", help="wf_autotune before the warm-up")
+    ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
+    ap.add_argument("--streams", type=int, default=4, help="fibers256: worker streams")
+    ap.add_argument("--graph", action="store_true", help="capture solves into hipGraphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -112,19 +130,32 @@ def main():
     import wavefiber3d_amd as wf
 
     W = workload(args.workload)
+    if args.depth is not None:
+        W["depth"] = "auto" if args.depth == "auto" else int(args.depth)
     print(f"[bench] workload {args.workload}: {W['desc']}", file=sys.stderr, flush=True)
+    if W.get("batch"):
+        return bench_batch(args, W)
     surf = W["surface"]()
     incs = W["incs"]
     nt, np_ = W["grid"]
     D = nt * np_
     S = len(incs)
     dev = [torch.from_numpy(a).cuda() for a in surf.arrays()]
-    solver = wf.POSolver(device=0, precision=args.precision, beta=args.beta)
+    solver = wf.POSolver(device=0, precision=args.precision, beta=args.beta, use_graph=args.graph)
     t0 = time.perf_counter()
     solver.set_geometry(dev, W["depth"])
     torch.cuda.synchronize()
     tree_ms = (time.perf_counter() - t0) * 1e3
     print(f"[bench] geometry + octree {tree_ms:.1f} ms (N={surf.n})", file=sys.stderr, flush=True)
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        best = solver.autotune(incs, (nt, np_))
+        tuned = {"best_depth": best, "tune": solver.tune,
+                 "ms": (time.perf_counter() - t0) * 1e3}
+        print(f"[bench] autotune {tuned}", file=sys.stderr, flush=True)
+    stream = torch.cuda.Stream() if args.graph else torch.cuda.current_stream()
+    torch.cuda.set_stream(stream)
 
     for _ in range(args.warmup):
         I = solver.solve(incs, (nt, np_), out="torch")
@@ -161,7 +192,7 @@ def main():
                 "work_per_launch": work, "avg_launch_ms": avg_ms}
 
     out = {
-        "metric": "PO point-direction pairs/s (tree vs direct) at <=1e-3 rel-L2 vs FP64 oracle",
+        "metric": "PO point-direction pairs/s (tree vs direct) at \u22641e-3 rel-L2 vs FP64 oracle",
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -175,8 +206,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "description": W["desc"], "n_points": surf.n,
                    "n_directions": D, "solves": S, "depth": W["depth"],
-                   "precision": args.precision, "beta": args.beta,
-                   "levels": st["levels"]},
+                   "depth_used": st["depth"], "precision": args.precision, "beta": args.beta,
+                   "graph": args.graph, "autotune": tuned, "levels": st["levels"]},
         "roofline": roof,
         "solves_per_s": S / (ms_step * 1e-3),
         "kernels_ms": {k: v["ms"] for k, v in kern.items()},
@@ -212,6 +243,78 @@ def main():
         print(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} pairs/s", file=sys.stderr,
               flush=True)
     solver.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_batch(args, W):
+    """Uneven multi-segment workload through wf_batch_*: a step is one batch
+    solve of every segment for all incidences (segments resident in HBM)."""
+    import torch
+
+    import wavefiber3d_amd as wf
+    from wavefiber3d_amd.geometry import fiber_family_segment
+
+    incs = W["incs"]
+    nt, np_ = W["grid"]
+    D = nt * np_
+    S = len(incs)
+    K = args.segments
+    t0 = time.perf_counter()
+    b = wf.Batch(n_streams=args.streams, device=0, precision=args.precision, beta=args.beta,
+                 use_graph=args.graph)
+    segs_n = []
+    for k in range(K):
+        s = fiber_family_segment(k)
+        b.add(s, W["depth"])
+        segs_n.append(s.n)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    N = float(sum(segs_n))
+    print(f"[bench] {K} segments, N total {N:.3e} (min {min(segs_n)}, max {max(segs_n)}), "
+          f"generate + upload + octrees {build_ms:.0f} ms", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        I = b.solve(incs, (nt, np_), out="torch")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        I = b.solve(incs, (nt, np_), out="torch")
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / max(1, args.steps)
+    info = b.info()
+    pairs = N * D * S
+    value = pairs / (ms_step * 1e-3)
+    print(f"[bench] batch: {ms_step:.1f} ms/step, {value:.3e} pairs/s, depths "
+          f"{np.bincount(info['depth']).tolist()}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "PO point-direction pairs/s (tree vs direct) at \u22641e-3 rel-L2 vs FP64 oracle",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "bf16x3-f32acc", "data": "synthetic",
+        "config": {"workload": args.workload, "description": W["desc"], "segments": K,
+                   "n_points_total": N, "n_points_min": min(segs_n), "n_points_max": max(segs_n),
+                   "n_directions": D, "solves_per_segment": S, "streams": args.streams,
+                   "graph": args.graph, "depth_hist": np.bincount(info["depth"]).tolist()},
+        "solves_per_s": K * S / (ms_step * 1e-3),
+        "build_ms": build_ms,
+    }
+    if not args.no_direct:
+        # gate: largest and smallest segment, first wavelength, vs the FP64 direct sum
+        I = I.cpu().numpy()
+        errs = {}
+        lam0 = [t for t in incs if t.lambda_m == incs[0].lambda_m]
+        with wf.POSolver(device=0) as sv:
+            for k in sorted({int(np.argmax(segs_n)), int(np.argmin(segs_n))}):
+                s = fiber_family_segment(k)
+                Id = sv.solve_fp64_direct(s, lam0, (nt, np_)).reshape(len(lam0), -1)
+                It = I[k, :len(lam0)].reshape(len(lam0), -1).astype(np.float64)
+                e = np.linalg.norm(It - Id, axis=1) / np.linalg.norm(Id, axis=1)
+                errs[f"segment{k}_n{segs_n[k]}"] = float(e.max())
+        out["accuracy"] = {"rel_l2_max_checked": max(errs.values()), "checked": errs,
+                           "gate": 1e-3, "pass": bool(max(errs.values()) <= 1e-3)}
+        print(f"[bench] accuracy {errs}", file=sys.stderr, flush=True)
+    b.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -170,3 +170,28 @@ def test_torch_fiber_generator_matches_numpy_recipe():
     sm = fiber_surface_torch(torch.tensor(3.0 - h, dtype=torch.float64), device="cpu", **kw)
     fd = ((sp[0] - sm[0]).sum() * 1e6 / (2 * h)).item()
     assert abs(t.grad.item() - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_fiber_family_sizes():
+    """The uneven multi-segment family: radii 20-50 um, lengths 5-40 um,
+    20k-400k samples (spacing widened or narrowed from 0.1 um to stay in range)."""
+    import math
+
+    from wavefiber3d_amd.geometry import fiber_family_params, fiber_family_segment
+
+    for seed in range(256):
+        p = fiber_family_params(seed)
+        assert 20e-6 <= p["r1"] <= 50e-6 and p["r2"] == pytest.approx(0.75 * p["r1"])
+        assert 5e-6 <= p["length"] <= 40e-6
+        t = np.linspace(0, 2 * np.pi, 4097)
+        per = np.sum(np.hypot(np.diff(p["r1"] * np.cos(t)), np.diff(p["r2"] * np.sin(t))))
+        n = round(per / p["spacing"]) * round(p["length"] / p["spacing"])
+        assert 0.98 * 2e4 <= n <= 1.02 * 4e5, (seed, n)
+        assert p["spacing"] >= 0.1e-6 or n >= 0.98 * 2e4
+    # the generated segment has exactly the predicted numel x numelvert samples
+    p = fiber_family_params(3)
+    s = fiber_family_segment(3)
+    t = np.linspace(0, 2 * np.pi, 4097)
+    per = np.sum(np.hypot(np.diff(p["r1"] * np.cos(t)), np.diff(p["r2"] * np.sin(t))))
+    assert s.n == round(per / p["spacing"]) * round(p["length"] / p["spacing"])
+    assert math.isfinite(float(s.area.sum())) and (s.area > 0).all()

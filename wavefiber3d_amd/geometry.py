@@ -15,6 +15,7 @@ areas (compareMie/sphere.cu:92-125).  These helpers produce such samples:
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -150,3 +151,30 @@ def fiber_segment(r1: float = 40e-6, r2: float = 30e-6, length: float = 30e-6,
     if max_points is not None and s.n > max_points:
         s = SurfaceSamples(*[a[:max_points] for a in s.arrays()])
     return s
+
+
+def fiber_family_params(seed: int, n_min: int = 20_000, n_max: int = 400_000) -> dict:
+    """Parameters of one segment of the uneven multi-segment workload: the
+    fiber recipe of fiber_segment with radius r1 ~ U[20, 50] um (r2 = 0.75 r1,
+    the 40 x 30 um aspect), length ~ U[5, 40] um, seeded by `seed`.  The
+    spacing is 0.1 um unless that gives more than n_max (or fewer than n_min)
+    samples, in which case it is set to give n_max (n_min); the actual count
+    is what quad_samples_from_heightfield returns."""
+    rng = np.random.default_rng(10_000 + seed)
+    r1 = float(rng.uniform(20e-6, 50e-6))
+    r2 = 0.75 * r1
+    length = float(rng.uniform(5e-6, 40e-6))
+    t = np.linspace(0, 2 * np.pi, 4097)
+    per = float(np.sum(np.hypot(np.diff(r1 * np.cos(t)), np.diff(r2 * np.sin(t)))))
+    area = per * length
+    spacing = 0.1e-6
+    if area / spacing ** 2 > n_max:
+        spacing = math.sqrt(area / n_max)
+    elif area / spacing ** 2 < n_min:
+        spacing = math.sqrt(area / n_min)
+    return dict(r1=r1, r2=r2, length=length, spacing=spacing, seed=seed)
+
+
+def fiber_family_segment(seed: int, **kw) -> SurfaceSamples:
+    """Segment `seed` of the uneven multi-segment workload (fiber_family_params)."""
+    return fiber_segment(**fiber_family_params(seed, **kw))
