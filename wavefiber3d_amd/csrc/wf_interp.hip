@@ -5,7 +5,9 @@
 // copies, Nyquist fix-ups, 2 scalings), translation_compute (fields.h:92-103,
 // host FP32 table treePO.cu:511-554) and sumchildren (fields.h:153-170), each
 // a separate launch, plus a linear host search for the child
-// (treePO.cu:725-732).  Here one level transition is two kernels:
+// (treePO.cu:725-732).  Here one level transition is one fused kernel when
+// its grids fit in LDS (agg_fused: one block per (parent, field), every
+// child pattern read once, the parent written once), otherwise two kernels:
 //
 //  phi_pass:  every child row (theta_j, all phi) -> FFT(npc) -> spectral
 //             pad/fold to npp -> IFFT(npp), in LDS.
