@@ -1,9 +1,11 @@
 """Reverse mode (wf_po_vjp) parity on the GPU.
 
 * FP64 direct-sum gradient vs central finite differences of the FP64 forward
-  (h = 1e-4 um); the FD quotient is compared with the gradient contracted
-  with the displacement actually applied after float32 rounding of the
-  coordinates, so the step is exact.
+  (h = 1e-4 um, 64 seeded points along seeded directions); the FD quotient is
+  compared with the gradient contracted with the displacement actually
+  applied after float32 rounding of the coordinates, so the step is exact.
+  With every point displaced, a Richardson extrapolation over h and h/2
+  removes the curvature term.
 * Tree gradient (FP32 leaf, exact adjoint of the tree) vs the FP64 gradient.
 * Cuticle-tilt gradient through the differentiable fiber generator vs a
   Richardson-extrapolated central difference of the FP64 forward.
@@ -40,8 +42,9 @@ def _setup(n=3000, seed=0):
 
 
 def test_fp64_vjp_matches_central_fd():
+    """64 seeded points displaced along seeded random 3D directions, h = 1e-4 um."""
     s, dev, target = _setup()
-    print("case: fp64 vjp vs FD n=3000", flush=True)
+    print("case: fp64 vjp vs FD n=3000, 64 displaced points", flush=True)
     with wf.POSolver(device=0) as sv:
         x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(dev)]
         I = po_intensity(x, INCS, GRID, 0, sv, fp64=True)
@@ -50,7 +53,9 @@ def test_fp64_vjp_matches_central_fd():
         gx = torch.stack([x[0].grad, x[1].grad, x[2].grad]).double()
         rng = np.random.default_rng(5)
         for trial in range(3):
-            v = torch.from_numpy(rng.standard_normal((3, s.n))).cuda()
+            v = torch.zeros((3, s.n), dtype=torch.float64, device="cuda")
+            idx = torch.from_numpy(rng.choice(s.n, 64, replace=False)).cuda()
+            v[:, idx] = torch.from_numpy(rng.standard_normal((3, 64))).cuda()
             h = 1e-10  # 1e-4 um
             xp = [(dev[c].double() + h * v[c]).float() for c in range(3)]
             xm = [(dev[c].double() - h * v[c]).float() for c in range(3)]
@@ -61,6 +66,31 @@ def test_fp64_vjp_matches_central_fd():
             err = abs((Lp - Lm) - pred) / abs(Lp - Lm)
             print(f"  trial {trial}: FD {Lp - Lm:.6e} adjoint {pred:.6e} rel {err:.2e}", flush=True)
             assert err < 1e-4
+
+
+def test_fp64_vjp_all_points_richardson():
+    """Every point displaced at once: the central difference carries an O(h^2)
+    term from the curvature of log I near pattern nulls, so the ratio
+    FD / (gradient . displacement) is extrapolated from h and h/2."""
+    s, dev, target = _setup()
+    print("case: fp64 vjp vs Richardson FD, all 3000 points displaced", flush=True)
+    with wf.POSolver(device=0) as sv:
+        x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(dev)]
+        L = log_intensity_loss(po_intensity(x, INCS, GRID, 0, sv, fp64=True), target)
+        L.backward()
+        gx = torch.stack([x[0].grad, x[1].grad, x[2].grad]).double()
+        v = torch.from_numpy(np.random.default_rng(6).standard_normal((3, s.n))).cuda()
+        ratios = []
+        for h in (2e-10, 1e-10):
+            xp = [(dev[c].double() + h * v[c]).float() for c in range(3)]
+            xm = [(dev[c].double() - h * v[c]).float() for c in range(3)]
+            dL = _loss64(sv, xp + dev[3:], target) - _loss64(sv, xm + dev[3:], target)
+            dx = torch.stack([xp[c].double() - xm[c].double() for c in range(3)])
+            ratios.append(dL / float((gx * dx).sum().item()))
+        r = (4 * ratios[1] - ratios[0]) / 3
+        print(f"  FD/adjoint at h, h/2: {ratios[0]:.6f} {ratios[1]:.6f}; extrapolated {r:.7f}",
+              flush=True)
+        assert abs(r - 1) < 1e-4
 
 
 def test_normal_displacement_gradient():

@@ -89,9 +89,23 @@ def test_octree_matches_reference_partition(solver, depth):
     np.testing.assert_array_equal(ne, re_)
 
 
+@pytest.fixture(params=[0, 1], ids=["agg_mfma", "agg_fft"])
+def agg_solver(request):
+    """The session solver with the level-transition kernels selected: dense
+    split-BF16 MFMA interpolation (0) or the LDS FFT kernels (1)."""
+    import wavefiber3d_amd as wfm
+
+    s = wfm.POSolver(device=0)
+    s.tune = {"agg_kernel": request.param}
+    yield s
+    s.close()
+
+
 @pytest.mark.parametrize("ntc,npc,ntp,npp,off", [
-    (9, 16, 15, 28, None), (12, 20, 20, 36, None), (10, 18, 7, 12, "out"), (14, 26, 30, 60, "out")])
-def test_interp_matches_bruteforce(solver, ntc, npc, ntp, npp, off):
+    (9, 16, 15, 28, None), (12, 20, 20, 36, None), (10, 18, 7, 12, "out"), (14, 26, 30, 60, "out"),
+    (24, 48, 40, 80, None), (21, 35, 11, 22, "out")])
+def test_interp_matches_bruteforce(agg_solver, ntc, npc, ntp, npp, off):
+    solver = agg_solver
     rng = np.random.default_rng(ntc * 100 + npp)
     # band-limited pattern: a few point sources radiating on the child grid
     off_c = -np.pi / 2 + np.pi / (2 * ntc)
@@ -114,10 +128,11 @@ def test_interp_matches_bruteforce(solver, ntc, npc, ntp, npp, off):
 
 
 @pytest.mark.parametrize("depth", [1, 2, 3])
-def test_tree_sphere_vs_fp64(solver, depth):
+def test_tree_sphere_vs_fp64(agg_solver, depth):
+    solver = agg_solver
     s = wf.fibonacci_sphere(12000, 2e-6)
     nt, np_ = 90, 180
-    print(f"case: tree sphere n=12000 depth={depth}", flush=True)
+    print(f"case: tree sphere n=12000 depth={depth} agg {solver.tune['agg_kernel']}", flush=True)
     solver.set_geometry(s, depth)
     I = solver.solve(INCS, (nt, np_))
     Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS], nt, np_)
@@ -136,7 +151,8 @@ def test_tree_complex_E_vs_fp64(solver):
         assert rel(E[k].reshape(3, -1), Eref[k]) < 1e-3
 
 
-def test_tree_ellipsoid_fixture(solver):
+def test_tree_ellipsoid_fixture(agg_solver):
+    solver = agg_solver
     import os
 
     g = os.path.join(os.path.dirname(__file__), "golden")

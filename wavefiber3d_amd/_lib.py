@@ -107,6 +107,7 @@ class Tune(ct.Structure):
         ("level_s", ct.c_double),
         ("leaf_quad", ct.c_int32),
         ("chunk_solves", ct.c_int32),
+        ("agg_kernel", ct.c_int32),
         ("calibrated", ct.c_int32),
     ]
 

@@ -64,6 +64,13 @@ struct FftEntry {
   float2* tw = nullptr;
 };
 
+// Split-BF16 interpolation matrix of one (input length, output length, shift).
+struct MatEntry {
+  uint16_t* hi = nullptr;
+  uint16_t* lo = nullptr;
+  int rows_pad = 0, kpad = 0;
+};
+
 // A captured solve; valid while the workspace it was recorded on has not
 // moved (ws_gen) and the call signature matches.
 struct GraphCache {
@@ -89,6 +96,7 @@ struct wf_handle {
   size_t bytes_in_use = 0;
   std::map<std::string, DevBuf> bufs;
   std::map<int, FftEntry> ffts;
+  std::map<std::pair<std::pair<int, int>, uint64_t>, MatEntry> mats;
   // geometry
   bool has_geom = false;
   int64_t n = 0;
@@ -229,6 +237,29 @@ wf_status get_fft(wf_handle* h, int n, FftPlanDev* out) {
   return WF_OK;
 }
 
+// Interpolation matrix from nc samples to np samples with output origin
+// shifted by `shift` (radians of the period-2pi parameter), cached per handle.
+wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out) {
+  uint64_t sb;
+  memcpy(&sb, &shift, sizeof sb);
+  const auto key = std::make_pair(std::make_pair(nc, np), sb);
+  auto it = h->mats.find(key);
+  if (it != h->mats.end()) {
+    *out = it->second;
+    return WF_OK;
+  }
+  std::vector<uint16_t> hi, lo;
+  MatEntry e;
+  wf::build_interp_matrix(nc, np, shift, hi, lo, e.rows_pad, e.kpad);
+  WF_HIP(hipMalloc(&e.hi, sizeof(uint16_t) * hi.size()));
+  WF_HIP(hipMalloc(&e.lo, sizeof(uint16_t) * lo.size()));
+  WF_HIP(hipMemcpy(e.hi, hi.data(), sizeof(uint16_t) * hi.size(), hipMemcpyHostToDevice));
+  WF_HIP(hipMemcpy(e.lo, lo.data(), sizeof(uint16_t) * lo.size(), hipMemcpyHostToDevice));
+  h->mats[key] = e;
+  *out = e;
+  return WF_OK;
+}
+
 // Incidence parameters in FP64 (treePO.cu:501-509, :580-601).
 WfIncDev make_inc(const wf_incidence& in) {
   WfIncDev d;
@@ -338,6 +369,7 @@ void default_tune(wf_tune* t) {
   t->level_s = 2.0e-6;
   t->leaf_quad = 1;
   t->chunk_solves = 4;
+  t->agg_kernel = 0;
   t->calibrated = 0;
 }
 
@@ -502,6 +534,10 @@ void wf_destroy(wf_handle* h) {
     if (kv.second.p) hipFree(kv.second.p);
   for (auto& kv : h->ffts)
     if (kv.second.tw) hipFree(kv.second.tw);
+  for (auto& kv : h->mats) {
+    if (kv.second.hi) hipFree(kv.second.hi);
+    if (kv.second.lo) hipFree(kv.second.lo);
+  }
   if (h->inc_host) hipHostFree(h->inc_host);
   if (h->inc_evt) hipEventDestroy(h->inc_evt);
   for (auto ev : h->evpool) hipEventDestroy(ev);
@@ -695,9 +731,35 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   WF_TRY(get_fft(h, 2 * ntp, &tp));
   float2* G = nullptr;
   WF_TRY(dalloc_t(h, "interp_G", (size_t)count * ntc * npp, &G));
-  WF_HIP(wf::launch_phi_pass((const float2*)child, G, count, ntc, npc, npp, pc, pp, st));
   const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
   const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
+  if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp)) {
+    MatEntry mp, mt;
+    WF_TRY(get_mat(h, npc, npp, 0.0, &mp));
+    WF_TRY(get_mat(h, 2 * ntc, 2 * ntp, off_tp - off_tc, &mt));
+    WF_HIP(wf::launch_phi_mfma((const float2*)child, G, (int64_t)count * ntc, ntc, npc, npp, mp.hi,
+                               mp.lo, mp.kpad, st));
+    wf::ThetaMfmaArgs ta;
+    ta.GT = G;
+    ta.F = 1;
+    ta.ntc = ntc;
+    ta.npp = npp;
+    ta.ntp = ntp;
+    ta.nparents = count;
+    ta.cb = nullptr;
+    ta.ce = nullptr;
+    ta.child_key = nullptr;
+    ta.Text = nullptr;
+    ta.Mhi = mt.hi;
+    ta.Mlo = mt.lo;
+    ta.kpad = mt.kpad;
+    ta.mirror = mirror;
+    ta.out = (float2*)parent;
+    ta.out_root = nullptr;
+    WF_HIP(wf::launch_theta_mfma(ta, st));
+    return WF_OK;
+  }
+  WF_HIP(wf::launch_phi_pass((const float2*)child, G, count, ntc, npc, npp, pc, pp, st));
   WF_HIP(wf::launch_theta_pass(G, 1, ntc, npp, ntp, count, nullptr, nullptr, nullptr, nullptr,
                                off_tp - off_tc, mirror, tc, tp, (float2*)parent, nullptr, st));
   return WF_OK;
@@ -782,7 +844,15 @@ struct GroupCtx {
   std::vector<Grid> g;
   float2* T = nullptr;
   std::vector<int64_t> toff;
+  float2* Tx = nullptr;  // extended tables of the MFMA level transition
+  std::vector<int64_t> txoff;
 };
+
+double theta_origin(int l, int nt) {
+  return (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * nt));
+}
+
+int mirror_of(int nt, double off) { return (int)std::lround(nt * (1.0 - 2.0 * off / WF_PI)); }
 
 // Output grids of a tree solve: even n_phi (theta extension), FFT lengths.
 wf_status check_tree_grid(wf_grid grid) {
@@ -858,6 +928,28 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
     WF_TRY(timed(h, st, WF_K_TRANS, 64.0 * gl.nt * gl.np,
                  [&] { return wf::launch_trans_table(Tl, gl.nt, gl.np, th0, hc, k0, st); }));
   }
+  if (h->tune.agg_kernel == 0) {
+    gc.txoff.assign(depth + 1, 0);
+    int64_t tx = 0;
+    for (int l = 0; l < depth; ++l) {
+      gc.txoff[l] = tx;
+      tx += 8LL * (2 * gc.g[l].nt + 1) * (gc.g[l].np / 2);
+    }
+    WF_TRY(dalloc_t(h, "trans_ext", (size_t)tx, &gc.Tx));
+    for (int l = 0; l < depth; ++l) {
+      const Grid gl = gc.g[l];
+      const double th0 = theta_origin(l, gl.nt);
+      const int mirror = mirror_of(gl.nt, th0);
+      const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
+                            0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
+                            0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
+      float2* Tl = gc.Tx + gc.txoff[l];
+      const double k0 = gc.k0;
+      WF_TRY(timed(h, st, WF_K_TRANS, 8.0 * 8 * (2 * gl.nt + 1) * (gl.np / 2), [&] {
+        return wf::launch_trans_ext(Tl, gl.nt, gl.np, mirror, th0, hc, k0, st);
+      }));
+    }
+  }
   for (int l = 0; l <= depth && l < 16; ++l) {
     h->stats.level_nt[l] = gc.g[l].nt;
     h->stats.level_np[l] = gc.g[l].np;
@@ -866,12 +958,6 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
   h->stats.n_levels = depth + 1;
   return WF_OK;
 }
-
-double theta_origin(int l, int nt) {
-  return (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * nt));
-}
-
-int mirror_of(int nt, double off) { return (int)std::lround(nt * (1.0 - 2.0 * off / WF_PI)); }
 
 wf::LeafArgs leaf_args(wf_handle* h, const GroupCtx& gc, const float2* C, int F, float2* out) {
   const int depth = h->depth;
@@ -962,6 +1048,40 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
     const int mirror = mirror_of(gp.nt, off_p);
     const float2* Tl = gc.T + gc.toff[l];
+    if (h->tune.agg_kernel == 0) {
+      // dense split-BF16 MFMA interpolation: phi GEMM -> GT, theta GEMM +
+      // translation + child sum -> parent (FP64 at the root)
+      MatEntry mp, mt;
+      WF_TRY(get_mat(h, gcg.np, gp.np, 0.0, &mp));
+      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt));
+      const int64_t rows = (int64_t)nchild * F * gcg.nt;
+      WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
+        return wf::launch_phi_mfma(cur, G, rows, gcg.nt, gcg.np, gp.np, mp.hi, mp.lo, mp.kpad, st);
+      }));
+      float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
+      wf::ThetaMfmaArgs ta;
+      ta.GT = G;
+      ta.F = F;
+      ta.ntc = gcg.nt;
+      ta.npp = gp.np;
+      ta.ntp = gp.nt;
+      ta.nparents = npar;
+      ta.cb = h->levels[l].cb;
+      ta.ce = h->levels[l].ce;
+      ta.child_key = h->levels[l + 1].key;
+      ta.Text = gc.Tx + gc.txoff[l];
+      ta.Mhi = mt.hi;
+      ta.Mlo = mt.lo;
+      ta.kpad = mt.kpad;
+      ta.mirror = mirror;
+      ta.out = (l == 0) ? nullptr : nxt;
+      ta.out_root = (l == 0) ? F0 : nullptr;
+      const double wbytes = (l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np;
+      WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * gp.np + wbytes,
+                   [&] { return wf::launch_theta_mfma(ta, st); }));
+      if (l > 0) cur = nxt;
+      continue;
+    }
     if (l > 0 && wf::fused_fits(gcg.nt, gcg.np, gp.nt, gp.np)) {
       // one kernel: child read once, parent written once
       float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;

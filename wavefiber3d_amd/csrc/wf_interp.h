@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "wf_fft.h"
 
 namespace wf {
@@ -31,6 +33,37 @@ hipError_t launch_agg_fused(const float2* child, float2* parent, int F, int ntc,
 
 hipError_t launch_trans_table(float2* T, int ntp, int npp, double theta0, const double h[3],
                               double k0, hipStream_t st);
+
+// Dense MFMA level transition (wf_agg_mfma.hip).  Interpolation matrices are
+// real [rows_pad][kpad] (row q = output sample, column j = input sample),
+// stored as split BF16 bit patterns (hi, lo).
+void build_interp_matrix(int nc, int np, double shift, std::vector<uint16_t>& hi,
+                         std::vector<uint16_t>& lo, int& rows_pad, int& kpad);
+int interp_rows_pad(int rows);
+int interp_kpad(int k);
+
+// phi pass: X [rows = nodes*F*ntc][npc] -> GT [nodes*F][npp][ntc] (theta contiguous).
+hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
+                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st);
+
+struct ThetaMfmaArgs {
+  const float2* GT;  // [child][F][npp][ntc]
+  int F, ntc, npp, ntp, nparents;
+  const int32_t* cb;  // per-parent child range (null: parent p <- child p)
+  const int32_t* ce;
+  const uint32_t* child_key;  // octant = key & 7 (nullable)
+  const float2* Text;         // [8][2 ntp + 1][npp/2] (null: no translation)
+  const uint16_t* Mhi;        // theta matrix [rows_pad >= 2 ntp][kpad >= 2 ntc]
+  const uint16_t* Mlo;
+  int kpad;
+  int mirror;  // 2 ntp - 1 (half-sample grid) or 2 ntp (sample at the pole)
+  float2* out;       // [parent][F][ntp][npp], or
+  double2* out_root; // FP64 accumulation target of the root level
+};
+hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
+
+hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
+                            const double h[3], double k0, hipStream_t st);
 
 // Leaf radiation sum: pattern[leaf][F][nt][np] of
 // sum_{n in leaf} C[n][f] exp(+i k0 s.(x_n - c_leaf)) on the leaf grid
