@@ -23,11 +23,11 @@
 //
 // Both run on v_mfma_f32_16x16x32_bf16 with split-BF16 x3 operands (matrix
 // split on the host, data split in registers), FP32 accumulation; the root
-// level accumulates the child sum in FP64.  No LDS: the matrix fragments come
-// from L1/L2 (one small matrix per level), the data fragments straight from
-// HBM in the MFMA lane layout.  The phi output is stored theta-contiguous
-// (GT[cf][q][theta]) so that the theta pass reads 8 consecutive theta samples
-// per lane.
+// level accumulates the child sum in FP64.  The matrix fragments come from
+// L1/L2 (one small matrix per level).  The phi pass reads its data fragments
+// straight from HBM in the MFMA lane layout and stores theta-contiguous
+// (GT[cf][q][theta]); the theta pass stages each 32-sample step of the
+// great-circle sequences once per block in LDS, pre-split, for all waves.
 #include <stdint.h>
 
 #include <cmath>
@@ -139,155 +139,186 @@ struct AccT<true> {
   typedef double2 type;
 };
 
-// theta pass + translation + child sum.  Wave task = (parent p, field f,
-// EG tiles of 16 great-circle rows e, KT tiles of 16 column pairs k); the
-// child loop runs inside, the parent accumulates in registers in the D
-// layout (lane: column k, rows 4g + j).  Row e of the extended grid feeds
-// parent (e, k) when e < ntp and (mirror - e mod 2ntp, k + npp/2) when that
-// row is < ntp; both hold only for e = 0 on grids with a sample at the pole
-// (mirror = 2 ntp), which keeps a second accumulator.
-template <int EG, int KT, bool ROOT>
+// theta pass + translation + child sum, block-tiled.  Block = 4 waves in
+// WE x WC, wave tile ET x CT 16-tiles: BE = 16 WE ET great-circle rows e by
+// BC = 16 WC CT column pairs k of one (parent p, field f); the child loop runs
+// inside.  Per 32-K step the B operand (the child's great-circle sequences of
+// the block's columns) is loaded once, coalesced, split into BF16 hi/lo and
+// staged in LDS for all waves; the A operand (matrix rows) comes from L1/L2.
+// Loads are branchless (clamped index + select).  The parent accumulates in
+// registers in the D layout (lane: column k, rows 4g + j); row e of the
+// extended grid feeds parent (e, k) when e < ntp and (mirror - e, k + npp/2)
+// otherwise when that row is < ntp; with a sample at the pole (mirror =
+// 2 ntp) row 0 also feeds (0, k + npp/2), kept in a second accumulator.
+template <int ET, int CT, int WE, int WC, bool ROOT>
 __global__ __launch_bounds__(256) void theta_mfma_kernel(
     const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, const int32_t* __restrict__ cb,
     const int32_t* __restrict__ ce, const uint32_t* __restrict__ child_key,
     const float2* __restrict__ Text, const bf16x8* __restrict__ Mhi,
-    const bf16x8* __restrict__ Mlo, int kpad, int mirror, int neg, int nkg, int64_t ntasks,
-    float2* __restrict__ out, double2* __restrict__ out_root) {
+    const bf16x8* __restrict__ Mlo, int kpad, int mrows, int mirror, int neb, int ncb,
+    int64_t total, float2* __restrict__ out, double2* __restrict__ out_root) {
   typedef typename AccT<ROOT>::type acc_t;
+  constexpr int BE = WE * ET * 16, BC = WC * CT * 16;
+  __shared__ bf16x8 Bs[BC][4][4];  // [column][k group][re hi, re lo, im hi, im lo]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t task = (int64_t)blockIdx.x * 4 + wave;
-  if (task >= ntasks) return;  // no barriers below
-  const int eg = (int)(task % neg);
-  int64_t t2 = task / neg;
-  const int kg = (int)(t2 % nkg);
-  t2 /= nkg;
-  const int f = (int)(t2 % F);
-  const int p = (int)(t2 / F);
   const int c16 = lane & 15, g = lane >> 4;
+  const int we = wave / WC, wc = wave % WC;
+  // XCD-aware mapping: blocks b = x (mod 8) run on XCD x and take a
+  // contiguous range of work items, so the e/k blocks of one (p, f) -- which
+  // read the same child data -- share an L2
+  const int64_t nb = gridDim.x, per = (nb + 7) / 8;
+  const int64_t w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (w >= total) return;  // whole block
+  const int cbk = (int)(w % ncb);
+  int64_t t = w / ncb;
+  const int eb = (int)(t % neb);
+  t /= neb;
+  const int f = (int)(t % F), p = (int)(t / F);
   const int half = npp / 2, n2c = 2 * ntc, n2p = 2 * ntp;
-  const int i2_0 = mirror % n2p;  // row fed by e = 0 in column k + half
+  const int e0 = eb * BE + we * ET * 16;
+  const int k0 = cbk * BC + wc * CT * 16;
+  const int i2_0 = mirror % n2p;
   const bool dual = (i2_0 < ntp);
-  acc_t acc[EG][KT][4];
-  acc_t acc2[KT];
+  acc_t acc[ET][CT][4];
+  acc_t acc2[CT];
 #pragma unroll
-  for (int et = 0; et < EG; ++et)
+  for (int a = 0; a < ET; ++a)
 #pragma unroll
-    for (int kt = 0; kt < KT; ++kt)
+    for (int b = 0; b < CT; ++b)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        acc[et][kt][j].x = 0;
-        acc[et][kt][j].y = 0;
+        acc[a][b][j].x = 0;
+        acc[a][b][j].y = 0;
       }
 #pragma unroll
-  for (int kt = 0; kt < KT; ++kt) {
-    acc2[kt].x = 0;
-    acc2[kt].y = 0;
+  for (int b = 0; b < CT; ++b) {
+    acc2[b].x = 0;
+    acc2[b].y = 0;
   }
+  // loader role (fixed per thread): column lc, k group lkg
+  const bool ldr = threadIdx.x < BC * 4;
+  const int lc = (threadIdx.x >> 2) % BC, lkg = threadIdx.x & 3;
+  const int lk = cbk * BC + lc;
+  const bool lkok = ldr && lk < half;
+  const int off1 = (lkok ? lk : 0) * ntc;                   // column k, theta ascending
+  const int off2 = (lkok ? lk + half : 0) * ntc + n2c - 1;  // column k + half, reversed
+  const bf16x8* ah_p[ET];
+  const bf16x8* al_p[ET];
+#pragma unroll
+  for (int a = 0; a < ET; ++a) {
+    const int row = e0 + 16 * a + c16;
+    const int rr = row < mrows ? row : 0;
+    ah_p[a] = Mhi + ((rr * kpad) >> 3);
+    al_p[a] = Mlo + ((rr * kpad) >> 3);
+  }
+  const int64_t slab = (int64_t)npp * ntc;
   const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
   for (int c = c0; c < c1; ++c) {
-    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
-    floatx4 ur[EG][KT], ui[EG][KT];
+    const float2* S = GT + ((int64_t)c * F + f) * slab;
+    floatx4 ur[ET][CT], ui[ET][CT];
 #pragma unroll
-    for (int et = 0; et < EG; ++et)
+    for (int a = 0; a < ET; ++a)
 #pragma unroll
-      for (int kt = 0; kt < KT; ++kt) {
-        ur[et][kt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        ui[et][kt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int b = 0; b < CT; ++b) {
+        ur[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        ui[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
       }
-    for (int k0 = 0; k0 < n2c; k0 += 32) {
-      const int jb = k0 + 8 * g;
-      bf16x8 brh[KT], brl[KT], bih[KT], bil[KT];
-#pragma unroll
-      for (int kt = 0; kt < KT; ++kt) {
-        const int col = (kg * KT + kt) * 16 + c16;
+    for (int k0s = 0; k0s < n2c; k0s += 32) {
+      __syncthreads();  // previous step's fragments consumed
+      if (ldr) {
+        const int jb = k0s + 8 * lkg;
         float vr[8], vi[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int jp = jb + j;
-          float2 v = make_float2(0.f, 0.f);
-          if (col < half) {
-            if (jp < ntc)
-              v = S[(int64_t)col * ntc + jp];
-            else if (jp < n2c)
-              v = S[(int64_t)(col + half) * ntc + (n2c - 1 - jp)];
-          }
-          vr[j] = v.x;
-          vi[j] = v.y;
+          const bool ok = lkok && jp < n2c;
+          const float2 v = S[ok ? (jp < ntc ? off1 + jp : off2 - jp) : 0];
+          vr[j] = ok ? v.x : 0.f;
+          vi[j] = ok ? v.y : 0.f;
         }
-        split8(vr, brh[kt], brl[kt]);
-        split8(vi, bih[kt], bil[kt]);
+        bf16x8 a0, a1, a2, a3;
+        split8(vr, a0, a1);
+        split8(vi, a2, a3);
+        Bs[lc][lkg][0] = a0;
+        Bs[lc][lkg][1] = a1;
+        Bs[lc][lkg][2] = a2;
+        Bs[lc][lkg][3] = a3;
+      }
+      __syncthreads();
+      const int kq = (k0s >> 3) + g;
+      bf16x8 ah[ET], al[ET];
+#pragma unroll
+      for (int a = 0; a < ET; ++a) {
+        ah[a] = ah_p[a][kq];
+        al[a] = al_p[a][kq];
       }
 #pragma unroll
-      for (int et = 0; et < EG; ++et) {
-        const int e = (eg * EG + et) * 16 + c16;  // < padded rows of Mth
-        const int64_t mi = ((int64_t)e * kpad + jb) >> 3;
-        const bf16x8 ah = Mhi[mi], al = Mlo[mi];
+      for (int b = 0; b < CT; ++b) {
+        const int col = wc * CT * 16 + 16 * b + c16;
+        const bf16x8 brh = Bs[col][g][0], brl = Bs[col][g][1], bih = Bs[col][g][2],
+                     bil = Bs[col][g][3];
 #pragma unroll
-        for (int kt = 0; kt < KT; ++kt) {
-          ur[et][kt] = mfma3(ah, al, brh[kt], brl[kt], ur[et][kt]);
-          ui[et][kt] = mfma3(ah, al, bih[kt], bil[kt], ui[et][kt]);
+        for (int a = 0; a < ET; ++a) {
+          ur[a][b] = mfma3(ah[a], al[a], brh, brl, ur[a][b]);
+          ui[a][b] = mfma3(ah[a], al[a], bih, bil, ui[a][b]);
         }
       }
     }
     const int oct = child_key ? (int)(child_key[c] & 7u) : 0;
-    const float2* To = Text ? Text + (int64_t)oct * (n2p + 1) * half : nullptr;
+    const float2* To = Text ? Text + oct * (n2p + 1) * half : nullptr;
 #pragma unroll
-    for (int kt = 0; kt < KT; ++kt) {
-      const int col = (kg * KT + kt) * 16 + c16;
-      if (col >= half) continue;
+    for (int b = 0; b < CT; ++b) {
+      const int col = k0 + 16 * b + c16;
+      const bool cok = col < half;
 #pragma unroll
-      for (int et = 0; et < EG; ++et)
+      for (int a = 0; a < ET; ++a)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int e = (eg * EG + et) * 16 + 4 * g + j;
-          if (e >= n2p) continue;
-          const float2 u = make_float2(ur[et][kt][j], ui[et][kt][j]);
-          const float2 tu = To ? cmul(To[(int64_t)e * half + col], u) : u;
-          acc[et][kt][j].x += tu.x;
-          acc[et][kt][j].y += tu.y;
-          if (e == 0 && dual) {
-            const float2 tu2 = To ? cmul(To[(int64_t)n2p * half + col], u) : u;
-            acc2[kt].x += tu2.x;
-            acc2[kt].y += tu2.y;
+          const int e = e0 + 16 * a + 4 * g + j;
+          const bool ok = cok && e < n2p;
+          const float2 u = make_float2(ur[a][b][j], ui[a][b][j]);
+          const float2 tu = To ? cmul(To[ok ? e * half + col : 0], u) : u;
+          acc[a][b][j].x += ok ? tu.x : 0.f;
+          acc[a][b][j].y += ok ? tu.y : 0.f;
+          if (dual && ok && e == 0) {
+            const float2 tu2 = To ? cmul(To[n2p * half + col], u) : u;
+            acc2[b].x += tu2.x;
+            acc2[b].y += tu2.y;
           }
         }
     }
   }
   // scatter to the parent grid
+  const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
 #pragma unroll
-  for (int kt = 0; kt < KT; ++kt) {
-    const int col = (kg * KT + kt) * 16 + c16;
+  for (int b = 0; b < CT; ++b) {
+    const int col = k0 + 16 * b + c16;
     if (col >= half) continue;
 #pragma unroll
-    for (int et = 0; et < EG; ++et)
+    for (int a = 0; a < ET; ++a)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int e = (eg * EG + et) * 16 + 4 * g + j;
+        const int e = e0 + 16 * a + 4 * g + j;
         if (e >= n2p) continue;
-        int row = -1, cc = col;
-        if (e < ntp) {
-          row = e;
-        } else {
-          int i2 = (mirror - e) % n2p;
-          if (i2 < 0) i2 += n2p;
-          if (i2 < ntp) {
-            row = i2;
-            cc = col + half;
-          }
+        int row = e, cc = col;
+        if (e >= ntp) {
+          int i2 = mirror - e;  // mirror is 2 ntp - 1 or 2 ntp: i2 in [0, ntp]
+          row = i2 < ntp ? i2 : -1;
+          cc = col + half;
         }
-        if (row < 0) continue;
-        const int64_t o = ((int64_t)p * F + f) * (int64_t)ntp * npp + (int64_t)row * npp + cc;
-        if (ROOT)
-          out_root[o] = make_double2((double)acc[et][kt][j].x, (double)acc[et][kt][j].y);
-        else
-          out[o] = make_float2((float)acc[et][kt][j].x, (float)acc[et][kt][j].y);
-        if (e == 0 && dual) {
-          const int64_t o2 =
-              ((int64_t)p * F + f) * (int64_t)ntp * npp + (int64_t)i2_0 * npp + col + half;
+        if (row >= 0) {
+          const int64_t o = obase + (int64_t)row * npp + cc;
           if (ROOT)
-            out_root[o2] = make_double2((double)acc2[kt].x, (double)acc2[kt].y);
+            out_root[o] = make_double2((double)acc[a][b][j].x, (double)acc[a][b][j].y);
           else
-            out[o2] = make_float2((float)acc2[kt].x, (float)acc2[kt].y);
+            out[o] = make_float2((float)acc[a][b][j].x, (float)acc[a][b][j].y);
+        }
+        if (dual && e == 0) {
+          const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
+          if (ROOT)
+            out_root[o2] = make_double2((double)acc2[b].x, (double)acc2[b].y);
+          else
+            out[o2] = make_float2((float)acc2[b].x, (float)acc2[b].y);
         }
       }
   }
@@ -336,19 +367,20 @@ __global__ __launch_bounds__(256) void trans_ext_kernel(float2* __restrict__ Tex
 
 int64_t blocks_for(int64_t tasks) { return (tasks + 3) / 4; }
 
-template <int EG, int KT, bool ROOT>
+template <int ET, int CT, int WE, int WC, bool ROOT>
 hipError_t launch_theta_t(const wf::ThetaMfmaArgs& a, hipStream_t st) {
+  constexpr int BE = WE * ET * 16, BC = WC * CT * 16;
   const int half = a.npp / 2;
-  const int neg = (2 * a.ntp + 16 * EG - 1) / (16 * EG);
-  const int nkg = (half + 16 * KT - 1) / (16 * KT);
-  const int64_t ntasks = (int64_t)a.nparents * a.F * neg * nkg;
-  const int64_t nb = blocks_for(ntasks);
+  const int neb = (2 * a.ntp + BE - 1) / BE, ncb = (half + BC - 1) / BC;
+  const int64_t total = (int64_t)a.nparents * a.F * neb * ncb;
+  if (total <= 0) return hipSuccess;
+  const int64_t nb = ((total + 7) / 8) * 8;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-  if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL((theta_mfma_kernel<EG, KT, ROOT>), dim3((unsigned)nb), dim3(256), 0, st,
+  const int mrows = wf::interp_rows_pad(2 * a.ntp);
+  hipLaunchKernelGGL((theta_mfma_kernel<ET, CT, WE, WC, ROOT>), dim3((unsigned)nb), dim3(256), 0, st,
                      a.GT, a.F, a.ntc, a.npp, a.ntp, a.cb, a.ce, a.child_key, a.Text,
                      reinterpret_cast<const bf16x8*>(a.Mhi), reinterpret_cast<const bf16x8*>(a.Mlo),
-                     a.kpad, a.mirror, neg, nkg, ntasks, a.out, a.out_root);
+                     a.kpad, mrows, a.mirror, neb, ncb, total, a.out, a.out_root);
   return hipGetLastError();
 }
 
@@ -420,13 +452,17 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
     return hipErrorInvalidValue;
   if (a.mirror != 2 * a.ntp - 1 && a.mirror != 2 * a.ntp) return hipErrorInvalidValue;
   if ((a.out_root != nullptr) == (a.out != nullptr)) return hipErrorInvalidValue;
-  const int ne = (2 * a.ntp + 15) / 16;
-  const bool root = a.out_root != nullptr;
-  // rows of the extended grid per wave: up to 4 tiles (64 rows), 2 column tiles
-  if (root) return launch_theta_t<2, 2, true>(a, st);  // FP64 accumulators
-  if (ne <= 2) return launch_theta_t<2, 2, false>(a, st);
-  if (ne <= 3) return launch_theta_t<3, 2, false>(a, st);
-  return launch_theta_t<4, 2, false>(a, st);
+  if ((int64_t)interp_rows_pad(2 * a.ntp) * a.kpad >= (1LL << 31) ||
+      (int64_t)a.npp * a.ntc >= (1LL << 31))
+    return hipErrorInvalidValue;
+  // block tile by extended-grid height (measured on the fiber level shapes):
+  // <= 96 rows: 64 x 16, <= 160: 64 x 32, <= 288: 96 x 32, else 64 x 64
+  const int n2p = 2 * a.ntp;
+  if (a.out_root != nullptr) return launch_theta_t<2, 2, 2, 2, true>(a, st);
+  if (n2p <= 96) return launch_theta_t<1, 1, 4, 1, false>(a, st);
+  if (n2p <= 160) return launch_theta_t<2, 1, 2, 2, false>(a, st);
+  if (n2p <= 288) return launch_theta_t<3, 1, 2, 2, false>(a, st);
+  return launch_theta_t<2, 2, 2, 2, false>(a, st);
 }
 
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
