@@ -244,8 +244,10 @@ typedef struct wf_tune {
   int32_t leaf_quad;      /* 1: quad-symmetric BF16x3 leaf kernel where eligible (<= 4
                              solves per tree pass); 0: plain */
   int32_t chunk_solves;   /* solves per tree pass (1..8); opts.field_chunk overrides */
-  int32_t agg_kernel;     /* level transitions: 0 = dense split-BF16 MFMA interpolation,
-                             1 = LDS FFT interpolation */
+  int32_t agg_kernel;     /* level transitions: 0 = dense split-BF16 MFMA interpolation for
+                             levels whose extended parent grid (2 Nt) has at most agg_fft_rows
+                             rows and the LDS FFT above; 1 = the original LDS FFT kernels */
+  int32_t agg_fft_rows;   /* see agg_kernel (default 600) */
   int32_t calibrated;     /* 1 after wf_autotune (or a wf_set_tune of a calibrated tune) */
 } wf_tune;
 

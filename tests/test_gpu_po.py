@@ -89,21 +89,23 @@ def test_octree_matches_reference_partition(solver, depth):
     np.testing.assert_array_equal(ne, re_)
 
 
-@pytest.fixture(params=[0, 1], ids=["agg_mfma", "agg_fft"])
+@pytest.fixture(params=[(0, 100000), (0, 0), (1, 288)], ids=["agg_mfma", "agg_fft2", "agg_fft"])
 def agg_solver(request):
-    """The session solver with the level-transition kernels selected: dense
-    split-BF16 MFMA interpolation (0) or the LDS FFT kernels (1)."""
+    """A solver with the level-transition kernels selected: dense split-BF16
+    MFMA interpolation on every level, the large-grid LDS FFT kernels on every
+    level (5-smooth grids), or the original LDS FFT kernels."""
     import wavefiber3d_amd as wfm
 
     s = wfm.POSolver(device=0)
-    s.tune = {"agg_kernel": request.param}
+    s.tune = {"agg_kernel": request.param[0], "agg_fft_rows": request.param[1]}
     yield s
     s.close()
 
 
 @pytest.mark.parametrize("ntc,npc,ntp,npp,off", [
     (9, 16, 15, 28, None), (12, 20, 20, 36, None), (10, 18, 7, 12, "out"), (14, 26, 30, 60, "out"),
-    (24, 48, 40, 80, None), (21, 35, 11, 22, "out")])
+    (24, 48, 40, 80, None), (21, 35, 11, 22, "out"), (45, 80, 75, 144, None),
+    (36, 60, 90, 180, "out"), (30, 50, 25, 40, None)])
 def test_interp_matches_bruteforce(agg_solver, ntc, npc, ntp, npp, off):
     solver = agg_solver
     rng = np.random.default_rng(ntc * 100 + npp)

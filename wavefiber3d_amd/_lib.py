@@ -108,6 +108,7 @@ class Tune(ct.Structure):
         ("leaf_quad", ct.c_int32),
         ("chunk_solves", ct.c_int32),
         ("agg_kernel", ct.c_int32),
+        ("agg_fft_rows", ct.c_int32),
         ("calibrated", ct.c_int32),
     ]
 

@@ -1,0 +1,264 @@
+// wf_agg_fft.hip -- level transitions with large angular grids through the
+// LDS FFT of wf_fft2.h (rows A9, A11, A12).
+//
+// For long lines the dense interpolation GEMM has to re-read either the data
+// or its (rows x columns) matrix many times; the FFT route reads each child
+// sample once and writes each parent sample once:
+//
+//   phi_fft:   per block RB theta rows of one (child, field): FFT(npc) ->
+//              resample to npp (pad / fold, Nyquist halved, 1/npc) ->
+//              inverse FFT(npp) -> GT[cf][q][theta] (theta contiguous).
+//   theta_fft: per block W column pairs of one (parent, field); for each
+//              child the great-circle sequences (column k ascending, column
+//              k + npp/2 reversed) -> FFT(2 ntc) -> resample with the origin
+//              shift -> inverse FFT(2 ntp) -> times the translation table of
+//              the child's octant -> parent accumulators in registers (FP64
+//              at the root).  Same extended-row mapping as the MFMA path.
+#include "wf_common.h"
+#include "wf_fft2.h"
+#include "wf_interp.h"
+
+namespace {
+
+constexpr int kLineBuf = 4096;  // complex elements per LDS ping-pong buffer
+
+template <bool ROOT>
+struct FftAcc {
+  typedef float2 type;
+};
+template <>
+struct FftAcc<true> {
+  typedef double2 type;
+};
+
+// Spectral resample of nlines lines: X (length nc spectra, stride xs) -> Y
+// (length np, stride ys), Y[q] = sum_{f = q mod np, f in band(nc)} fac[f - fmin] X[f mod nc]
+// with fac[] = w_f exp(i f shift) / nc precomputed in LDS.
+__device__ inline void resample(const float2* __restrict__ X, float2* __restrict__ Y, int nlines,
+                                int xs, int ys, int nc, int np, const float2* __restrict__ fac) {
+  const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
+  const uint32_t mag = fmagic((uint32_t)np);
+  const int total = nlines * np;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int line = (int)fdiv((uint32_t)t, mag);
+    const int q = t - line * np;
+    const float2* x = X + line * xs;
+    // smallest f >= fmin with f = q (mod np): f = q - np * floor((q - fmin) / np)
+    const int f0 = q - np * (int)fdiv((uint32_t)(q - fmin), mag);
+    float2 acc = make_float2(0.f, 0.f);
+    for (int f = f0; f <= fmax; f += np) {
+      const int bin = f < 0 ? f + nc : f;
+      acc = cadd(acc, cmul(x[bin], fac[f - fmin]));
+    }
+    Y[line * ys + q] = acc;
+  }
+}
+
+__device__ inline void load_table(float2* dst, const float2* src, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src[t];
+}
+
+__global__ __launch_bounds__(256) void phi_fft_kernel(const float2* __restrict__ X,
+                                                      float2* __restrict__ GT, int ntc, int npc,
+                                                      int npp, int RB, Fft2Plan pc, Fft2Plan pp,
+                                                      const float2* __restrict__ fac_g) {
+  extern __shared__ float2 sm[];
+  const int lmax = npc > npp ? npc : npp;
+  float2* a = sm;
+  float2* b = a + RB * lmax;
+  float2* twc = b + RB * lmax;
+  float2* twp = twc + pc.twn;
+  float2* fac = twp + pp.twn;
+  load_table(twc, pc.tw, pc.twn);
+  load_table(twp, pp.tw, pp.twn);
+  load_table(fac, fac_g, npc + 1);
+  const int rblocks = (ntc + RB - 1) / RB;
+  const int64_t cf = blockIdx.x / rblocks;
+  const int r0 = (int)(blockIdx.x - cf * rblocks) * RB;
+  const int nr = min(RB, ntc - r0);
+  const float2* src = X + (cf * ntc + r0) * (int64_t)npc;
+  for (int t = threadIdx.x; t < nr * npc; t += blockDim.x) {
+    const int r = t / npc, k = t - r * npc;
+    a[r * lmax + k] = src[t];
+  }
+  __syncthreads();
+  float2* Xs = wffft2::lines(a, b, nr, lmax, pc, twc, 0);
+  float2* Ys = (Xs == a) ? b : a;
+  resample(Xs, Ys, nr, lmax, lmax, npc, npp, fac);
+  __syncthreads();
+  float2* Z = wffft2::lines(Ys, Xs, nr, lmax, pp, twp, 1);
+  // GT[cf][q][r0 + r]: consecutive r contiguous
+  float2* dst = GT + cf * (int64_t)npp * ntc + r0;
+  for (int t = threadIdx.x; t < nr * npp; t += blockDim.x) {
+    const int q = t / nr, r = t - q * nr;
+    dst[(int64_t)q * ntc + r] = Z[r * lmax + q];
+  }
+}
+
+template <int MAXS, bool ROOT>
+__global__ __launch_bounds__(256) void theta_fft_kernel(
+    const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, int W,
+    const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
+    const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int mirror,
+    Fft2Plan pc, Fft2Plan pp, const float2* __restrict__ fac_g, float2* __restrict__ out,
+    double2* __restrict__ out_root) {
+  typedef typename FftAcc<ROOT>::type acc_t;
+  extern __shared__ float2 sm[];
+  const int n2c = 2 * ntc, n2p = 2 * ntp, half = npp / 2;
+  const int lmax = n2c > n2p ? n2c : n2p;
+  float2* a = sm;
+  float2* b = a + W * lmax;
+  float2* twc = b + W * lmax;
+  float2* twp = twc + pc.twn;
+  float2* fac = twp + pp.twn;
+  load_table(twc, pc.tw, pc.twn);
+  load_table(twp, pp.tw, pp.twn);
+  load_table(fac, fac_g, n2c + 1);
+  const int nwb = (half + W - 1) / W;
+  const int f = (int)(blockIdx.x % F);
+  const int64_t rest = blockIdx.x / F;
+  const int wb = (int)(rest % nwb);
+  const int p = (int)(rest / nwb);
+  const int k0 = wb * W;
+  const int nw = min(W, half - k0);
+  const int i2_0 = mirror % n2p;
+  const bool dual = (i2_0 < ntp);
+  // slot s = threadIdx.x + 256 m -> (column w = s / n2p, row e = s % n2p)
+  acc_t acc[MAXS], acc2[MAXS];  // acc2: second sample fed by e = 0 (pole grids)
+#pragma unroll
+  for (int m = 0; m < MAXS; ++m) {
+    acc[m].x = 0;
+    acc[m].y = 0;
+    acc2[m].x = 0;
+    acc2[m].y = 0;
+  }
+  const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
+  for (int c = c0; c < c1; ++c) {
+    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
+    __syncthreads();  // previous child done with a/b
+    for (int t = threadIdx.x; t < nw * n2c; t += blockDim.x) {
+      const int w = t / n2c, jp = t - w * n2c;
+      const int k = k0 + w;
+      a[w * lmax + jp] = (jp < ntc) ? S[(int64_t)k * ntc + jp]
+                                    : S[(int64_t)(k + half) * ntc + (n2c - 1 - jp)];
+    }
+    __syncthreads();
+    float2* Xs = wffft2::lines(a, b, nw, lmax, pc, twc, 0);
+    float2* Ys = (Xs == a) ? b : a;
+    resample(Xs, Ys, nw, lmax, lmax, n2c, n2p, fac);
+    __syncthreads();
+    float2* Z = wffft2::lines(Ys, Xs, nw, lmax, pp, twp, 1);
+    const int oct = child_key ? (int)(child_key[c] & 7u) : 0;
+    const float2* To = Text ? Text + (int64_t)oct * (n2p + 1) * half : nullptr;
+#pragma unroll
+    for (int m = 0; m < MAXS; ++m) {
+      const int s = threadIdx.x + 256 * m;
+      if (s < nw * n2p) {
+        const int w = s / n2p, e = s - w * n2p;
+        const float2 u = Z[w * lmax + e];
+        const float2 tu = To ? cmul(To[(int64_t)e * half + k0 + w], u) : u;
+        acc[m].x += tu.x;
+        acc[m].y += tu.y;
+        if (dual && e == 0) {
+          const float2 tu2 = To ? cmul(To[(int64_t)n2p * half + k0 + w], u) : u;
+          acc2[m].x += tu2.x;
+          acc2[m].y += tu2.y;
+        }
+      }
+    }
+  }
+  const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
+#pragma unroll
+  for (int m = 0; m < MAXS; ++m) {
+    const int s = threadIdx.x + 256 * m;
+    if (s >= nw * n2p) continue;
+    const int w = s / n2p, e = s - w * n2p;
+    const int col = k0 + w;
+    int row = e, cc = col;
+    if (e >= ntp) {
+      const int i2 = mirror - e;  // in [0, ntp]
+      row = i2 < ntp ? i2 : -1;
+      cc = col + half;
+    }
+    if (row >= 0) {
+      const int64_t o = obase + (int64_t)row * npp + cc;
+      if (ROOT)
+        out_root[o] = make_double2((double)acc[m].x, (double)acc[m].y);
+      else
+        out[o] = make_float2((float)acc[m].x, (float)acc[m].y);
+    }
+    if (dual && e == 0) {
+      const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
+      if (ROOT)
+        out_root[o2] = make_double2((double)acc2[m].x, (double)acc2[m].y);
+      else
+        out[o2] = make_float2((float)acc2[m].x, (float)acc2[m].y);
+    }
+  }
+}
+
+template <bool ROOT>
+hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_t sh,
+                              hipStream_t st) {
+  const int half = a.npp / 2;
+  const int64_t nblk = (int64_t)a.nparents * a.F * ((half + W - 1) / W);
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+#define WF_THETA_FFT(MS)                                                                          \
+  do {                                                                                            \
+    hipError_t e = hipFuncSetAttribute((const void*)theta_fft_kernel<MS, ROOT>,                   \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);      \
+    if (e != hipSuccess) return e;                                                                \
+    hipLaunchKernelGGL((theta_fft_kernel<MS, ROOT>), dim3((unsigned)nblk), dim3(256), sh, st,      \
+                       a.GT, a.F, a.ntc, a.npp, a.ntp, W, a.cb, a.ce, a.child_key, a.Text,         \
+                       a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root);                           \
+    return hipGetLastError();                                                                     \
+  } while (0)
+  if (slots <= 4) WF_THETA_FFT(4);
+  if (slots <= 8) WF_THETA_FFT(8);
+  WF_THETA_FFT(16);
+#undef WF_THETA_FFT
+}
+
+}  // namespace
+
+namespace wf {
+
+hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc, int npc, int npp,
+                          const Fft2Plan& pc, const Fft2Plan& pp, const float2* fac,
+                          hipStream_t st) {
+  if (lines_cf <= 0) return hipSuccess;
+  const int lmax = npc > npp ? npc : npp;
+  if (lmax > kLineBuf || pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
+  int RB = kLineBuf / lmax;
+  RB = RB < 1 ? 1 : (RB > ntc ? ntc : RB);
+  const size_t sh = sizeof(float2) * ((size_t)2 * RB * lmax + pc.twn + pp.twn + npc + 1);
+  if (sh > 160 * 1024) return hipErrorInvalidValue;
+  const int64_t nblk = lines_cf * ((ntc + RB - 1) / RB);
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void*)phi_fft_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(phi_fft_kernel, dim3((unsigned)nblk), dim3(256), sh, st, X, GT, ntc, npc, npp,
+                     RB, pc, pp, fac);
+  return hipGetLastError();
+}
+
+hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st) {
+  if (a.nparents <= 0) return hipSuccess;
+  const int n2c = 2 * a.ntc, n2p = 2 * a.ntp, half = a.npp / 2;
+  if (a.npp < 2 || (a.npp & 1) || a.pc.n != n2c || a.pp.n != n2p) return hipErrorInvalidValue;
+  if (a.mirror != n2p - 1 && a.mirror != n2p) return hipErrorInvalidValue;
+  if ((a.out_root != nullptr) == (a.out != nullptr)) return hipErrorInvalidValue;
+  const int lmax = n2c > n2p ? n2c : n2p;
+  if (lmax > kLineBuf || n2p > 256 * 16) return hipErrorInvalidValue;
+  int W = kLineBuf / lmax;
+  W = W < 1 ? 1 : (W > half ? half : W);
+  while (W > 1 && W * n2p > 256 * 16) --W;
+  const int slots = (W * n2p + 255) / 256;
+  const size_t sh = sizeof(float2) * ((size_t)2 * W * lmax + a.pc.twn + a.pp.twn + n2c + 1);
+  if (sh > 160 * 1024) return hipErrorInvalidValue;
+  if (a.out_root) return launch_theta_fft_t<true>(a, W, slots, sh, st);
+  return launch_theta_fft_t<false>(a, W, slots, sh, st);
+}
+
+}  // namespace wf

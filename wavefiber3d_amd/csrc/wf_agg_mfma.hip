@@ -62,6 +62,8 @@ __device__ inline floatx4 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8
 // phi pass.  Wave task = (MTW tiles of 16 complex rows) x (4 tiles of 16
 // output columns).  A = data rows (re and im parts as two M tiles sharing one
 // load), B = Mphi^T: lane (col q, k-group g) reads Mphi[q][k0 + 8g .. +7].
+// Loads are branchless (clamped index + select); the next K step's data is
+// loaded before the current step's MFMAs.
 template <int MTW>
 __global__ __launch_bounds__(256) void phi_mfma_kernel(
     const float2* __restrict__ X, float2* __restrict__ GT, int rows, int ntc, int npc, int npp,
@@ -81,28 +83,53 @@ __global__ __launch_bounds__(256) void phi_mfma_kernel(
       accr[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
       acci[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
+  const float2* xr[MTW];
+  bool rok[MTW];
+#pragma unroll
+  for (int m = 0; m < MTW; ++m) {
+    const int r = row0 + 16 * m + c16;
+    rok[m] = r < rows;
+    xr[m] = X + (int64_t)(rok[m] ? r : 0) * npc;
+  }
+  const bf16x8* mh[4];
+  const bf16x8* ml[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int q = ng * 64 + 16 * n + c16;  // < padded rows of Mphi
+    mh[n] = Mhi + ((q * kpad) >> 3);
+    ml[n] = Mlo + ((q * kpad) >> 3);
+  }
+  float2 nx[MTW][8];
+  auto load = [&](int k0) {
+    const int kb = k0 + 8 * g;
+#pragma unroll
+    for (int m = 0; m < MTW; ++m)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool ok = rok[m] && (kb + j < npc);
+        const float2 v = xr[m][ok ? kb + j : 0];
+        nx[m][j] = ok ? v : make_float2(0.f, 0.f);
+      }
+  };
+  load(0);
   for (int k0 = 0; k0 < npc; k0 += 32) {
     const int kb = k0 + 8 * g;
     bf16x8 arh[MTW], arl[MTW], aih[MTW], ail[MTW];
 #pragma unroll
     for (int m = 0; m < MTW; ++m) {
-      const int r = row0 + 16 * m + c16;
       float vr[8], vi[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        float2 v = make_float2(0.f, 0.f);
-        if (r < rows && kb + j < npc) v = X[(int64_t)r * npc + kb + j];
-        vr[j] = v.x;
-        vi[j] = v.y;
+        vr[j] = nx[m][j].x;
+        vi[j] = nx[m][j].y;
       }
       split8(vr, arh[m], arl[m]);
       split8(vi, aih[m], ail[m]);
     }
+    if (k0 + 32 < npc) load(k0 + 32);  // in flight during the MFMAs below
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      const int q = ng * 64 + 16 * n + c16;  // < padded rows of Mphi
-      const int64_t mi = ((int64_t)q * kpad + kb) >> 3;
-      const bf16x8 bh = Mhi[mi], bl = Mlo[mi];
+      const bf16x8 bh = mh[n][kb >> 3], bl = ml[n][kb >> 3];
 #pragma unroll
       for (int m = 0; m < MTW; ++m) {
         accr[m][n] = mfma3(arh[m], arl[m], bh, bl, accr[m][n]);
@@ -114,16 +141,15 @@ __global__ __launch_bounds__(256) void phi_mfma_kernel(
 #pragma unroll
   for (int m = 0; m < MTW; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const int q = ng * 64 + 16 * n + c16;
-      if (q >= npp) continue;
+    for (int j = 0; j < 4; ++j) {
+      const int r = row0 + 16 * m + 4 * g + j;
+      if (r >= rows) continue;
+      const int cf = r / ntc, th = r - cf * ntc;
+      float2* o = GT + (int64_t)cf * npp * ntc + th;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int r = row0 + 16 * m + 4 * g + j;
-        if (r < rows) {
-          const int cf = r / ntc, th = r - cf * ntc;
-          GT[((int64_t)cf * npp + q) * ntc + th] = make_float2(accr[m][n][j], acci[m][n][j]);
-        }
+      for (int n = 0; n < 4; ++n) {
+        const int q = ng * 64 + 16 * n + c16;
+        if (q < npp) o[(int64_t)q * ntc] = make_float2(accr[m][n][j], acci[m][n][j]);
       }
     }
 }
@@ -435,6 +461,7 @@ hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, i
   if (rows <= 0) return hipSuccess;
   if (rows > 0x7fffffffLL - 64 || ntc < 1 || npc < 1 || npp < 1 || kpad < npc || (kpad & 31))
     return hipErrorInvalidValue;
+  if ((int64_t)interp_rows_pad(npp) * kpad >= (1LL << 31)) return hipErrorInvalidValue;
   constexpr int MTW = 2;
   const int nng = (npp + 63) / 64;
   const int64_t nmg = (rows + 16 * MTW - 1) / (16 * MTW);

@@ -64,6 +64,11 @@ struct FftEntry {
   float2* tw = nullptr;
 };
 
+struct Fft2Entry {
+  Fft2Plan plan;
+  float2* tw = nullptr;
+};
+
 // Split-BF16 interpolation matrix of one (input length, output length, shift).
 struct MatEntry {
   uint16_t* hi = nullptr;
@@ -97,6 +102,8 @@ struct wf_handle {
   std::map<std::string, DevBuf> bufs;
   std::map<int, FftEntry> ffts;
   std::map<std::pair<std::pair<int, int>, uint64_t>, MatEntry> mats;
+  std::map<int, Fft2Entry> ffts2;
+  std::map<std::pair<int, uint64_t>, float2*> facs;
   // geometry
   bool has_geom = false;
   int64_t n = 0;
@@ -260,6 +267,80 @@ wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out) {
   return WF_OK;
 }
 
+// FFT plan of the large-level kernels: radices 4, then 3, 5, 2; per-stage
+// twiddles exp(-2 pi i r k / (Ns R)) in FP64, stored float.
+wf_status get_fft2(wf_handle* h, int n, Fft2Plan* out) {
+  auto it = h->ffts2.find(n);
+  if (it != h->ffts2.end()) {
+    *out = it->second.plan;
+    return WF_OK;
+  }
+  Fft2Entry e;
+  e.plan.n = n;
+  e.plan.nstages = 0;
+  e.plan.radix_packed = 0;
+  int m = n;
+  std::vector<int> rad;
+  while (m % 4 == 0) {
+    rad.push_back(4);
+    m /= 4;
+  }
+  for (int r : {3, 5, 2})
+    while (m % r == 0) {
+      rad.push_back(r);
+      m /= r;
+    }
+  if (m != 1 || n < 1 || (int)rad.size() > WF_FFT2_MAX_STAGES)
+    return set_err(WF_ERR_UNSUPPORTED, "FFT length %d is not 5-smooth", n);
+  std::vector<float2> tw;
+  int Ns = 1;
+  for (size_t s = 0; s < rad.size(); ++s) {
+    const int R = rad[s];
+    e.plan.radix_packed |= (uint64_t)R << (4 * s);
+    for (int k = 0; k < Ns; ++k)
+      for (int r = 1; r < R; ++r) {
+        const double a = -2.0 * WF_PI * (double)r * k / ((double)Ns * R);
+        tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+      }
+    Ns *= R;
+  }
+  e.plan.nstages = (int)rad.size();
+  e.plan.twn = (int)tw.size();
+  WF_HIP(hipMalloc(&e.tw, sizeof(float2) * std::max<size_t>(tw.size(), 1)));
+  if (!tw.empty())
+    WF_HIP(hipMemcpy(e.tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+  e.plan.tw = e.tw;
+  h->ffts2[n] = e;
+  *out = e.plan;
+  return WF_OK;
+}
+
+// Resample factors w_f exp(i f shift) / nc for f = -(nc/2) .. band top
+// (nc + 1 entries; the Nyquist pair of an even nc gets 1/2 each).
+wf_status get_fac(wf_handle* h, int nc, double shift, const float2** out) {
+  uint64_t sb;
+  memcpy(&sb, &shift, sizeof sb);
+  const auto key = std::make_pair(nc, sb);
+  auto it = h->facs.find(key);
+  if (it != h->facs.end()) {
+    *out = it->second;
+    return WF_OK;
+  }
+  std::vector<float2> fac(nc + 1, make_float2(0.f, 0.f));
+  const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
+  for (int f = fmin; f <= fmax; ++f) {
+    double w = 1.0 / nc;
+    if (nc % 2 == 0 && (f == nc / 2 || f == -(nc / 2))) w *= 0.5;
+    fac[f - fmin] = make_float2((float)(w * std::cos(f * shift)), (float)(w * std::sin(f * shift)));
+  }
+  float2* d = nullptr;
+  WF_HIP(hipMalloc(&d, sizeof(float2) * fac.size()));
+  WF_HIP(hipMemcpy(d, fac.data(), sizeof(float2) * fac.size(), hipMemcpyHostToDevice));
+  h->facs[key] = d;
+  *out = d;
+  return WF_OK;
+}
+
 // Incidence parameters in FP64 (treePO.cu:501-509, :580-601).
 WfIncDev make_inc(const wf_incidence& in) {
   WfIncDev d;
@@ -370,6 +451,7 @@ void default_tune(wf_tune* t) {
   t->leaf_quad = 1;
   t->chunk_solves = 4;
   t->agg_kernel = 0;
+  t->agg_fft_rows = 600;
   t->calibrated = 0;
 }
 
@@ -538,6 +620,10 @@ void wf_destroy(wf_handle* h) {
     if (kv.second.hi) hipFree(kv.second.hi);
     if (kv.second.lo) hipFree(kv.second.lo);
   }
+  for (auto& kv : h->ffts2)
+    if (kv.second.tw) hipFree(kv.second.tw);
+  for (auto& kv : h->facs)
+    if (kv.second) hipFree(kv.second);
   if (h->inc_host) hipHostFree(h->inc_host);
   if (h->inc_evt) hipEventDestroy(h->inc_evt);
   for (auto ev : h->evpool) hipEventDestroy(ev);
@@ -733,6 +819,38 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   WF_TRY(dalloc_t(h, "interp_G", (size_t)count * ntc * npp, &G));
   const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
   const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
+  if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
+      2 * ntp > h->tune.agg_fft_rows && is_smooth5(npc) && is_smooth5(npp) && is_smooth5(2 * ntc) &&
+      is_smooth5(2 * ntp)) {
+    Fft2Plan f_pc, f_pp, f_tc, f_tp;
+    WF_TRY(get_fft2(h, npc, &f_pc));
+    WF_TRY(get_fft2(h, npp, &f_pp));
+    WF_TRY(get_fft2(h, 2 * ntc, &f_tc));
+    WF_TRY(get_fft2(h, 2 * ntp, &f_tp));
+    const float2 *fac_p = nullptr, *fac_t = nullptr;
+    WF_TRY(get_fac(h, npc, 0.0, &fac_p));
+    WF_TRY(get_fac(h, 2 * ntc, off_tp - off_tc, &fac_t));
+    WF_HIP(wf::launch_phi_fft((const float2*)child, G, count, ntc, npc, npp, f_pc, f_pp, fac_p, st));
+    wf::ThetaFftArgs ta;
+    ta.GT = G;
+    ta.F = 1;
+    ta.ntc = ntc;
+    ta.npp = npp;
+    ta.ntp = ntp;
+    ta.nparents = count;
+    ta.cb = nullptr;
+    ta.ce = nullptr;
+    ta.child_key = nullptr;
+    ta.Text = nullptr;
+    ta.mirror = mirror;
+    ta.pc = f_tc;
+    ta.pp = f_tp;
+    ta.fac = fac_t;
+    ta.out = (float2*)parent;
+    ta.out_root = nullptr;
+    WF_HIP(wf::launch_theta_fft(ta, st));
+    return WF_OK;
+  }
   if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp)) {
     MatEntry mp, mt;
     WF_TRY(get_mat(h, npc, npp, 0.0, &mp));
@@ -1048,6 +1166,45 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
     const int mirror = mirror_of(gp.nt, off_p);
     const float2* Tl = gc.T + gc.toff[l];
+    if (h->tune.agg_kernel == 0 && 2 * gp.nt > h->tune.agg_fft_rows && is_smooth5(gp.np) &&
+        is_smooth5(2 * gp.nt) && is_smooth5(gcg.np) && is_smooth5(2 * gcg.nt)) {
+      // large grids: LDS FFT (each sample read and written once)
+      Fft2Plan f_pc, f_pp, f_tc, f_tp;
+      WF_TRY(get_fft2(h, gcg.np, &f_pc));
+      WF_TRY(get_fft2(h, gp.np, &f_pp));
+      WF_TRY(get_fft2(h, 2 * gcg.nt, &f_tc));
+      WF_TRY(get_fft2(h, 2 * gp.nt, &f_tp));
+      const float2 *fac_p = nullptr, *fac_t = nullptr;
+      WF_TRY(get_fac(h, gcg.np, 0.0, &fac_p));
+      WF_TRY(get_fac(h, 2 * gcg.nt, off_p - off_c, &fac_t));
+      WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
+        return wf::launch_phi_fft(cur, G, (int64_t)nchild * F, gcg.nt, gcg.np, gp.np, f_pc, f_pp,
+                                  fac_p, st);
+      }));
+      float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
+      wf::ThetaFftArgs ta;
+      ta.GT = G;
+      ta.F = F;
+      ta.ntc = gcg.nt;
+      ta.npp = gp.np;
+      ta.ntp = gp.nt;
+      ta.nparents = npar;
+      ta.cb = h->levels[l].cb;
+      ta.ce = h->levels[l].ce;
+      ta.child_key = h->levels[l + 1].key;
+      ta.Text = gc.Tx + gc.txoff[l];
+      ta.mirror = mirror;
+      ta.pc = f_tc;
+      ta.pp = f_tp;
+      ta.fac = fac_t;
+      ta.out = (l == 0) ? nullptr : nxt;
+      ta.out_root = (l == 0) ? F0 : nullptr;
+      const double wbytes = (l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np;
+      WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * gp.np + wbytes,
+                   [&] { return wf::launch_theta_fft(ta, st); }));
+      if (l > 0) cur = nxt;
+      continue;
+    }
     if (h->tune.agg_kernel == 0) {
       // dense split-BF16 MFMA interpolation: phi GEMM -> GT, theta GEMM +
       // translation + child sum -> parent (FP64 at the root)
@@ -1511,6 +1668,8 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: coefficients must be finite and >= 0");
   if (t->chunk_solves < 0 || t->chunk_solves > 8)
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: chunk_solves must be in [0, 8]");
+  if (t->agg_kernel < 0 || t->agg_kernel > 1 || t->agg_fft_rows < 0)
+    return set_err(WF_ERR_INVALID_ARG, "wf_tune: agg_kernel must be 0 or 1, agg_fft_rows >= 0");
   h->tune = *t;
   h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
   return WF_OK;
@@ -1615,7 +1774,10 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
     const Grid gd = level_grid(h, r.d, lam_s);
     sl += r.leaf;
     pl += (double)h->n * gd.nt * gd.np * S;
-    wf_tune unit = {0.0, 1.0, 0.0, 0, 0, 0};
+    wf_tune unit = h->tune;
+    unit.leaf_s_per_pair = 0.0;
+    unit.agg_s_per_byte = 1.0;
+    unit.level_s = 0.0;
     sa += r.agg;
     ba += model_cost(h, unit, r.d, lam_s, NT, NP) * S;
     rest += std::max(0.0, r.total - r.leaf - r.agg - r.other) / ((double)r.d * S);

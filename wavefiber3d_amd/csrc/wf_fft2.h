@@ -1,0 +1,141 @@
+// wf_fft2.h -- batched mixed-radix Stockham FFT on LDS lines for the large
+// level transitions (radices 4, 2, 3, 5; lengths up to 4096).
+//
+// Per stage of radix R with Ns = product of the radices already applied,
+// butterfly b of a line (b < n/R, k = b mod Ns, h = b div Ns) reads
+// v[r] = in[b + r n/R], multiplies v[r] by w_{Ns R}^{r k}, does an R-point DFT
+// and writes out[h Ns R + k + r Ns].  Every stage's twiddles
+// w_{Ns R}^{r k} (r = 1..R-1, k < Ns) sit in one table (built on the host in
+// FP64), copied to LDS once per block; quotients by per-stage constants use
+// exact multiply-high reciprocals (numerators < 2^24), so the inner loops have
+// no divisions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wf_common.h"
+
+#define WF_FFT2_MAX_STAGES 12
+
+struct Fft2Plan {
+  int n;
+  int nstages;
+  uint64_t radix_packed;  // 4 bits per stage
+  const float2* tw;       // device table, twn entries (forward sign)
+  int twn;
+};
+
+__host__ __device__ inline int fft2_radix(const Fft2Plan& p, int s) {
+  return (int)((p.radix_packed >> (4 * s)) & 15u);
+}
+
+// floor(x / d) for x < 2^24 and d >= 2 with magic = ceil(2^32 / d); d = 1
+// (whose magic does not fit 32 bits) is passed as magic 0 and returns x.
+__device__ inline uint32_t fdiv(uint32_t x, uint32_t magic) {
+  return magic ? __umulhi(x, magic) : x;
+}
+__host__ __device__ inline uint32_t fmagic(uint32_t d) {
+  return d <= 1 ? 0u : (uint32_t)((0x100000000ULL + d - 1) / d);
+}
+
+namespace wffft2 {
+
+// forward R-point DFT (exp(-2 pi i q r / R)); inverse uses conjugation around it
+template <int R>
+__device__ inline void dft(float2* v) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+    const float2 b1r = make_float2(b1.y, -b1.x);  // -i b1
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1r);
+    v[3] = csub(a1, b1r);
+  } else if constexpr (R == 3) {
+    // c = cos(2 pi/3) = -1/2, s = sin(2 pi/3) = sqrt(3)/2
+    const float s3 = 0.86602540378443864676f;
+    const float2 t = cadd(v[1], v[2]);
+    const float2 d = csub(v[1], v[2]);
+    const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    const float2 j = make_float2(s3 * d.y, -s3 * d.x);  // -i s d
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, j);
+    v[2] = csub(m, j);
+  } else {  // R == 5
+    // c1 = cos(2 pi/5), c2 = cos(4 pi/5), s1 = sin(2 pi/5), s2 = sin(4 pi/5)
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    const float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    const float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    const float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    const float2 m2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    // -i (s1 b1 + s2 b2) and -i (s2 b1 - s1 b2)
+    const float2 n1 = make_float2(s1 * b1.y + s2 * b2.y, -(s1 * b1.x + s2 * b2.x));
+    const float2 n2 = make_float2(s2 * b1.y - s1 * b2.y, -(s2 * b1.x - s1 * b2.x));
+    v[0] = cadd(v[0], cadd(a1, a2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+  }
+}
+
+template <int R>
+__device__ inline void stage(const float2* __restrict__ src, float2* __restrict__ dst, int n,
+                             int nlines, int lstride, int Ns, const float2* __restrict__ tw,
+                             int inverse) {
+  const int m = n / R;
+  const uint32_t mag_m = fmagic((uint32_t)m), mag_ns = fmagic((uint32_t)Ns);
+  const int total = nlines * m;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int line = (int)fdiv((uint32_t)t, mag_m);
+    const int b = t - line * m;
+    const int h = (int)fdiv((uint32_t)b, mag_ns);
+    const int k = b - h * Ns;
+    const float2* s = src + line * lstride;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const float2 x = s[b + r * m];
+      v[r] = inverse ? make_float2(x.x, -x.y) : x;  // inverse via conjugation
+    }
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * (R - 1) + r - 1]);
+    }
+    dft<R>(v);
+    float2* d = dst + line * lstride + h * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r * Ns] = inverse ? make_float2(v[r].x, -v[r].y) : v[r];
+  }
+}
+
+// Batched FFT of nlines lines (length p.n, stride lstride) held in a; b is
+// scratch of the same shape; tw is the plan's table in LDS.  Returns the
+// buffer holding the result.  Contains barriers: call from every thread.
+__device__ inline float2* lines(float2* a, float2* b, int nlines, int lstride, const Fft2Plan& p,
+                                const float2* tw, int inverse) {
+  int Ns = 1, off = 0;
+  for (int s = 0; s < p.nstages; ++s) {
+    const int R = fft2_radix(p, s);
+    switch (R) {
+      case 4: stage<4>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
+      case 2: stage<2>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
+      case 3: stage<3>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
+      default: stage<5>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
+    }
+    __syncthreads();
+    off += Ns * (R - 1);
+    float2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+}  // namespace wffft2

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "wf_fft.h"
+#include "wf_fft2.h"
 
 namespace wf {
 
@@ -64,6 +65,27 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
 
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
                             const double h[3], double k0, hipStream_t st);
+
+// FFT level transition for large grids (wf_agg_fft.hip).  fac tables hold
+// w_f exp(i f shift) / nc for f = -(nc/2) .. band top (nc + 1 entries).
+hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc, int npc, int npp,
+                          const Fft2Plan& pc, const Fft2Plan& pp, const float2* fac,
+                          hipStream_t st);
+struct ThetaFftArgs {
+  const float2* GT;  // [child][F][npp][ntc]
+  int F, ntc, npp, ntp, nparents;
+  const int32_t* cb;
+  const int32_t* ce;
+  const uint32_t* child_key;
+  const float2* Text;  // [8][2 ntp + 1][npp/2] (nullable)
+  int mirror;
+  Fft2Plan pc;         // length 2 ntc
+  Fft2Plan pp;         // length 2 ntp
+  const float2* fac;   // resample factors, 2 ntc + 1
+  float2* out;
+  double2* out_root;
+};
+hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st);
 
 // Leaf radiation sum: pattern[leaf][F][nt][np] of
 // sum_{n in leaf} C[n][f] exp(+i k0 s.(x_n - c_leaf)) on the leaf grid
