@@ -235,8 +235,10 @@ wf_status wf_get_stats(wf_handle* h, wf_stats* out);
  * the current geometry at the model's best candidate depths, picks the faster
  * of (quad-symmetric BF16x3 leaf kernel, 4 solves per pass) and (plain
  * kernel, 8 solves per pass), and stores the result in the handle;
- * wf_get_tune/wf_set_tune move a calibration between handles.  On a geometry
- * of fixed depth the solves keep that depth (best_depth is advisory). */
+ * wf_get_tune/wf_set_tune move a calibration between handles.  Later solves
+ * of the same WF_DEPTH_AUTO geometry at the tuned sampling wavelength use the
+ * measured fastest depth; other wavelengths use the fitted model.  On a
+ * geometry of fixed depth the solves keep that depth (best_depth is advisory). */
 typedef struct wf_tune {
   double leaf_s_per_pair; /* leaf sum, per (point, leaf-grid direction, solve) */
   double agg_s_per_byte;  /* aggregation, per compulsory HBM byte (complex64 patterns) */

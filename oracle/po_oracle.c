@@ -114,7 +114,9 @@ static void current_one(const double p[3], const double nrm[3], double area,
     cplx Hi0 = Ei0 / Z0;                                            /* :36 */
     for (int c = 0; c < 3; ++c) { Ei[c] = Ed[c] * Ei0; Hi[c] = Hd[c] * Hi0; } /* :37-38 */
     const double tol = 1e-10;
-    if (fabs(fabs(dotvalue) - 1.0) < tol) { /* :40-49 normal incidence branch */
+    /* :40-49 normal incidence branch; |n.w| > 1 (rounding, |n| slightly above 1)
+       is taken as normal incidence too instead of acos/division NaNs */
+    if (fabs(dotvalue) > 1.0 - tol) {
       const cplx ein = 1.0;
       cplx R = (mode == 'M') ? (ein - eta) / (ein + eta) : (eta - ein) / (ein + eta);
       for (int c = 0; c < 3; ++c) { Es[c] = Ei[c] * R; Hs[c] = Hi[c] * R; }
@@ -134,7 +136,8 @@ static void current_one(const double p[3], const double nrm[3], double area,
         Hpar[c] = t[c] * (1.0 / Z0);                                /* :55 */
         Hperp[c] = Hi[c] - Hpar[c];                                 /* :56 */
       }
-      double lphi = acos(fabs(dotvalue));                           /* :58 */
+      /* :58 (cosine clamped to 1: |n.w| > 1 by rounding would give NaN) */
+      double lphi = acos(fmin(fabs(dotvalue), 1.0));
       cplx Rs, Rp;
       fresnel(lphi, eta, &Rs, &Rp);                                 /* :59-60 */
       for (int c = 0; c < 3; ++c) {

@@ -339,3 +339,20 @@ def test_ellipsoid_fixture_properties():
     assert abs(p[0].max() - 3e-6) < 1e-8 and abs(p[2].max() - 2e-6) < 1e-8
     # under +z incidence exactly two samples take the normal-incidence branch
     assert int(np.sum(np.abs(np.abs(q[2].astype(np.float64)) - 1) < 1e-10)) == 2
+
+
+def test_normal_slightly_longer_than_one_stays_finite():
+    """A normal of length 1 + 1e-7 along the incidence direction (|n.w| > 1 by
+    rounding) takes the normal-incidence branch: finite currents equal to
+    those of the unit normal scaled by the tiny length change."""
+    import oracle as orc
+
+    pts = [np.array([0.0]), np.array([0.0]), np.array([0.0]),
+           np.array([0.0]), np.array([0.0]), np.array([-1.0 - 1e-7]), np.array([1e-12])]
+    unit = [a.copy() for a in pts]
+    unit[5] = np.array([-1.0])
+    inc = (np.pi / 2, 0.0, 0.5e-6, "M", 1.55, 0.0)
+    c = orc.currents(pts, inc)
+    cu = orc.currents(unit, inc)
+    assert np.all(np.isfinite(c))
+    assert np.allclose(c, cu, rtol=1e-6, atol=1e-6 * np.abs(cu).max())

@@ -112,6 +112,11 @@ struct wf_handle {
   bool auto_depth = false; // WF_DEPTH_AUTO: per-group depth from the cost model
   int force_depth = -1;    // autotuning: depth forced for every group
   wf_tune tune;
+  // depth measured fastest by wf_autotune for this geometry (geom_gen) at the
+  // sampling wavelength tuned_lam; other wavelengths use the cost model
+  uint64_t tuned_gen = 0;
+  double tuned_lam = 0;
+  int tuned_depth = -1;
   float boxf[6] = {0, 0, 0, 0, 0, 0};
   double box[6] = {0, 0, 0, 0, 0, 0};
   float* pts = nullptr;     // sorted SoA: x y z nx ny nz area (7 n)
@@ -1004,6 +1009,8 @@ int choose_depth(const wf_handle* h, double lam, int NT, int NP) {
   if (h->force_depth >= 0) return h->force_depth;
   if (!h->auto_depth) return h->depth_max;
   const double lam_s = sample_lambda(h, lam);
+  if (h->tuned_depth >= 0 && h->tuned_gen == h->geom_gen && h->tuned_lam == lam_s)
+    return std::min(h->tuned_depth, h->depth_max);
   int best = h->depth_max;
   double cbest = 0;
   for (int d = 1; d <= h->depth_max; ++d) {
@@ -1788,6 +1795,9 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
   if (ba > 0) h->tune.agg_s_per_byte = sa / ba;
   if (nrest) h->tune.level_s = rest / nrest;
   h->tune.calibrated = 1;
+  h->tuned_gen = h->geom_gen;
+  h->tuned_lam = lam_s;
+  h->tuned_depth = best->d;
   if (best_depth) *best_depth = best->d;
   return WF_OK;
 }

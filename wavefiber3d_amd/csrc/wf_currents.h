@@ -63,7 +63,10 @@ __device__ inline void current_point(const double p[3], const double nr[3], doub
       Hi.v[c] = zscale(Hi0, inc.H[c]);  // :38
     }
     const double2 eta = make_double2(inc.eta_re, inc.eta_im);
-    if (fabs(fabs(dotvalue) - 1.0) < 1e-10) {  // :40-49
+    // :40-49 (|n.w| within 1e-10 of 1); |n.w| above 1, possible by rounding for
+    // normals slightly longer than 1, is taken as normal incidence too (the
+    // oblique branch would divide by |w x n| = 0 or take acos of a value > 1)
+    if (fabs(dotvalue) > 1.0 - 1e-10) {
       const double2 one = make_double2(1.0, 0.0);
       const double2 R = (inc.mode == 'M') ? zdiv(zsub(one, eta), zadd(one, eta))
                                           : zdiv(zsub(eta, one), zadd(one, eta));
@@ -91,7 +94,9 @@ __device__ inline void current_point(const double p[3], const double nr[3], doub
         Hpar.v[c] = zscale(t.v[c], 1.0 / inc.z0);  // :55
         Hperp.v[c] = zsub(Hi.v[c], Hpar.v[c]);     // :56
       }
-      const double lphi = acos(fabs(dotvalue));   // :58
+      // :58; |n.w| can exceed 1 by rounding for normals of |n| slightly above 1,
+      // where acos would return NaN: clamp to normal incidence
+      const double lphi = acos(fmin(fabs(dotvalue), 1.0));
       const double st = sin(lphi), ct0 = cos(lphi);
       // rs/rp with n0 = 1 (util.h:370-382): p0 = cos, pt = eta*ct, q0 = cos, qt = ct/eta
       const double2 one = make_double2(1.0, 0.0);
