@@ -23,13 +23,15 @@
 //
 // Both run on v_mfma_f32_16x16x32_bf16 with split-BF16 x3 operands (matrix
 // split on the host, data split in registers), FP32 accumulation; the root
-// level accumulates the child sum in FP64.  The matrix fragments come from
-// L1/L2 (one small matrix per level).  The phi pass reads its data fragments
-// straight from HBM in the MFMA lane layout and stores theta-contiguous
-// (GT[cf][q][theta]); the theta pass stages each 32-sample step of the
-// great-circle sequences once per block in LDS, pre-split, for all waves.
+// level accumulates the child sum in FP64.  The phi pass reads its data
+// fragments straight from HBM in the MFMA lane layout (matrix fragments from
+// L1/L2) and stores each column pair as one contiguous great-circle row
+// (S above, zero padded to a multiple of 8).  The theta pass keeps its
+// matrix fragments in registers for a whole persistent block and stages each
+// child's column tile once per block in LDS, pre-split, for all waves.
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -67,7 +69,7 @@ __device__ inline floatx4 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8
 template <int MTW>
 __global__ __launch_bounds__(256) void phi_mfma_kernel(
     const float2* __restrict__ X, float2* __restrict__ GT, int rows, int ntc, int npc, int npp,
-    const bf16x8* __restrict__ Mhi, const bf16x8* __restrict__ Mlo, int kpad, int nng,
+    int n2c8, const bf16x8* __restrict__ Mhi, const bf16x8* __restrict__ Mlo, int kpad, int nng,
     int ntasks) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int task = blockIdx.x * 4 + wave;
@@ -137,7 +139,10 @@ __global__ __launch_bounds__(256) void phi_mfma_kernel(
       }
     }
   }
-  // D lane: column q, rows 4g + j of each M tile
+  // D lane: column q, rows 4g + j of each M tile; stored in great-circle
+  // order (column q < npp/2 ascending in theta, column q + npp/2 descending
+  // after it), the row padded with zeros to n2c8
+  const int half = npp / 2, n2c = 2 * ntc;
 #pragma unroll
   for (int m = 0; m < MTW; ++m)
 #pragma unroll
@@ -145,11 +150,17 @@ __global__ __launch_bounds__(256) void phi_mfma_kernel(
       const int r = row0 + 16 * m + 4 * g + j;
       if (r >= rows) continue;
       const int cf = r / ntc, th = r - cf * ntc;
-      float2* o = GT + (int64_t)cf * npp * ntc + th;
+      float2* o = GT + (int64_t)cf * half * n2c8;
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
         const int q = ng * 64 + 16 * n + c16;
-        if (q < npp) o[(int64_t)q * ntc] = make_float2(accr[m][n][j], acci[m][n][j]);
+        if (q < npp) {
+          const bool first = q < half;
+          float2* orow = o + (first ? q : q - half) * n2c8;
+          orow[first ? th : n2c - 1 - th] = make_float2(accr[m][n][j], acci[m][n][j]);
+          if (!first && th == 0)
+            for (int z = n2c; z < n2c8; ++z) orow[z] = make_float2(0.f, 0.f);
+        }
       }
     }
 }
@@ -165,46 +176,171 @@ struct AccT<true> {
   typedef double2 type;
 };
 
-// theta pass + translation + child sum, block-tiled.  Block = 4 waves in
-// WE x WC, wave tile ET x CT 16-tiles: BE = 16 WE ET great-circle rows e by
-// BC = 16 WC CT column pairs k of one (parent p, field f); the child loop runs
-// inside.  Per 32-K step the B operand (the child's great-circle sequences of
-// the block's columns) is loaded once, coalesced, split into BF16 hi/lo and
-// staged in LDS for all waves; the A operand (matrix rows) comes from L1/L2.
-// Loads are branchless (clamped index + select).  The parent accumulates in
-// registers in the D layout (lane: column k, rows 4g + j); row e of the
-// extended grid feeds parent (e, k) when e < ntp and (mirror - e, k + npp/2)
-// otherwise when that row is < ntp; with a sample at the pole (mirror =
-// 2 ntp) row 0 also feeds (0, k + npp/2), kept in a second accumulator.
-template <int ET, int CT, int WE, int WC, bool ROOT>
-__global__ __launch_bounds__(256) void theta_mfma_kernel(
-    const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, const int32_t* __restrict__ cb,
-    const int32_t* __restrict__ ce, const uint32_t* __restrict__ child_key,
-    const float2* __restrict__ Text, const bf16x8* __restrict__ Mhi,
-    const bf16x8* __restrict__ Mlo, int kpad, int mrows, int mirror, int neb, int ncb,
-    int64_t total, float2* __restrict__ out, double2* __restrict__ out_root) {
+struct ThetaK {
+  const float2* GT;  // [child][F][npp/2][n2c8], great-circle rows
+  int F, ntc, npp, ntp, n2c8;
+  int nitems, ng, ngF, ngR;  // items (parent, field); stride ng = ngF F + ngR
+  int neb, ncb, kpad, mrows, mirror;
+  int64_t used;  // blocks with work
+  const int32_t* cb;
+  const int32_t* ce;
+  const uint32_t* child_key;
+  const float2* Text;
+  const bf16x8* Mhi;
+  const bf16x8* Mlo;
+  float2* out;
+  double2* out_root;
+};
+
+// theta pass + translation + child sum.  Block = 4 waves stacked along the
+// great-circle rows e (BE = 64 ET rows) x BC = 16 CT column pairs k.  Each
+// wave keeps its matrix fragments (ET row tiles x all NKS K steps, hi and
+// lo) in registers for the block's lifetime, so the matrix is read once per
+// block.  The block walks the (parent, field) items grp, grp + ng, ...; for
+// every child of an item it stages the child's column tile (all K steps),
+// split into BF16 hi/lo, in LDS ([buffer][k step][plane][k group][column]:
+// MFMA lanes read consecutive 16 B), double buffered with one barrier per
+// child, while the data of the next D children are in flight in registers.
+// The great-circle rows are contiguous in GT, so each load item is 4 x 16 B
+// of one row.  The parent accumulates in registers in the D layout (lane:
+// column k, rows 4g + j); row e of the extended grid feeds parent (e, k)
+// when e < ntp and (mirror - e, k + npp/2) otherwise when that row is < ntp;
+// with a sample at the pole (mirror = 2 ntp) row 0 also feeds (0, k +
+// npp/2), kept in a second accumulator.  Lanes whose (e, k) lies outside the
+// grid accumulate harmless values that are never stored.
+template <int ET, int CT, int NKS, int D, bool ROOT, int MINB>
+__global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
   typedef typename AccT<ROOT>::type acc_t;
-  constexpr int BE = WE * ET * 16, BC = WC * CT * 16;
-  __shared__ bf16x8 Bs[BC][4][4];  // [column][k group][re hi, re lo, im hi, im lo]
+  constexpr int BE = 4 * ET * 16, BC = CT * 16;
+  constexpr int NITEM = NKS * 4 * BC;  // 8-sample load items per child
+  constexpr int LI = (NITEM + 255) / 256;
+  __shared__ bf16x8 Bs[2][NKS][4][4][BC];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c16 = lane & 15, g = lane >> 4;
-  const int we = wave / WC, wc = wave % WC;
   // XCD-aware mapping: blocks b = x (mod 8) run on XCD x and take a
-  // contiguous range of work items, so the e/k blocks of one (p, f) -- which
-  // read the same child data -- share an L2
+  // contiguous range of work items
   const int64_t nb = gridDim.x, per = (nb + 7) / 8;
   const int64_t w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (w >= total) return;  // whole block
-  const int cbk = (int)(w % ncb);
-  int64_t t = w / ncb;
-  const int eb = (int)(t % neb);
-  t /= neb;
-  const int f = (int)(t % F), p = (int)(t / F);
-  const int half = npp / 2, n2c = 2 * ntc, n2p = 2 * ntp;
-  const int e0 = eb * BE + we * ET * 16;
-  const int k0 = cbk * BC + wc * CT * 16;
-  const int i2_0 = mirror % n2p;
-  const bool dual = (i2_0 < ntp);
+  if (w >= P.used) return;  // whole block
+  const int cbk = (int)(w % P.ncb);
+  const int64_t t1 = w / P.ncb;
+  const int eb = (int)(t1 % P.neb);
+  const int grp = (int)(t1 / P.neb);
+  if (grp >= P.nitems) return;  // whole block
+  const int F = P.F, ntc = P.ntc, ntp = P.ntp, npp = P.npp, n2c8 = P.n2c8;
+  const int half = npp / 2, n2p = 2 * ntp;
+  const int nks = (2 * ntc + 31) >> 5;
+  const int e0 = eb * BE + wave * ET * 16;
+  const int k0 = cbk * BC;
+  const int i2_0 = P.mirror % n2p;
+  const bool dual = (i2_0 < ntp) && e0 == 0 && g == 0;  // this lane holds row 0
+  // resident matrix fragments
+  bf16x8 ah[ET][NKS], al[ET][NKS];
+#pragma unroll
+  for (int a = 0; a < ET; ++a) {
+    const int row = e0 + 16 * a + c16;
+    const int rr = row < P.mrows ? row : 0;
+    const bf16x8* ph = P.Mhi + ((rr * P.kpad) >> 3);
+    const bf16x8* pl = P.Mlo + ((rr * P.kpad) >> 3);
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int kq = ks < nks ? ks * 4 + g : 0;
+      ah[a][ks] = ph[kq];
+      al[a][ks] = pl[kq];
+    }
+  }
+  // load items: i -> k group i & 3, column (i >> 2) % BC, K step (i >> 2) / BC
+  int l_off[LI];
+  bool l_ok[LI];
+#pragma unroll
+  for (int q = 0; q < LI; ++q) {
+    const int i = threadIdx.x + 256 * q;
+    const int lg = i & 3, lcol = (i >> 2) % BC, lks = (i >> 2) / BC;
+    const int lk = k0 + lcol;
+    const int jb = (lks * 4 + lg) * 8;
+    l_ok[q] = i < NITEM && lk < half && jb < n2c8;
+    l_off[q] = l_ok[q] ? lk * n2c8 + jb : 0;
+  }
+  // translation-table offsets (clamped for lanes that are never stored)
+  int tofs[ET][CT][4], tofs2[CT];
+#pragma unroll
+  for (int b = 0; b < CT; ++b) {
+    const int col = k0 + 16 * b + c16;
+    const bool cok = col < half;
+#pragma unroll
+    for (int a = 0; a < ET; ++a)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = e0 + 16 * a + 4 * g + j;
+        tofs[a][b][j] = (cok && e < n2p) ? e * half + col : 0;
+      }
+    tofs2[b] = cok ? n2p * half + col : 0;
+  }
+  const int64_t slab = (int64_t)half * n2c8;
+  float2 R[D][LI][8];
+  int okey[D];
+  auto load = [&](int d, int c, int f) {
+    const float2* S = P.GT + ((int64_t)c * F + f) * slab;
+#pragma unroll
+    for (int q = 0; q < LI; ++q) {
+      const float4* src = reinterpret_cast<const float4*>(S + l_off[q]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float4 v = src[j];
+        if (!l_ok[q]) v = make_float4(0.f, 0.f, 0.f, 0.f);
+        R[d][q][2 * j] = make_float2(v.x, v.y);
+        R[d][q][2 * j + 1] = make_float2(v.z, v.w);
+      }
+    }
+    okey[d] = P.child_key ? (int)(P.child_key[c] & 7u) : 0;
+  };
+  auto stage = [&](int d, int buf) {
+#pragma unroll
+    for (int q = 0; q < LI; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      if (i < NITEM) {
+        const int lg = i & 3, lcol = (i >> 2) % BC, lks = (i >> 2) / BC;
+        float vr[8], vi[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          vr[j] = R[d][q][j].x;
+          vi[j] = R[d][q][j].y;
+        }
+        bf16x8 a0, a1, a2, a3;
+        split8(vr, a0, a1);
+        split8(vi, a2, a3);
+        Bs[buf][lks][0][lg][lcol] = a0;
+        Bs[buf][lks][1][lg][lcol] = a1;
+        Bs[buf][lks][2][lg][lcol] = a2;
+        Bs[buf][lks][3][lg][lcol] = a3;
+      }
+    }
+  };
+  // (item, child) cursors: compute, and loads D children ahead
+  auto advance = [&](int& xit, int& xp, int& xf, int& xc, int& xcend) {
+    if (++xc >= xcend) {
+      xit += P.ng;
+      xf += P.ngR;
+      xp += P.ngF;
+      if (xf >= F) {
+        xf -= F;
+        ++xp;
+      }
+      if (xit < P.nitems) {
+        xc = P.cb ? P.cb[xp] : xp;
+        xcend = P.cb ? P.ce[xp] : xp + 1;
+      }
+    }
+  };
+  int it = grp;
+  int p = it / F, f = it - p * F;
+  int c = P.cb ? P.cb[p] : p, cend = P.cb ? P.ce[p] : p + 1;
+  int lit = it, lp = p, lf = f, lc = c, lcend = cend;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (lit < P.nitems) load(d, lc, lf);
+    advance(lit, lp, lf, lc, lcend);
+  }
   acc_t acc[ET][CT][4];
   acc_t acc2[CT];
 #pragma unroll
@@ -221,132 +357,142 @@ __global__ __launch_bounds__(256) void theta_mfma_kernel(
     acc2[b].x = 0;
     acc2[b].y = 0;
   }
-  // loader role (fixed per thread): column lc, k group lkg
-  const bool ldr = threadIdx.x < BC * 4;
-  const int lc = (threadIdx.x >> 2) % BC, lkg = threadIdx.x & 3;
-  const int lk = cbk * BC + lc;
-  const bool lkok = ldr && lk < half;
-  const int off1 = (lkok ? lk : 0) * ntc;                   // column k, theta ascending
-  const int off2 = (lkok ? lk + half : 0) * ntc + n2c - 1;  // column k + half, reversed
-  const bf16x8* ah_p[ET];
-  const bf16x8* al_p[ET];
+  int buf = 0;
+  bool live = true;
+  while (live) {
 #pragma unroll
-  for (int a = 0; a < ET; ++a) {
-    const int row = e0 + 16 * a + c16;
-    const int rr = row < mrows ? row : 0;
-    ah_p[a] = Mhi + ((rr * kpad) >> 3);
-    al_p[a] = Mlo + ((rr * kpad) >> 3);
-  }
-  const int64_t slab = (int64_t)npp * ntc;
-  const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
-  for (int c = c0; c < c1; ++c) {
-    const float2* S = GT + ((int64_t)c * F + f) * slab;
-    floatx4 ur[ET][CT], ui[ET][CT];
-#pragma unroll
-    for (int a = 0; a < ET; ++a)
-#pragma unroll
-      for (int b = 0; b < CT; ++b) {
-        ur[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-        ui[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-      }
-    for (int k0s = 0; k0s < n2c; k0s += 32) {
-      __syncthreads();  // previous step's fragments consumed
-      if (ldr) {
-        const int jb = k0s + 8 * lkg;
-        float vr[8], vi[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int jp = jb + j;
-          const bool ok = lkok && jp < n2c;
-          const float2 v = S[ok ? (jp < ntc ? off1 + jp : off2 - jp) : 0];
-          vr[j] = ok ? v.x : 0.f;
-          vi[j] = ok ? v.y : 0.f;
-        }
-        bf16x8 a0, a1, a2, a3;
-        split8(vr, a0, a1);
-        split8(vi, a2, a3);
-        Bs[lc][lkg][0] = a0;
-        Bs[lc][lkg][1] = a1;
-        Bs[lc][lkg][2] = a2;
-        Bs[lc][lkg][3] = a3;
-      }
+    for (int d = 0; d < D; ++d) {
+      if (!live) break;
+      stage(d, buf);
+      const int oct = okey[d];
       __syncthreads();
-      const int kq = (k0s >> 3) + g;
-      bf16x8 ah[ET], al[ET];
+      if (lit < P.nitems) load(d, lc, lf);
+      advance(lit, lp, lf, lc, lcend);
+      float2 tf[ET][CT][4], tf2[CT];
+      if (P.Text) {
+        const float2* To = P.Text + oct * (n2p + 1) * half;
 #pragma unroll
-      for (int a = 0; a < ET; ++a) {
-        ah[a] = ah_p[a][kq];
-        al[a] = al_p[a][kq];
-      }
+        for (int b = 0; b < CT; ++b) {
 #pragma unroll
-      for (int b = 0; b < CT; ++b) {
-        const int col = wc * CT * 16 + 16 * b + c16;
-        const bf16x8 brh = Bs[col][g][0], brl = Bs[col][g][1], bih = Bs[col][g][2],
-                     bil = Bs[col][g][3];
+          for (int a = 0; a < ET; ++a)
 #pragma unroll
-        for (int a = 0; a < ET; ++a) {
-          ur[a][b] = mfma3(ah[a], al[a], brh, brl, ur[a][b]);
-          ui[a][b] = mfma3(ah[a], al[a], bih, bil, ui[a][b]);
+            for (int j = 0; j < 4; ++j) tf[a][b][j] = To[tofs[a][b][j]];
+          tf2[b] = To[tofs2[b]];
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < CT; ++b) {
+#pragma unroll
+          for (int a = 0; a < ET; ++a)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) tf[a][b][j] = make_float2(1.f, 0.f);
+          tf2[b] = make_float2(1.f, 0.f);
         }
       }
-    }
-    const int oct = child_key ? (int)(child_key[c] & 7u) : 0;
-    const float2* To = Text ? Text + oct * (n2p + 1) * half : nullptr;
-#pragma unroll
-    for (int b = 0; b < CT; ++b) {
-      const int col = k0 + 16 * b + c16;
-      const bool cok = col < half;
+      floatx4 ur[ET][CT], ui[ET][CT];
 #pragma unroll
       for (int a = 0; a < ET; ++a)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int e = e0 + 16 * a + 4 * g + j;
-          const bool ok = cok && e < n2p;
-          const float2 u = make_float2(ur[a][b][j], ui[a][b][j]);
-          const float2 tu = To ? cmul(To[ok ? e * half + col : 0], u) : u;
-          acc[a][b][j].x += ok ? tu.x : 0.f;
-          acc[a][b][j].y += ok ? tu.y : 0.f;
-          if (dual && ok && e == 0) {
-            const float2 tu2 = To ? cmul(To[n2p * half + col], u) : u;
-            acc2[b].x += tu2.x;
-            acc2[b].y += tu2.y;
+        for (int b = 0; b < CT; ++b) {
+          ur[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+          ui[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        if (ks < nks) {
+#pragma unroll
+          for (int b = 0; b < CT; ++b) {
+            const int col = 16 * b + c16;
+            const bf16x8 brh = Bs[buf][ks][0][g][col], brl = Bs[buf][ks][1][g][col],
+                         bih = Bs[buf][ks][2][g][col], bil = Bs[buf][ks][3][g][col];
+#pragma unroll
+            for (int a = 0; a < ET; ++a) {
+              ur[a][b] = mfma3(ah[a][ks], al[a][ks], brh, brl, ur[a][b]);
+              ui[a][b] = mfma3(ah[a][ks], al[a][ks], bih, bil, ui[a][b]);
+            }
           }
         }
-    }
-  }
-  // scatter to the parent grid
-  const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
+      }
 #pragma unroll
-  for (int b = 0; b < CT; ++b) {
-    const int col = k0 + 16 * b + c16;
-    if (col >= half) continue;
+      for (int b = 0; b < CT; ++b)
 #pragma unroll
-    for (int a = 0; a < ET; ++a)
+        for (int a = 0; a < ET; ++a)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int e = e0 + 16 * a + 4 * g + j;
-        if (e >= n2p) continue;
-        int row = e, cc = col;
-        if (e >= ntp) {
-          int i2 = mirror - e;  // mirror is 2 ntp - 1 or 2 ntp: i2 in [0, ntp]
-          row = i2 < ntp ? i2 : -1;
-          cc = col + half;
+          for (int j = 0; j < 4; ++j) {
+            const float2 u = make_float2(ur[a][b][j], ui[a][b][j]);
+            const float2 tu = cmul(tf[a][b][j], u);
+            acc[a][b][j].x += tu.x;
+            acc[a][b][j].y += tu.y;
+            if (a == 0 && j == 0 && dual) {
+              const float2 tu2 = cmul(tf2[b], u);
+              acc2[b].x += tu2.x;
+              acc2[b].y += tu2.y;
+            }
+          }
+      if (c + 1 >= cend) {
+        // item done: scatter to the parent grid and reset
+        const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
+#pragma unroll
+        for (int b = 0; b < CT; ++b) {
+          const int col = k0 + 16 * b + c16;
+          if (col < half) {
+#pragma unroll
+            for (int a = 0; a < ET; ++a)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const int e = e0 + 16 * a + 4 * g + j;
+                if (e < n2p) {
+                  int row = e, cc = col;
+                  if (e >= ntp) {
+                    const int i2 = P.mirror - e;  // in [0, ntp]
+                    row = i2 < ntp ? i2 : -1;
+                    cc = col + half;
+                  }
+                  if (row >= 0) {
+                    const int64_t o = obase + (int64_t)row * npp + cc;
+                    if (ROOT)
+                      P.out_root[o] = make_double2((double)acc[a][b][j].x, (double)acc[a][b][j].y);
+                    else
+                      P.out[o] = make_float2((float)acc[a][b][j].x, (float)acc[a][b][j].y);
+                  }
+                  if (a == 0 && j == 0 && dual) {
+                    const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
+                    if (ROOT)
+                      P.out_root[o2] = make_double2((double)acc2[b].x, (double)acc2[b].y);
+                    else
+                      P.out[o2] = make_float2((float)acc2[b].x, (float)acc2[b].y);
+                  }
+                }
+              }
+          }
         }
-        if (row >= 0) {
-          const int64_t o = obase + (int64_t)row * npp + cc;
-          if (ROOT)
-            out_root[o] = make_double2((double)acc[a][b][j].x, (double)acc[a][b][j].y);
-          else
-            out[o] = make_float2((float)acc[a][b][j].x, (float)acc[a][b][j].y);
-        }
-        if (dual && e == 0) {
-          const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
-          if (ROOT)
-            out_root[o2] = make_double2((double)acc2[b].x, (double)acc2[b].y);
-          else
-            out[o2] = make_float2((float)acc2[b].x, (float)acc2[b].y);
+#pragma unroll
+        for (int a = 0; a < ET; ++a)
+#pragma unroll
+          for (int b = 0; b < CT; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              acc[a][b][j].x = 0;
+              acc[a][b][j].y = 0;
+            }
+#pragma unroll
+        for (int b = 0; b < CT; ++b) {
+          acc2[b].x = 0;
+          acc2[b].y = 0;
         }
       }
+      {
+        int nit = it, np_ = p, nf = f, nc = c, ncend = cend;
+        advance(nit, np_, nf, nc, ncend);
+        live = nit < P.nitems;
+        it = nit;
+        p = np_;
+        f = nf;
+        c = nc;
+        cend = ncend;
+      }
+      buf ^= 1;
+    }
   }
 }
 
@@ -393,20 +539,44 @@ __global__ __launch_bounds__(256) void trans_ext_kernel(float2* __restrict__ Tex
 
 int64_t blocks_for(int64_t tasks) { return (tasks + 3) / 4; }
 
-template <int ET, int CT, int WE, int WC, bool ROOT>
-hipError_t launch_theta_t(const wf::ThetaMfmaArgs& a, hipStream_t st) {
-  constexpr int BE = WE * ET * 16, BC = WC * CT * 16;
+// blocks_per_cu: resident blocks aimed at per CU (sets the item stride)
+template <int ET, int CT, int NKS, int D, bool ROOT, int MINB>
+hipError_t launch_theta_t(const wf::ThetaMfmaArgs& a, int blocks_per_cu, hipStream_t st) {
+  constexpr int BE = 4 * ET * 16, BC = CT * 16;
+  ThetaK P;
+  P.GT = a.GT;
+  P.F = a.F;
+  P.ntc = a.ntc;
+  P.npp = a.npp;
+  P.ntp = a.ntp;
+  P.n2c8 = wf::gc_row_len(a.ntc);
   const int half = a.npp / 2;
-  const int neb = (2 * a.ntp + BE - 1) / BE, ncb = (half + BC - 1) / BC;
-  const int64_t total = (int64_t)a.nparents * a.F * neb * ncb;
-  if (total <= 0) return hipSuccess;
-  const int64_t nb = ((total + 7) / 8) * 8;
+  P.neb = (2 * a.ntp + BE - 1) / BE;
+  P.ncb = (half + BC - 1) / BC;
+  const int64_t nitems = (int64_t)a.nparents * a.F;
+  if (nitems > 0x7fffffffLL) return hipErrorInvalidValue;
+  P.nitems = (int)nitems;
+  int64_t ng = std::max<int64_t>(1, 256LL * blocks_per_cu / ((int64_t)P.neb * P.ncb));
+  ng = std::min<int64_t>(ng, nitems);
+  P.ng = (int)ng;
+  P.ngF = P.ng / a.F;
+  P.ngR = P.ng % a.F;
+  P.used = (int64_t)P.neb * P.ncb * ng;
+  const int64_t nb = ((P.used + 7) / 8) * 8;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-  const int mrows = wf::interp_rows_pad(2 * a.ntp);
-  hipLaunchKernelGGL((theta_mfma_kernel<ET, CT, WE, WC, ROOT>), dim3((unsigned)nb), dim3(256), 0, st,
-                     a.GT, a.F, a.ntc, a.npp, a.ntp, a.cb, a.ce, a.child_key, a.Text,
-                     reinterpret_cast<const bf16x8*>(a.Mhi), reinterpret_cast<const bf16x8*>(a.Mlo),
-                     a.kpad, mrows, a.mirror, neb, ncb, total, a.out, a.out_root);
+  P.kpad = a.kpad;
+  P.mrows = wf::interp_rows_pad(2 * a.ntp);
+  P.mirror = a.mirror;
+  P.cb = a.cb;
+  P.ce = a.ce;
+  P.child_key = a.child_key;
+  P.Text = a.Text;
+  P.Mhi = reinterpret_cast<const bf16x8*>(a.Mhi);
+  P.Mlo = reinterpret_cast<const bf16x8*>(a.Mlo);
+  P.out = a.out;
+  P.out_root = a.out_root;
+  hipLaunchKernelGGL((theta_mfma_kernel<ET, CT, NKS, D, ROOT, MINB>), dim3((unsigned)nb), dim3(256), 0,
+                     st, P);
   return hipGetLastError();
 }
 
@@ -459,8 +629,10 @@ hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, i
                            const uint16_t* Mhi, const uint16_t* Mlo, int kpad,
                            hipStream_t st) {
   if (rows <= 0) return hipSuccess;
-  if (rows > 0x7fffffffLL - 64 || ntc < 1 || npc < 1 || npp < 1 || kpad < npc || (kpad & 31))
+  if (rows > 0x7fffffffLL - 64 || ntc < 1 || npc < 1 || npp < 2 || (npp & 1) || kpad < npc ||
+      (kpad & 31))
     return hipErrorInvalidValue;
+  if ((int64_t)(npp / 2) * gc_row_len(ntc) >= (1LL << 31)) return hipErrorInvalidValue;
   if ((int64_t)interp_rows_pad(npp) * kpad >= (1LL << 31)) return hipErrorInvalidValue;
   constexpr int MTW = 2;
   const int nng = (npp + 63) / 64;
@@ -468,28 +640,36 @@ hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, i
   const int64_t ntasks = nmg * nng;
   if (ntasks > 0x7fffffffLL - 4) return hipErrorInvalidValue;
   hipLaunchKernelGGL((phi_mfma_kernel<MTW>), dim3((unsigned)blocks_for(ntasks)), dim3(256), 0, st,
-                     X, GT, (int)rows, ntc, npc, npp, reinterpret_cast<const bf16x8*>(Mhi),
+                     X, GT, (int)rows, ntc, npc, npp, gc_row_len(ntc), reinterpret_cast<const bf16x8*>(Mhi),
                      reinterpret_cast<const bf16x8*>(Mlo), kpad, nng, (int)ntasks);
   return hipGetLastError();
 }
 
 hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
   if (a.nparents <= 0) return hipSuccess;
-  if (a.ntc < 1 || a.ntp < 1 || a.npp < 2 || (a.npp & 1) || a.kpad < 2 * a.ntc || (a.kpad & 31))
+  if (a.ntc < 1 || a.ntp < 1 || a.npp < 2 || (a.npp & 1) || a.kpad < 2 * a.ntc || (a.kpad & 31) ||
+      a.F < 1 || 2 * a.ntc > kThetaMfmaMaxK)
     return hipErrorInvalidValue;
   if (a.mirror != 2 * a.ntp - 1 && a.mirror != 2 * a.ntp) return hipErrorInvalidValue;
   if ((a.out_root != nullptr) == (a.out != nullptr)) return hipErrorInvalidValue;
   if ((int64_t)interp_rows_pad(2 * a.ntp) * a.kpad >= (1LL << 31) ||
-      (int64_t)a.npp * a.ntc >= (1LL << 31))
+      (int64_t)(a.npp / 2) * gc_row_len(a.ntc) >= (1LL << 31) ||
+      (int64_t)(2 * a.ntp + 1) * 8 * (a.npp / 2) >= (1LL << 31))
     return hipErrorInvalidValue;
-  // block tile by extended-grid height (measured on the fiber level shapes):
-  // <= 96 rows: 64 x 16, <= 160: 64 x 32, <= 288: 96 x 32, else 64 x 64
-  const int n2p = 2 * a.ntp;
-  if (a.out_root != nullptr) return launch_theta_t<2, 2, 2, 2, true>(a, st);
-  if (n2p <= 96) return launch_theta_t<1, 1, 4, 1, false>(a, st);
-  if (n2p <= 160) return launch_theta_t<2, 1, 2, 2, false>(a, st);
-  if (n2p <= 288) return launch_theta_t<3, 1, 2, 2, false>(a, st);
-  return launch_theta_t<2, 2, 2, 2, false>(a, st);
+  // tile by K steps (32 great-circle samples each), measured on the sphere
+  // and fiber level shapes
+  const int nks = (2 * a.ntc + 31) / 32;
+  if (a.out_root != nullptr) {
+    if (nks <= 4) return launch_theta_t<2, 1, 4, 2, true, 1>(a, 2, st);
+    if (nks <= 9) return launch_theta_t<2, 2, 9, 1, true, 1>(a, 2, st);
+    return launch_theta_t<1, 1, 16, 1, true, 1>(a, 1, st);
+  }
+  if (nks <= 2) return launch_theta_t<2, 1, 2, 2, false, 2>(a, 2, st);
+  if (nks == 3) return launch_theta_t<2, 1, 3, 2, false, 2>(a, 2, st);
+  if (nks == 4) return launch_theta_t<2, 1, 4, 2, false, 2>(a, 2, st);
+  if (nks == 5) return launch_theta_t<1, 1, 5, 2, false, 2>(a, 4, st);
+  if (nks <= 9) return launch_theta_t<2, 2, 9, 1, false, 1>(a, 2, st);
+  return launch_theta_t<1, 1, 16, 1, false, 1>(a, 1, st);
 }
 
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,

@@ -821,7 +821,8 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   WF_TRY(get_fft(h, 2 * ntc, &tc));
   WF_TRY(get_fft(h, 2 * ntp, &tp));
   float2* G = nullptr;
-  WF_TRY(dalloc_t(h, "interp_G", (size_t)count * ntc * npp, &G));
+  WF_TRY(dalloc_t(h, "interp_G",
+                  std::max((size_t)count * ntc * npp, (size_t)wf::gc_elems(count, ntc, npp)), &G));
   const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
   const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
   if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
@@ -856,7 +857,8 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
     WF_HIP(wf::launch_theta_fft(ta, st));
     return WF_OK;
   }
-  if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp)) {
+  if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
+      2 * ntc <= wf::kThetaMfmaMaxK && npp % 2 == 0) {
     MatEntry mp, mt;
     WF_TRY(get_mat(h, npc, npp, 0.0, &mp));
     WF_TRY(get_mat(h, 2 * ntc, 2 * ntp, off_tp - off_tc, &mt));
@@ -1124,12 +1126,17 @@ wf_status tree_buffers(wf_handle* h, const GroupCtx& gc, int F, int NP, TreeBufs
   size_t pat_max = 16, g_max = 16;
   for (int l = 1; l <= depth; ++l)
     pat_max = std::max(pat_max, (size_t)h->levels[l].count * F * gc.g[l].nt * gc.g[l].np);
+  // G holds the phi-pass output of one level: [child][F][np_parent][nt_child]
+  // for the FFT kernels, padded great-circle rows for the MFMA kernels
+  auto g_elems = [&](int64_t nchild, int ntc, int npp) {
+    return std::max((size_t)nchild * F * ntc * npp, (size_t)wf::gc_elems(nchild * F, ntc, npp));
+  };
   for (int l = 1; l < depth; ++l)
-    g_max = std::max(g_max, (size_t)h->levels[l + 1].count * F * gc.g[l + 1].nt * gc.g[l].np);
+    g_max = std::max(g_max, g_elems(h->levels[l + 1].count, gc.g[l + 1].nt, gc.g[l].np));
   WF_TRY(dalloc_t(h, "patA", pat_max, &tb.patA));
   WF_TRY(dalloc_t(h, "patB", pat_max, &tb.patB));
   WF_TRY(dalloc_t(h, "G", g_max, &tb.G));
-  WF_TRY(dalloc_t(h, "G0", (size_t)h->levels[1].count * F * gc.g[1].nt * NP, &tb.G0));
+  WF_TRY(dalloc_t(h, "G0", g_elems(h->levels[1].count, gc.g[1].nt, NP), &tb.G0));
   return WF_OK;
 }
 
@@ -1212,7 +1219,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       if (l > 0) cur = nxt;
       continue;
     }
-    if (h->tune.agg_kernel == 0) {
+    if (h->tune.agg_kernel == 0 && 2 * gcg.nt <= wf::kThetaMfmaMaxK && gp.np % 2 == 0) {
       // dense split-BF16 MFMA interpolation: phi GEMM -> GT, theta GEMM +
       // translation + child sum -> parent (FP64 at the root)
       MatEntry mp, mt;

@@ -43,12 +43,24 @@ void build_interp_matrix(int nc, int np, double shift, std::vector<uint16_t>& hi
 int interp_rows_pad(int rows);
 int interp_kpad(int k);
 
-// phi pass: X [rows = nodes*F*ntc][npc] -> GT [nodes*F][npp][ntc] (theta contiguous).
+// phi pass: X [rows = nodes*F*ntc][npc] -> GT in great-circle order:
+// GT[nodes*F][npp/2][gc_row_len(ntc)], row k holds column k at theta j
+// (j < ntc), then column k + npp/2 at theta 2 ntc - 1 - j (ntc <= j < 2 ntc),
+// then zeros up to the padded length.
 hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
                            const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st);
+// padded great-circle row length: 2 ntc rounded up to a multiple of 8
+inline int gc_row_len(int ntc) { return ((2 * ntc + 7) / 8) * 8; }
+// elements of the phi-pass output for `planes` = nodes*F
+inline int64_t gc_elems(int64_t planes, int ntc, int npp) {
+  return planes * (int64_t)(npp / 2) * gc_row_len(ntc);
+}
+// largest 2 ntc the theta MFMA pass accepts (its matrix fragments stay in
+// registers for all K steps)
+constexpr int kThetaMfmaMaxK = 512;
 
 struct ThetaMfmaArgs {
-  const float2* GT;  // [child][F][npp][ntc]
+  const float2* GT;  // phi_mfma output: [child][F][npp/2][gc_row_len(ntc)]
   int F, ntc, npp, ntp, nparents;
   const int32_t* cb;  // per-parent child range (null: parent p <- child p)
   const int32_t* ce;
