@@ -165,6 +165,170 @@ __global__ __launch_bounds__(256) void phi_mfma_kernel(
     }
 }
 
+// phi pass for K <= 288 (NKS 32-sample steps), matrix-stationary.  Block =
+// 4 waves side by side in q (wave: QT 16-column tiles, block: 64 QT
+// columns); every wave keeps its matrix fragments (QT tiles x NKS K steps,
+// hi and lo) in registers.  The block walks 32-row tiles with stride ng;
+// each tile (32 rows x npc, contiguous in X) is loaded coalesced, split
+// into BF16 hi/lo and staged in LDS [buffer][k step][plane][k group][row]
+// (MFMA lanes read consecutive 16 B), double buffered with one barrier per
+// tile and the next D tiles in flight in registers.  Output as phi_mfma.
+template <int NKS, int QT, int D>
+__global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict__ X,
+                                                    float2* __restrict__ GT, int rows, int ntc,
+                                                    int npc, int npp, int n2c8,
+                                                    const bf16x8* __restrict__ Mhi,
+                                                    const bf16x8* __restrict__ Mlo, int kpad,
+                                                    int mrows, int nqb, int ng, int64_t used) {
+  constexpr int RT = 32;
+  constexpr int NITEM = RT * NKS * 4;  // (row, 8-sample group) items per tile
+  constexpr int LI = (NITEM + 255) / 256;
+  __shared__ bf16x8 As[2][NKS][4][4][RT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c16 = lane & 15, g = lane >> 4;
+  const int64_t nb = gridDim.x, per = (nb + 7) / 8;
+  const int64_t w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (w >= used) return;
+  const int qb = (int)(w % nqb);
+  const int grp = (int)(w / nqb);
+  const int nrt = (rows + RT - 1) / RT;
+  if (grp >= nrt) return;
+  const int nks = (npc + 31) >> 5;
+  const int half = npp / 2, n2c = 2 * ntc;
+  const int q0 = qb * 64 * QT + wave * 16 * QT;
+  // resident matrix fragments: B lane (col q = c16, k group g)
+  bf16x8 bh[QT][NKS], bl[QT][NKS];
+#pragma unroll
+  for (int n = 0; n < QT; ++n) {
+    const int q = q0 + 16 * n + c16;
+    const int qq = q < mrows ? q : 0;
+    const bf16x8* ph = Mhi + ((qq * kpad) >> 3);
+    const bf16x8* pl = Mlo + ((qq * kpad) >> 3);
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int kq = ks < nks ? ks * 4 + g : 0;
+      bh[n][ks] = ph[kq];
+      bl[n][ks] = pl[kq];
+    }
+  }
+  // load items: i -> row i % RT, (ks, g) = i / RT
+  int l_row[LI], l_jb[LI];
+#pragma unroll
+  for (int q = 0; q < LI; ++q) {
+    const int i = threadIdx.x + 256 * q;
+    l_row[q] = i % RT;
+    l_jb[q] = (i / RT) * 8;
+  }
+  float2 R[D][LI][8];
+  auto load = [&](int d, int tile) {
+#pragma unroll
+    for (int q = 0; q < LI; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int r = tile * RT + l_row[q];
+      const bool rok = i < NITEM && r < rows;
+      const float4* src = reinterpret_cast<const float4*>(X + (int64_t)(rok ? r : 0) * npc);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int jj = l_jb[q] + 2 * j;  // npc even: a pair never straddles the row end
+        const bool ok = rok && jj < npc;
+        float4 v = src[ok ? (jj >> 1) : 0];
+        if (!ok) v = make_float4(0.f, 0.f, 0.f, 0.f);
+        R[d][q][2 * j] = make_float2(v.x, v.y);
+        R[d][q][2 * j + 1] = make_float2(v.z, v.w);
+      }
+    }
+  };
+  auto stage = [&](int d, int buf) {
+#pragma unroll
+    for (int q = 0; q < LI; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      if (i < NITEM) {
+        const int row = l_row[q], kg = i / RT;  // kg = ks * 4 + g
+        float vr[8], vi[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          vr[j] = R[d][q][j].x;
+          vi[j] = R[d][q][j].y;
+        }
+        bf16x8 a0, a1, a2, a3;
+        split8(vr, a0, a1);
+        split8(vi, a2, a3);
+        As[buf][kg >> 2][0][kg & 3][row] = a0;
+        As[buf][kg >> 2][1][kg & 3][row] = a1;
+        As[buf][kg >> 2][2][kg & 3][row] = a2;
+        As[buf][kg >> 2][3][kg & 3][row] = a3;
+      }
+    }
+  };
+  int tile = grp, ltile = grp;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (ltile < nrt) load(d, ltile);
+    ltile += ng;
+  }
+  int buf = 0;
+  bool live = true;
+  while (live) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (!live) break;
+      stage(d, buf);
+      __syncthreads();
+      if (ltile < nrt) load(d, ltile);
+      ltile += ng;
+      floatx4 accr[2][QT], acci[2][QT];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < QT; ++n) {
+          accr[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+          acci[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        if (ks < nks) {
+#pragma unroll
+          for (int m = 0; m < 2; ++m) {
+            const int row = 16 * m + c16;
+            const bf16x8 arh = As[buf][ks][0][g][row], arl = As[buf][ks][1][g][row],
+                         aih = As[buf][ks][2][g][row], ail = As[buf][ks][3][g][row];
+#pragma unroll
+            for (int n = 0; n < QT; ++n) {
+              accr[m][n] = mfma3(arh, arl, bh[n][ks], bl[n][ks], accr[m][n]);
+              acci[m][n] = mfma3(aih, ail, bh[n][ks], bl[n][ks], acci[m][n]);
+            }
+          }
+        }
+      }
+      // D lane: column q, rows 4g + j of each M tile
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int r = tile * RT + 16 * m + 4 * g + j;
+          if (r < rows) {
+            const int cf = r / ntc, th = r - cf * ntc;
+            float2* o = GT + (int64_t)cf * half * n2c8;
+#pragma unroll
+            for (int n = 0; n < QT; ++n) {
+              const int q = q0 + 16 * n + c16;
+              if (q < npp) {
+                const bool first = q < half;
+                float2* orow = o + (first ? q : q - half) * n2c8;
+                orow[first ? th : n2c - 1 - th] = make_float2(accr[m][n][j], acci[m][n][j]);
+                if (!first && th == 0)
+                  for (int z = n2c; z < n2c8; ++z) orow[z] = make_float2(0.f, 0.f);
+              }
+            }
+          }
+        }
+      tile += ng;
+      live = tile < nrt;
+      buf ^= 1;
+    }
+  }
+}
+
 template <bool ROOT>
 struct AccT;
 template <>
@@ -625,6 +789,23 @@ void build_interp_matrix(int nc, int np, double shift, std::vector<uint16_t>& hi
     }
 }
 
+template <int NKS>
+hipError_t launch_phi_tile(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
+                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st) {
+  constexpr int QT = 2, D = 1, BPC = 2;
+  const int nqb = (npp + 64 * QT - 1) / (64 * QT);
+  const int64_t nrt = (rows + 31) / 32;
+  int64_t ng = std::max<int64_t>(1, 256LL * BPC / nqb);
+  ng = std::min<int64_t>(ng, nrt);
+  const int64_t used = (int64_t)nqb * ng;
+  const int64_t nb = ((used + 7) / 8) * 8;
+  hipLaunchKernelGGL((phi_tile_kernel<NKS, QT, D>), dim3((unsigned)nb), dim3(256), 0, st, X, GT,
+                     (int)rows, ntc, npc, npp, gc_row_len(ntc),
+                     reinterpret_cast<const bf16x8*>(Mhi), reinterpret_cast<const bf16x8*>(Mlo),
+                     kpad, interp_rows_pad(npp), nqb, (int)ng, used);
+  return hipGetLastError();
+}
+
 hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
                            const uint16_t* Mhi, const uint16_t* Mlo, int kpad,
                            hipStream_t st) {
@@ -634,6 +815,15 @@ hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, i
     return hipErrorInvalidValue;
   if ((int64_t)(npp / 2) * gc_row_len(ntc) >= (1LL << 31)) return hipErrorInvalidValue;
   if ((int64_t)interp_rows_pad(npp) * kpad >= (1LL << 31)) return hipErrorInvalidValue;
+  if (!(npc & 1)) {
+    const int nks = (npc + 31) / 32;
+    if (nks <= 2) return launch_phi_tile<2>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
+    if (nks == 3) return launch_phi_tile<3>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
+    if (nks == 4) return launch_phi_tile<4>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
+    if (nks == 5) return launch_phi_tile<5>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
+    if (nks <= 7) return launch_phi_tile<7>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
+    if (nks <= 9) return launch_phi_tile<9>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  }
   constexpr int MTW = 2;
   const int nng = (npp + 63) / 64;
   const int64_t nmg = (rows + 16 * MTW - 1) / (16 * MTW);
