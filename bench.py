@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--depth", default=None, help="override the workload's depth (int or 'auto')")
     ap.add_argument("--autotune", action="store_true", help="wf_autotune before the warm-up")
+    ap.add_argument("--agg-fft-rows", type=int, default=None,
+                    help="tune.agg_fft_rows: levels with 2 nt above it use the FFT transition")
     ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
     ap.add_argument("--streams", type=int, default=4, help="fibers256: worker streams")
     ap.add_argument("--graph", action="store_true", help="capture solves into hipGraphs")
@@ -152,6 +154,8 @@ def main():
     S = len(incs)
     dev = [torch.from_numpy(a).cuda() for a in surf.arrays()]
     solver = wf.POSolver(device=0, precision=args.precision, beta=args.beta, use_graph=args.graph)
+    if args.agg_fft_rows is not None:
+        solver.tune = {"agg_fft_rows": args.agg_fft_rows}
     t0 = time.perf_counter()
     solver.set_geometry(dev, W["depth"])
     torch.cuda.synchronize()
@@ -217,7 +221,8 @@ def main():
         "config": {"workload": args.workload, "description": W["desc"], "n_points": surf.n,
                    "n_directions": D, "solves": S, "depth": W["depth"],
                    "depth_used": st["depth"], "precision": args.precision, "beta": args.beta,
-                   "graph": args.graph, "autotune": tuned, "levels": st["levels"]},
+                   "graph": args.graph, "autotune": tuned, "tune": solver.tune,
+                   "levels": st["levels"]},
         "roofline": roof,
         "solves_per_s": S / (ms_step * 1e-3),
         "kernels_ms": {k: v["ms"] for k, v in kern.items()},

@@ -58,7 +58,8 @@ __device__ inline void load_table(float2* dst, const float2* src, int n) {
   for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = src[t];
 }
 
-__global__ __launch_bounds__(256) void phi_fft_kernel(const float2* __restrict__ X,
+template <int NT>
+__global__ __launch_bounds__(NT) void phi_fft_kernel(const float2* __restrict__ X,
                                                       float2* __restrict__ GT, int ntc, int npc,
                                                       int npp, int RB, Fft2Plan pc, Fft2Plan pp,
                                                       const float2* __restrict__ fac_g) {
@@ -95,8 +96,8 @@ __global__ __launch_bounds__(256) void phi_fft_kernel(const float2* __restrict__
   }
 }
 
-template <int MAXS, bool ROOT>
-__global__ __launch_bounds__(256) void theta_fft_kernel(
+template <int MAXS, bool ROOT, int NT>
+__global__ __launch_bounds__(NT) void theta_fft_kernel(
     const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, int W,
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
     const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int mirror,
@@ -123,7 +124,10 @@ __global__ __launch_bounds__(256) void theta_fft_kernel(
   const int nw = min(W, half - k0);
   const int i2_0 = mirror % n2p;
   const bool dual = (i2_0 < ntp);
-  // slot s = threadIdx.x + 256 m -> (column w = s / n2p, row e = s % n2p)
+  // slot s = threadIdx.x + NT m -> (row e = s / nw, column w = s % nw): lanes
+  // with consecutive w read the translation table and write the parent grid
+  // at consecutive addresses
+  const uint32_t mag_nw = fmagic((uint32_t)nw);
   acc_t acc[MAXS], acc2[MAXS];  // acc2: second sample fed by e = 0 (pole grids)
 #pragma unroll
   for (int m = 0; m < MAXS; ++m) {
@@ -152,9 +156,9 @@ __global__ __launch_bounds__(256) void theta_fft_kernel(
     const float2* To = Text ? Text + (int64_t)oct * (n2p + 1) * half : nullptr;
 #pragma unroll
     for (int m = 0; m < MAXS; ++m) {
-      const int s = threadIdx.x + 256 * m;
+      const int s = threadIdx.x + NT * m;
       if (s < nw * n2p) {
-        const int w = s / n2p, e = s - w * n2p;
+        const int e = (int)fdiv((uint32_t)s, mag_nw), w = s - e * nw;
         const float2 u = Z[w * lmax + e];
         const float2 tu = To ? cmul(To[(int64_t)e * half + k0 + w], u) : u;
         acc[m].x += tu.x;
@@ -170,9 +174,9 @@ __global__ __launch_bounds__(256) void theta_fft_kernel(
   const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
 #pragma unroll
   for (int m = 0; m < MAXS; ++m) {
-    const int s = threadIdx.x + 256 * m;
+    const int s = threadIdx.x + NT * m;
     if (s >= nw * n2p) continue;
-    const int w = s / n2p, e = s - w * n2p;
+    const int e = (int)fdiv((uint32_t)s, mag_nw), w = s - e * nw;
     const int col = k0 + w;
     int row = e, cc = col;
     if (e >= ntp) {
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(256) void theta_fft_kernel(
   }
 }
 
-template <bool ROOT>
+template <bool ROOT, int NT>
 hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_t sh,
                               hipStream_t st) {
   const int half = a.npp / 2;
@@ -205,10 +209,10 @@ hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
 #define WF_THETA_FFT(MS)                                                                          \
   do {                                                                                            \
-    hipError_t e = hipFuncSetAttribute((const void*)theta_fft_kernel<MS, ROOT>,                   \
+    hipError_t e = hipFuncSetAttribute((const void*)theta_fft_kernel<MS, ROOT, NT>,               \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);      \
     if (e != hipSuccess) return e;                                                                \
-    hipLaunchKernelGGL((theta_fft_kernel<MS, ROOT>), dim3((unsigned)nblk), dim3(256), sh, st,      \
+    hipLaunchKernelGGL((theta_fft_kernel<MS, ROOT, NT>), dim3((unsigned)nblk), dim3(NT), sh, st,   \
                        a.GT, a.F, a.ntc, a.npp, a.ntp, W, a.cb, a.ce, a.child_key, a.Text,         \
                        a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root);                           \
     return hipGetLastError();                                                                     \
@@ -218,6 +222,10 @@ hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_
   WF_THETA_FFT(16);
 #undef WF_THETA_FFT
 }
+
+// threads per block of the FFT kernels and the LDS they may use for lines
+int fft_threads() { return 512; }
+constexpr size_t kFftLds = 150 * 1024;
 
 }  // namespace
 
@@ -229,18 +237,27 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
   if (lines_cf <= 0) return hipSuccess;
   const int lmax = npc > npp ? npc : npp;
   if (lmax > kLineBuf || pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
-  int RB = kLineBuf / lmax;
+  const int NT = fft_threads();
+  const size_t tables = sizeof(float2) * ((size_t)pc.twn + pp.twn + npc + 1);
+  int RB = NT == 256 ? kLineBuf / lmax : (int)((kFftLds - tables) / (2 * sizeof(float2) * lmax));
   RB = RB < 1 ? 1 : (RB > ntc ? ntc : RB);
-  const size_t sh = sizeof(float2) * ((size_t)2 * RB * lmax + pc.twn + pp.twn + npc + 1);
+  const size_t sh = sizeof(float2) * (size_t)2 * RB * lmax + tables;
   if (sh > 160 * 1024) return hipErrorInvalidValue;
   const int64_t nblk = lines_cf * ((ntc + RB - 1) / RB);
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute((const void*)phi_fft_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(phi_fft_kernel, dim3((unsigned)nblk), dim3(256), sh, st, X, GT, ntc, npc, npp,
-                     RB, pc, pp, fac);
-  return hipGetLastError();
+#define WF_PHI_FFT(T)                                                                             \
+  do {                                                                                            \
+    hipError_t e = hipFuncSetAttribute((const void*)phi_fft_kernel<T>,                            \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);      \
+    if (e != hipSuccess) return e;                                                                \
+    hipLaunchKernelGGL(phi_fft_kernel<T>, dim3((unsigned)nblk), dim3(T), sh, st, X, GT, ntc, npc,  \
+                       npp, RB, pc, pp, fac);                                                     \
+    return hipGetLastError();                                                                     \
+  } while (0)
+  if (NT == 1024) WF_PHI_FFT(1024);
+  if (NT == 512) WF_PHI_FFT(512);
+  WF_PHI_FFT(256);
+#undef WF_PHI_FFT
 }
 
 hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st) {
@@ -250,15 +267,26 @@ hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st) {
   if (a.mirror != n2p - 1 && a.mirror != n2p) return hipErrorInvalidValue;
   if ((a.out_root != nullptr) == (a.out != nullptr)) return hipErrorInvalidValue;
   const int lmax = n2c > n2p ? n2c : n2p;
-  if (lmax > kLineBuf || n2p > 256 * 16) return hipErrorInvalidValue;
-  int W = kLineBuf / lmax;
+  const int NT = fft_threads();
+  if (lmax > kLineBuf || n2p > NT * 16) return hipErrorInvalidValue;
+  const size_t tables = sizeof(float2) * ((size_t)a.pc.twn + a.pp.twn + n2c + 1);
+  int W = NT == 256 ? kLineBuf / lmax : (int)((kFftLds - tables) / (2 * sizeof(float2) * lmax));
   W = W < 1 ? 1 : (W > half ? half : W);
-  while (W > 1 && W * n2p > 256 * 16) --W;
-  const int slots = (W * n2p + 255) / 256;
-  const size_t sh = sizeof(float2) * ((size_t)2 * W * lmax + a.pc.twn + a.pp.twn + n2c + 1);
+  const int max_slots = NT == 1024 ? 8 : 16;  // accumulator registers per thread
+  while (W > 1 && W * n2p > NT * max_slots) --W;
+  const int slots = (W * n2p + NT - 1) / NT;
+  const size_t sh = sizeof(float2) * (size_t)2 * W * lmax + tables;
   if (sh > 160 * 1024) return hipErrorInvalidValue;
-  if (a.out_root) return launch_theta_fft_t<true>(a, W, slots, sh, st);
-  return launch_theta_fft_t<false>(a, W, slots, sh, st);
+  if (NT == 1024) {
+    if (a.out_root) return launch_theta_fft_t<true, 1024>(a, W, slots, sh, st);
+    return launch_theta_fft_t<false, 1024>(a, W, slots, sh, st);
+  }
+  if (NT == 512) {
+    if (a.out_root) return launch_theta_fft_t<true, 512>(a, W, slots, sh, st);
+    return launch_theta_fft_t<false, 512>(a, W, slots, sh, st);
+  }
+  if (a.out_root) return launch_theta_fft_t<true, 256>(a, W, slots, sh, st);
+  return launch_theta_fft_t<false, 256>(a, W, slots, sh, st);
 }
 
 }  // namespace wf

@@ -362,8 +362,9 @@ struct ThetaK {
 // lo) in registers for the block's lifetime, so the matrix is read once per
 // block.  The block walks the (parent, field) items grp, grp + ng, ...; for
 // every child of an item it stages the child's column tile (all K steps),
-// split into BF16 hi/lo, in LDS ([buffer][k step][plane][k group][column]:
-// MFMA lanes read consecutive 16 B), double buffered with one barrier per
+// split into BF16 hi/lo, in LDS ([buffer][k step][plane][column][k group]:
+// staging writes and MFMA reads both cover consecutive 16 B, free of bank
+// conflicts), double buffered with one barrier per
 // child, while the data of the next D children are in flight in registers.
 // The great-circle rows are contiguous in GT, so each load item is 4 x 16 B
 // of one row.  The parent accumulates in registers in the D layout (lane:
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
   constexpr int BE = 4 * ET * 16, BC = CT * 16;
   constexpr int NITEM = NKS * 4 * BC;  // 8-sample load items per child
   constexpr int LI = (NITEM + 255) / 256;
-  __shared__ bf16x8 Bs[2][NKS][4][4][BC];
+  __shared__ bf16x8 Bs[2][NKS][4][BC][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c16 = lane & 15, g = lane >> 4;
   // XCD-aware mapping: blocks b = x (mod 8) run on XCD x and take a
@@ -473,10 +474,10 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
         bf16x8 a0, a1, a2, a3;
         split8(vr, a0, a1);
         split8(vi, a2, a3);
-        Bs[buf][lks][0][lg][lcol] = a0;
-        Bs[buf][lks][1][lg][lcol] = a1;
-        Bs[buf][lks][2][lg][lcol] = a2;
-        Bs[buf][lks][3][lg][lcol] = a3;
+        Bs[buf][lks][0][lcol][lg] = a0;
+        Bs[buf][lks][1][lcol][lg] = a1;
+        Bs[buf][lks][2][lcol][lg] = a2;
+        Bs[buf][lks][3][lcol][lg] = a3;
       }
     }
   };
@@ -567,8 +568,8 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
 #pragma unroll
           for (int b = 0; b < CT; ++b) {
             const int col = 16 * b + c16;
-            const bf16x8 brh = Bs[buf][ks][0][g][col], brl = Bs[buf][ks][1][g][col],
-                         bih = Bs[buf][ks][2][g][col], bil = Bs[buf][ks][3][g][col];
+            const bf16x8 brh = Bs[buf][ks][0][col][g], brl = Bs[buf][ks][1][col][g],
+                         bih = Bs[buf][ks][2][col][g], bil = Bs[buf][ks][3][col][g];
 #pragma unroll
             for (int a = 0; a < ET; ++a) {
               ur[a][b] = mfma3(ah[a][ks], al[a][ks], brh, brl, ur[a][b]);

@@ -825,9 +825,12 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
                   std::max((size_t)count * ntc * npp, (size_t)wf::gc_elems(count, ntc, npp)), &G));
   const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
   const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
-  if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
-      2 * ntp > h->tune.agg_fft_rows && is_smooth5(npc) && is_smooth5(npp) && is_smooth5(2 * ntc) &&
-      is_smooth5(2 * ntp)) {
+  const bool mfma_ok = h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
+                       2 * ntc <= wf::kThetaMfmaMaxK && npp % 2 == 0;
+  const bool fft2_ok = h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
+                       is_smooth5(npc) && is_smooth5(npp) && is_smooth5(2 * ntc) &&
+                       is_smooth5(2 * ntp);
+  if (fft2_ok && (2 * ntp > h->tune.agg_fft_rows || !mfma_ok)) {
     Fft2Plan f_pc, f_pp, f_tc, f_tp;
     WF_TRY(get_fft2(h, npc, &f_pc));
     WF_TRY(get_fft2(h, npp, &f_pp));
@@ -857,8 +860,7 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
     WF_HIP(wf::launch_theta_fft(ta, st));
     return WF_OK;
   }
-  if (h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
-      2 * ntc <= wf::kThetaMfmaMaxK && npp % 2 == 0) {
+  if (mfma_ok) {
     MatEntry mp, mt;
     WF_TRY(get_mat(h, npc, npp, 0.0, &mp));
     WF_TRY(get_mat(h, 2 * ntc, 2 * ntp, off_tp - off_tc, &mt));
@@ -1180,8 +1182,11 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
     const int mirror = mirror_of(gp.nt, off_p);
     const float2* Tl = gc.T + gc.toff[l];
-    if (h->tune.agg_kernel == 0 && 2 * gp.nt > h->tune.agg_fft_rows && is_smooth5(gp.np) &&
-        is_smooth5(2 * gp.nt) && is_smooth5(gcg.np) && is_smooth5(2 * gcg.nt)) {
+    const bool mfma_ok = h->tune.agg_kernel == 0 && 2 * gcg.nt <= wf::kThetaMfmaMaxK &&
+                         gp.np % 2 == 0;
+    const bool fft2_ok = h->tune.agg_kernel == 0 && is_smooth5(gp.np) && is_smooth5(2 * gp.nt) &&
+                         is_smooth5(gcg.np) && is_smooth5(2 * gcg.nt);
+    if (fft2_ok && (2 * gp.nt > h->tune.agg_fft_rows || !mfma_ok)) {
       // large grids: LDS FFT (each sample read and written once)
       Fft2Plan f_pc, f_pp, f_tc, f_tp;
       WF_TRY(get_fft2(h, gcg.np, &f_pc));
@@ -1219,7 +1224,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       if (l > 0) cur = nxt;
       continue;
     }
-    if (h->tune.agg_kernel == 0 && 2 * gcg.nt <= wf::kThetaMfmaMaxK && gp.np % 2 == 0) {
+    if (mfma_ok) {
       // dense split-BF16 MFMA interpolation: phi GEMM -> GT, theta GEMM +
       // translation + child sum -> parent (FP64 at the root)
       MatEntry mp, mt;
