@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
     ap.add_argument("--streams", type=int, default=4, help="fibers256: worker streams")
     ap.add_argument("--graph", action="store_true", help="capture solves into hipGraphs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 1 GiB device-to-device copy (reference for the PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -136,6 +138,16 @@ def main():
         return 2
 
     import torch
+
+    if args.copy_calibration:
+        n = 1 << 28  # 1 GiB of float32
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device-to-device copy", "bytes": 4 * n}), flush=True)
+        return 0
 
     import wavefiber3d_amd as wf
 
@@ -204,6 +216,16 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "work_per_launch": work, "avg_launch_ms": avg_ms}
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    # this workload (calibrated on a device copy), kept beside the profiles
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("bucket") == dom and tr.get("precision", args.precision) == args.precision:
+            roof["traffic"] = tr["traffic_bytes_per_launch"]
+            roof["traffic_source"] = os.path.relpath(tpath, os.path.dirname(os.path.abspath(__file__)))
 
     out = {
         "metric": "PO point-direction pairs/s (tree vs direct) at \u22641e-3 rel-L2 vs FP64 oracle",
