@@ -142,6 +142,26 @@ def test_tree_sphere_vs_fp64(agg_solver, depth):
         assert rel(I[k], Iref[k]) < 1e-3, (k, rel(I[k], Iref[k]))
 
 
+@pytest.mark.parametrize("depth", [2, 3])
+def test_tree_root_reads_deeper_level(agg_solver, depth):
+    """Output grid coarser than the level-1 grid: with the MFMA / LDS FFT
+    kernels the root evaluates the first level whose grid fits the output grid
+    directly (per-child translation table); the legacy kernels keep every
+    level.  Both must match the FP64 direct sum."""
+    solver = agg_solver
+    s = wf.fibonacci_sphere(20000, 3e-6)
+    nt, np_ = 20, 40
+    print(f"case: root from a deeper level, depth={depth} agg {solver.tune['agg_kernel']}",
+          flush=True)
+    solver.set_geometry(s, depth)
+    I = solver.solve(INCS, (nt, np_))
+    lv = solver.stats()["levels"]
+    assert lv[1][0] > nt or lv[1][1] > np_  # level 1 finer than the output grid
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS], nt, np_)
+    for k in range(len(INCS)):
+        assert rel(I[k], Iref[k]) < 1e-3, (k, rel(I[k], Iref[k]))
+
+
 def test_tree_complex_E_vs_fp64(solver):
     s = wf.fibonacci_sphere(6000, 1.5e-6)
     nt, np_ = 36, 72

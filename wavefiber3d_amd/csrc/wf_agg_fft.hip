@@ -100,9 +100,9 @@ template <int MAXS, bool ROOT, int NT>
 __global__ __launch_bounds__(NT) void theta_fft_kernel(
     const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, int W,
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
-    const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int mirror,
-    Fft2Plan pc, Fft2Plan pp, const float2* __restrict__ fac_g, float2* __restrict__ out,
-    double2* __restrict__ out_root) {
+    const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int text_per_child,
+    int mirror, Fft2Plan pc, Fft2Plan pp, const float2* __restrict__ fac_g,
+    float2* __restrict__ out, double2* __restrict__ out_root) {
   typedef typename FftAcc<ROOT>::type acc_t;
   extern __shared__ float2 sm[];
   const int n2c = 2 * ntc, n2p = 2 * ntp, half = npp / 2;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
     resample(Xs, Ys, nw, lmax, lmax, n2c, n2p, fac);
     __syncthreads();
     float2* Z = wffft2::lines(Ys, Xs, nw, lmax, pp, twp, 1);
-    const int oct = child_key ? (int)(child_key[c] & 7u) : 0;
+    const int oct = text_per_child ? c : (child_key ? (int)(child_key[c] & 7u) : 0);
     const float2* To = Text ? Text + (int64_t)oct * (n2p + 1) * half : nullptr;
 #pragma unroll
     for (int m = 0; m < MAXS; ++m) {
@@ -214,7 +214,7 @@ hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_
     if (e != hipSuccess) return e;                                                                \
     hipLaunchKernelGGL((theta_fft_kernel<MS, ROOT, NT>), dim3((unsigned)nblk), dim3(NT), sh, st,   \
                        a.GT, a.F, a.ntc, a.npp, a.ntp, W, a.cb, a.ce, a.child_key, a.Text,         \
-                       a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root);                           \
+                       a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root);         \
     return hipGetLastError();                                                                     \
   } while (0)
   if (slots <= 4) WF_THETA_FFT(4);

@@ -21,14 +21,15 @@
 //          (the column pair phi_k, phi_k + pi is one great circle), then
 //          parent[e][k] (e < ntp) / parent[mirror - e][k + npp/2] += T U.
 //
-// Both run on v_mfma_f32_16x16x32_bf16 with split-BF16 x3 operands (matrix
-// split on the host, data split in registers), FP32 accumulation; the root
-// level accumulates the child sum in FP64.  The phi pass reads its data
-// fragments straight from HBM in the MFMA lane layout (matrix fragments from
-// L1/L2) and stores each column pair as one contiguous great-circle row
-// (S above, zero padded to a multiple of 8).  The theta pass keeps its
-// matrix fragments in registers for a whole persistent block and stages each
-// child's column tile once per block in LDS, pre-split, for all waves.
+// Both run on v_mfma_f32_16x16x32_bf16 with split-BF16 x3 operands (hi*hi +
+// hi*lo + lo*hi), FP32 accumulation; the root level accumulates the child sum
+// in FP64.  The matrices are split on the host; the data are split once, by
+// the phi pass, which stores for every column pair k one row holding column
+// k and then column k + npp/2, both theta-ascending and padded to ceil8(ntc),
+// as groups of 8 samples in 4 BF16 planes (re hi, re lo, im hi, im lo).  The
+// theta matrix's columns are permuted to that order (build_theta_matrix_gc),
+// so the theta pass copies 64 B groups straight into LDS.  Each pass keeps
+// its matrix fragments in registers for a whole persistent block.
 #include <stdint.h>
 
 #include <algorithm>
@@ -61,125 +62,29 @@ __device__ inline floatx4 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8
   return c;
 }
 
-// phi pass.  Wave task = (MTW tiles of 16 complex rows) x (4 tiles of 16
-// output columns).  A = data rows (re and im parts as two M tiles sharing one
-// load), B = Mphi^T: lane (col q, k-group g) reads Mphi[q][k0 + 8g .. +7].
-// Loads are branchless (clamped index + select); the next K step's data is
-// loaded before the current step's MFMAs.
-template <int MTW>
-__global__ __launch_bounds__(256) void phi_mfma_kernel(
-    const float2* __restrict__ X, float2* __restrict__ GT, int rows, int ntc, int npc, int npp,
-    int n2c8, const bf16x8* __restrict__ Mhi, const bf16x8* __restrict__ Mlo, int kpad, int nng,
-    int ntasks) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int task = blockIdx.x * 4 + wave;
-  if (task >= ntasks) return;  // no barriers below
-  const int ng = task % nng, mg = task / nng;
-  const int c16 = lane & 15, g = lane >> 4;
-  const int row0 = mg * 16 * MTW;
-  floatx4 accr[MTW][4], acci[MTW][4];
-#pragma unroll
-  for (int m = 0; m < MTW; ++m)
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      accr[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
-      acci[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
-    }
-  const float2* xr[MTW];
-  bool rok[MTW];
-#pragma unroll
-  for (int m = 0; m < MTW; ++m) {
-    const int r = row0 + 16 * m + c16;
-    rok[m] = r < rows;
-    xr[m] = X + (int64_t)(rok[m] ? r : 0) * npc;
-  }
-  const bf16x8* mh[4];
-  const bf16x8* ml[4];
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    const int q = ng * 64 + 16 * n + c16;  // < padded rows of Mphi
-    mh[n] = Mhi + ((q * kpad) >> 3);
-    ml[n] = Mlo + ((q * kpad) >> 3);
-  }
-  float2 nx[MTW][8];
-  auto load = [&](int k0) {
-    const int kb = k0 + 8 * g;
-#pragma unroll
-    for (int m = 0; m < MTW; ++m)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const bool ok = rok[m] && (kb + j < npc);
-        const float2 v = xr[m][ok ? kb + j : 0];
-        nx[m][j] = ok ? v : make_float2(0.f, 0.f);
-      }
-  };
-  load(0);
-  for (int k0 = 0; k0 < npc; k0 += 32) {
-    const int kb = k0 + 8 * g;
-    bf16x8 arh[MTW], arl[MTW], aih[MTW], ail[MTW];
-#pragma unroll
-    for (int m = 0; m < MTW; ++m) {
-      float vr[8], vi[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        vr[j] = nx[m][j].x;
-        vi[j] = nx[m][j].y;
-      }
-      split8(vr, arh[m], arl[m]);
-      split8(vi, aih[m], ail[m]);
-    }
-    if (k0 + 32 < npc) load(k0 + 32);  // in flight during the MFMAs below
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const bf16x8 bh = mh[n][kb >> 3], bl = ml[n][kb >> 3];
-#pragma unroll
-      for (int m = 0; m < MTW; ++m) {
-        accr[m][n] = mfma3(arh[m], arl[m], bh, bl, accr[m][n]);
-        acci[m][n] = mfma3(aih[m], ail[m], bh, bl, acci[m][n]);
-      }
-    }
-  }
-  // D lane: column q, rows 4g + j of each M tile; stored in great-circle
-  // order (column q < npp/2 ascending in theta, column q + npp/2 descending
-  // after it), the row padded with zeros to n2c8
-  const int half = npp / 2, n2c = 2 * ntc;
-#pragma unroll
-  for (int m = 0; m < MTW; ++m)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = row0 + 16 * m + 4 * g + j;
-      if (r >= rows) continue;
-      const int cf = r / ntc, th = r - cf * ntc;
-      float2* o = GT + (int64_t)cf * half * n2c8;
-#pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        const int q = ng * 64 + 16 * n + c16;
-        if (q < npp) {
-          const bool first = q < half;
-          float2* orow = o + (first ? q : q - half) * n2c8;
-          orow[first ? th : n2c - 1 - th] = make_float2(accr[m][n][j], acci[m][n][j]);
-          if (!first && th == 0)
-            for (int z = n2c; z < n2c8; ++z) orow[z] = make_float2(0.f, 0.f);
-        }
-      }
-    }
-}
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 
-// phi pass for K <= 288 (NKS 32-sample steps), matrix-stationary.  Block =
-// 4 waves side by side in q (wave: QT 16-column tiles, block: 64 QT
-// columns); every wave keeps its matrix fragments (QT tiles x NKS K steps,
-// hi and lo) in registers.  The block walks 32-row tiles with stride ng;
-// each tile (32 rows x npc, contiguous in X) is loaded coalesced, split
-// into BF16 hi/lo and staged in LDS [buffer][k step][plane][k group][row]
-// (MFMA lanes read consecutive 16 B), double buffered with one barrier per
-// tile and the next D tiles in flight in registers.  Output as phi_mfma.
+// phi pass, matrix-stationary.  Rows are virtual: plane cf = (node, field)
+// owns ntc8 = ceil8(ntc) consecutive rows (theta), the ones with theta >= ntc
+// read as zero, so every aligned group of 8 rows lies in one plane and starts
+// at a theta that is a multiple of 8.  Block = 4 waves side by side in q
+// (wave: QT 16-column tiles, block: 64 QT columns); every wave keeps its
+// matrix fragments (QT tiles x NKS K steps, hi and lo) in registers.  The
+// block walks 32-row tiles with stride ng; each tile (32 rows x npc,
+// contiguous in X within a plane) is loaded coalesced, split into BF16 hi/lo
+// and staged in LDS [buffer][k step][plane][k group][row] (MFMA lanes read
+// consecutive 16 B), double buffered with one barrier per tile and the next
+// D tiles in flight in registers.  Output: the D lane holds rows 4g..4g+3 of
+// column q, i.e. half of an 8-sample group; it splits them into BF16 hi/lo
+// and writes 8 B into each of the group's 4 planes (re hi, re lo, im hi,
+// im lo) of the pre-split great-circle row of q.
 template <int NKS, int QT, int D>
 __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict__ X,
-                                                    float2* __restrict__ GT, int rows, int ntc,
-                                                    int npc, int npp, int n2c8,
-                                                    const bf16x8* __restrict__ Mhi,
-                                                    const bf16x8* __restrict__ Mlo, int kpad,
-                                                    int mrows, int nqb, int ng, int64_t used) {
+                                                       bf16x4* __restrict__ GT, int planes,
+                                                       int ntc, int npc, int npp,
+                                                       const bf16x8* __restrict__ Mhi,
+                                                       const bf16x8* __restrict__ Mlo, int kpad,
+                                                       int mrows, int nqb, int ng, int64_t used) {
   constexpr int RT = 32;
   constexpr int NITEM = RT * NKS * 4;  // (row, 8-sample group) items per tile
   constexpr int LI = (NITEM + 255) / 256;
@@ -191,10 +96,13 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
   if (w >= used) return;
   const int qb = (int)(w % nqb);
   const int grp = (int)(w / nqb);
+  const int ntc8 = (ntc + 7) & ~7;
+  const int rows = planes * ntc8;  // virtual rows
   const int nrt = (rows + RT - 1) / RT;
   if (grp >= nrt) return;
   const int nks = (npc + 31) >> 5;
-  const int half = npp / 2, n2c = 2 * ntc;
+  const int half = npp / 2;
+  const int ngrp = ntc8 >> 2;  // 8-sample groups per great-circle row (2 ntc8 / 8)
   const int q0 = qb * 64 * QT + wave * 16 * QT;
   // resident matrix fragments: B lane (col q = c16, k group g)
   bf16x8 bh[QT][NKS], bl[QT][NKS];
@@ -212,24 +120,20 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
     }
   }
   // load items: i -> row i % RT, (ks, g) = i / RT
-  int l_row[LI], l_jb[LI];
-#pragma unroll
-  for (int q = 0; q < LI; ++q) {
-    const int i = threadIdx.x + 256 * q;
-    l_row[q] = i % RT;
-    l_jb[q] = (i / RT) * 8;
-  }
   float2 R[D][LI][8];
   auto load = [&](int d, int tile) {
 #pragma unroll
     for (int q = 0; q < LI; ++q) {
       const int i = threadIdx.x + 256 * q;
-      const int r = tile * RT + l_row[q];
-      const bool rok = i < NITEM && r < rows;
-      const float4* src = reinterpret_cast<const float4*>(X + (int64_t)(rok ? r : 0) * npc);
+      const int rv = tile * RT + (i % RT);
+      const int cf = (int)((uint32_t)rv / (uint32_t)ntc8), th = rv - cf * ntc8;
+      const bool rok = i < NITEM && rv < rows && th < ntc;
+      const float4* src =
+          reinterpret_cast<const float4*>(X + (int64_t)(rok ? cf * ntc + th : 0) * npc);
+      const int jb = (i / RT) * 8;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int jj = l_jb[q] + 2 * j;  // npc even: a pair never straddles the row end
+        const int jj = jb + 2 * j;  // npc even: a pair never straddles the row end
         const bool ok = rok && jj < npc;
         float4 v = src[ok ? (jj >> 1) : 0];
         if (!ok) v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -243,7 +147,7 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
     for (int q = 0; q < LI; ++q) {
       const int i = threadIdx.x + 256 * q;
       if (i < NITEM) {
-        const int row = l_row[q], kg = i / RT;  // kg = ks * 4 + g
+        const int row = i % RT, kg = i / RT;  // kg = ks * 4 + g
         float vr[8], vi[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -300,28 +204,39 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
           }
         }
       }
-      // D lane: column q, rows 4g + j of each M tile
+      // D lane: column q, rows 4g..4g+3 of each M tile = half of one group
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+      for (int m = 0; m < 2; ++m) {
+        const int r4 = tile * RT + 16 * m + 4 * g;  // first of the lane's 4 rows
+        if (r4 < rows) {
+          const int cf = (int)((uint32_t)r4 / (uint32_t)ntc8), th = r4 - cf * ntc8;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int r = tile * RT + 16 * m + 4 * g + j;
-          if (r < rows) {
-            const int cf = r / ntc, th = r - cf * ntc;
-            float2* o = GT + (int64_t)cf * half * n2c8;
+          for (int n = 0; n < QT; ++n) {
+            const int q = q0 + 16 * n + c16;
+            if (q < npp) {
+              const bool second = q >= half;
+              const int k = second ? q - half : q;
+              const int jg = ((second ? ntc8 : 0) + th) >> 3;
+              bf16x4 rh, rl, ih, il;
 #pragma unroll
-            for (int n = 0; n < QT; ++n) {
-              const int q = q0 + 16 * n + c16;
-              if (q < npp) {
-                const bool first = q < half;
-                float2* orow = o + (first ? q : q - half) * n2c8;
-                orow[first ? th : n2c - 1 - th] = make_float2(accr[m][n][j], acci[m][n][j]);
-                if (!first && th == 0)
-                  for (int z = n2c; z < n2c8; ++z) orow[z] = make_float2(0.f, 0.f);
+              for (int j = 0; j < 4; ++j) {
+                const float vr = accr[m][n][j], vi = acci[m][n][j];
+                const __bf16 hr = (__bf16)vr, hi_ = (__bf16)vi;
+                rh[j] = hr;
+                rl[j] = (__bf16)(vr - (float)hr);
+                ih[j] = hi_;
+                il[j] = (__bf16)(vi - (float)hi_);
               }
+              // group (cf, k, jg): 4 planes x 2 halves of 4 samples
+              bf16x4* o = GT + ((((int64_t)cf * half + k) * ngrp + jg) << 3) + (g & 1);
+              o[0] = rh;
+              o[2] = rl;
+              o[4] = ih;
+              o[6] = il;
             }
           }
         }
+      }
       tile += ng;
       live = tile < nrt;
       buf ^= 1;
@@ -341,7 +256,7 @@ struct AccT<true> {
 };
 
 struct ThetaK {
-  const float2* GT;  // [child][F][npp/2][n2c8], great-circle rows
+  const uint4* GT;  // [child][F][npp/2][n2c8 / 8][4 planes], pre-split great-circle rows
   int F, ntc, npp, ntp, n2c8;
   int nitems, ng, ngF, ngR;  // items (parent, field); stride ng = ngF F + ngR
   int neb, ncb, kpad, mrows, mirror;
@@ -350,6 +265,7 @@ struct ThetaK {
   const int32_t* ce;
   const uint32_t* child_key;
   const float2* Text;
+  int text_per_child;  // Text indexed by child instead of octant
   const bf16x8* Mhi;
   const bf16x8* Mlo;
   float2* out;
@@ -366,8 +282,8 @@ struct ThetaK {
 // staging writes and MFMA reads both cover consecutive 16 B, free of bank
 // conflicts), double buffered with one barrier per
 // child, while the data of the next D children are in flight in registers.
-// The great-circle rows are contiguous in GT, so each load item is 4 x 16 B
-// of one row.  The parent accumulates in registers in the D layout (lane:
+// The great-circle rows are contiguous in GT and already split, so each load
+// item is one 64 B group (4 planes x 8 samples) copied to LDS unchanged.  The parent accumulates in registers in the D layout (lane:
 // column k, rows 4g + j); row e of the extended grid feeds parent (e, k)
 // when e < ntp and (mirror - e, k + npp/2) otherwise when that row is < ntp;
 // with a sample at the pole (mirror = 2 ntp) row 0 also feeds (0, k +
@@ -392,9 +308,9 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
   const int eb = (int)(t1 % P.neb);
   const int grp = (int)(t1 / P.neb);
   if (grp >= P.nitems) return;  // whole block
-  const int F = P.F, ntc = P.ntc, ntp = P.ntp, npp = P.npp, n2c8 = P.n2c8;
+  const int F = P.F, ntp = P.ntp, npp = P.npp, n2c8 = P.n2c8;
   const int half = npp / 2, n2p = 2 * ntp;
-  const int nks = (2 * ntc + 31) >> 5;
+  const int nks = (n2c8 + 31) >> 5;
   const int e0 = eb * BE + wave * ET * 16;
   const int k0 = cbk * BC;
   const int i2_0 = P.mirror % n2p;
@@ -424,7 +340,7 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
     const int lk = k0 + lcol;
     const int jb = (lks * 4 + lg) * 8;
     l_ok[q] = i < NITEM && lk < half && jb < n2c8;
-    l_off[q] = l_ok[q] ? lk * n2c8 + jb : 0;
+    l_off[q] = l_ok[q] ? (lk * n2c8 + jb) >> 1 : 0;  // 16 B units: 8 samples = 4 x 16 B
   }
   // translation-table offsets (clamped for lanes that are never stored)
   int tofs[ET][CT][4], tofs2[CT];
@@ -441,23 +357,21 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
       }
     tofs2[b] = cok ? n2p * half + col : 0;
   }
-  const int64_t slab = (int64_t)half * n2c8;
-  float2 R[D][LI][8];
+  const int64_t slab = (int64_t)half * n2c8 / 2;  // 16 B units per (child, field)
+  uint4 R[D][LI][4];
   int okey[D];
   auto load = [&](int d, int c, int f) {
-    const float2* S = P.GT + ((int64_t)c * F + f) * slab;
+    const uint4* S = P.GT + ((int64_t)c * F + f) * slab;
 #pragma unroll
     for (int q = 0; q < LI; ++q) {
-      const float4* src = reinterpret_cast<const float4*>(S + l_off[q]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        float4 v = src[j];
-        if (!l_ok[q]) v = make_float4(0.f, 0.f, 0.f, 0.f);
-        R[d][q][2 * j] = make_float2(v.x, v.y);
-        R[d][q][2 * j + 1] = make_float2(v.z, v.w);
+        uint4 v = S[l_off[q] + j];
+        if (!l_ok[q]) v = make_uint4(0u, 0u, 0u, 0u);
+        R[d][q][j] = v;
       }
     }
-    okey[d] = P.child_key ? (int)(P.child_key[c] & 7u) : 0;
+    okey[d] = P.text_per_child ? c : (P.child_key ? (int)(P.child_key[c] & 7u) : 0);
   };
   auto stage = [&](int d, int buf) {
 #pragma unroll
@@ -465,19 +379,9 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
       const int i = threadIdx.x + 256 * q;
       if (i < NITEM) {
         const int lg = i & 3, lcol = (i >> 2) % BC, lks = (i >> 2) / BC;
-        float vr[8], vi[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          vr[j] = R[d][q][j].x;
-          vi[j] = R[d][q][j].y;
-        }
-        bf16x8 a0, a1, a2, a3;
-        split8(vr, a0, a1);
-        split8(vi, a2, a3);
-        Bs[buf][lks][0][lcol][lg] = a0;
-        Bs[buf][lks][1][lcol][lg] = a1;
-        Bs[buf][lks][2][lcol][lg] = a2;
-        Bs[buf][lks][3][lcol][lg] = a3;
+        for (int pl = 0; pl < 4; ++pl)
+          Bs[buf][lks][pl][lcol][lg] = *reinterpret_cast<const bf16x8*>(&R[d][q][pl]);
       }
     }
   };
@@ -496,6 +400,10 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
         xcend = P.cb ? P.ce[xp] : xp + 1;
       }
     }
+  };
+  // does the compute cursor have a child after (xit, xc)?
+  auto next_exists = [&](int xit, int xc, int xcend) {
+    return xc + 1 < xcend || xit + P.ng < P.nitems;
   };
   int it = grp;
   int p = it / F, f = it - p * F;
@@ -522,6 +430,29 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
     acc2[b].x = 0;
     acc2[b].y = 0;
   }
+  // translation factors of the next child to be computed (unit if none)
+  float2 tfn[ET][CT][4], tf2n[CT];
+#pragma unroll
+  for (int b = 0; b < CT; ++b) {
+#pragma unroll
+    for (int a = 0; a < ET; ++a)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tfn[a][b][j] = make_float2(1.f, 0.f);
+    tf2n[b] = make_float2(1.f, 0.f);
+  }
+  auto load_t = [&](int oct) {
+    const float2* To = P.Text + (int64_t)oct * (n2p + 1) * half;
+#pragma unroll
+    for (int b = 0; b < CT; ++b) {
+#pragma unroll
+      for (int a = 0; a < ET; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tfn[a][b][j] = To[tofs[a][b][j]];
+      tf2n[b] = To[tofs2[b]];
+    }
+  };
+  if (P.Text) load_t(okey[0]);
+  // children still to stage after the current one (compute order)
   int buf = 0;
   bool live = true;
   while (live) {
@@ -529,31 +460,22 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
     for (int d = 0; d < D; ++d) {
       if (!live) break;
       stage(d, buf);
-      const int oct = okey[d];
       __syncthreads();
+      const bool more_next = next_exists(it, c, cend);
       if (lit < P.nitems) load(d, lc, lf);
       advance(lit, lp, lf, lc, lcend);
+      // translation factors: this child's were loaded one child ahead; issue
+      // the next child's now (its octant is already in the queue)
       float2 tf[ET][CT][4], tf2[CT];
-      if (P.Text) {
-        const float2* To = P.Text + oct * (n2p + 1) * half;
 #pragma unroll
-        for (int b = 0; b < CT; ++b) {
+      for (int b = 0; b < CT; ++b) {
 #pragma unroll
-          for (int a = 0; a < ET; ++a)
+        for (int a = 0; a < ET; ++a)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) tf[a][b][j] = To[tofs[a][b][j]];
-          tf2[b] = To[tofs2[b]];
-        }
-      } else {
-#pragma unroll
-        for (int b = 0; b < CT; ++b) {
-#pragma unroll
-          for (int a = 0; a < ET; ++a)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) tf[a][b][j] = make_float2(1.f, 0.f);
-          tf2[b] = make_float2(1.f, 0.f);
-        }
+          for (int j = 0; j < 4; ++j) tf[a][b][j] = tfn[a][b][j];
+        tf2[b] = tf2n[b];
       }
+      if (P.Text && more_next) load_t(okey[(d + 1) % D]);
       floatx4 ur[ET][CT], ui[ET][CT];
 #pragma unroll
       for (int a = 0; a < ET; ++a)
@@ -702,14 +624,81 @@ __global__ __launch_bounds__(256) void trans_ext_kernel(float2* __restrict__ Tex
   }
 }
 
-int64_t blocks_for(int64_t tasks) { return (tasks + 3) / 4; }
+// Per-child extended translation table (children at arbitrary offsets from
+// the parent centre, e.g. the level-r nodes feeding the output grid
+// directly): T[c][e][k] = exp(+i k0 s(e, k) . d_c), same rows as above.
+__global__ __launch_bounds__(256) void trans_child_kernel(float2* __restrict__ T, int ntp, int npp,
+                                                          int mirror, double theta0,
+                                                          const double* __restrict__ cx,
+                                                          const double* __restrict__ cy,
+                                                          const double* __restrict__ cz, int nch,
+                                                          double k_rev) {
+  const int half = npp / 2, n2p = 2 * ntp;
+  const int64_t per_c = (int64_t)(n2p + 1) * half;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < per_c * nch;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t / per_c);
+    const int64_t d = t - (int64_t)c * per_c;
+    const int e = (int)(d / half), k = (int)(d - (int64_t)e * half);
+    int row = -1, col = k;
+    if (e < n2p && e < ntp) {
+      row = e;
+    } else {
+      const int ee = (e == n2p) ? 0 : e;
+      int i2 = (mirror - ee) % n2p;
+      if (i2 < 0) i2 += n2p;
+      if (i2 < ntp) {
+        row = i2;
+        col = k + half;
+      }
+    }
+    float2 v = make_float2(0.f, 0.f);
+    if (row >= 0) {
+      const double th = theta0 + WF_PI / ntp * row;
+      const double ph = WF_TWO_PI / npp * col;
+      double st, ct, sp, cp;
+      sincos(th, &st, &ct);
+      sincos(ph, &sp, &cp);
+      v = cis_rev(k_rev * (cx[c] * ct * cp + cy[c] * ct * sp + cz[c] * st));
+    }
+    T[t] = v;
+  }
+}
+
+// Offsets d_c = c_node - c_root of the level-r nodes from their keys (r
+// octant digits, level 1 most significant): the digit o of level l + 1 moves
+// the centre by (sx hx_l, sy hy_l, sz hz_l), sx = o&1 ? 1 : -1, sy = o&2 ? -1
+// : 1, sz = o&4 ? -1 : 1 (the octant code of the split planes).
+struct LevelOffsets {
+  double hx[16], hy[16], hz[16];
+};
+__global__ __launch_bounds__(256) void child_offsets_kernel(const uint32_t* __restrict__ keys,
+                                                            int n, int r, LevelOffsets H,
+                                                            double* __restrict__ cx,
+                                                            double* __restrict__ cy,
+                                                            double* __restrict__ cz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t key = keys[i];
+  double x = 0, y = 0, z = 0;
+  for (int l = 0; l < r; ++l) {
+    const uint32_t o = (key >> (3 * (r - 1 - l))) & 7u;
+    x += (o & 1) ? H.hx[l] : -H.hx[l];
+    y += (o & 2) ? -H.hy[l] : H.hy[l];
+    z += (o & 4) ? -H.hz[l] : H.hz[l];
+  }
+  cx[i] = x;
+  cy[i] = y;
+  cz[i] = z;
+}
+
 
 // blocks_per_cu: resident blocks aimed at per CU (sets the item stride)
 template <int ET, int CT, int NKS, int D, bool ROOT, int MINB>
 hipError_t launch_theta_t(const wf::ThetaMfmaArgs& a, int blocks_per_cu, hipStream_t st) {
   constexpr int BE = 4 * ET * 16, BC = CT * 16;
   ThetaK P;
-  P.GT = a.GT;
+  P.GT = reinterpret_cast<const uint4*>(a.GT);
   P.F = a.F;
   P.ntc = a.ntc;
   P.npp = a.npp;
@@ -736,6 +725,7 @@ hipError_t launch_theta_t(const wf::ThetaMfmaArgs& a, int blocks_per_cu, hipStre
   P.ce = a.ce;
   P.child_key = a.child_key;
   P.Text = a.Text;
+  P.text_per_child = a.text_per_child;
   P.Mhi = reinterpret_cast<const bf16x8*>(a.Mhi);
   P.Mlo = reinterpret_cast<const bf16x8*>(a.Mlo);
   P.out = a.out;
@@ -790,56 +780,66 @@ void build_interp_matrix(int nc, int np, double shift, std::vector<uint16_t>& hi
     }
 }
 
+void build_theta_matrix_gc(int ntc, int n2p, double shift, std::vector<uint16_t>& hi,
+                           std::vector<uint16_t>& lo, int& rows_pad, int& kpad) {
+  const int nc = 2 * ntc, h8 = gc_half_len(ntc);
+  rows_pad = interp_rows_pad(n2p);
+  kpad = interp_kpad(2 * h8);
+  hi.assign((size_t)rows_pad * kpad, 0);
+  lo.assign((size_t)rows_pad * kpad, 0);
+  for (int q = 0; q < n2p; ++q)
+    for (int j = 0; j < 2 * h8; ++j) {
+      const int t = j < h8 ? j : j - h8;  // theta of the stored sample
+      if (t >= ntc) continue;
+      const int pos = j < h8 ? t : nc - 1 - t;  // its great-circle position
+      const double v = dirichlet(nc, WF_TWO_PI * q / n2p + shift - WF_TWO_PI * pos / nc);
+      const uint16_t hb = bf16_bits((float)v);
+      hi[(size_t)q * kpad + j] = hb;
+      lo[(size_t)q * kpad + j] = bf16_bits((float)(v - (double)bf16_value(hb)));
+    }
+}
+
 template <int NKS>
-hipError_t launch_phi_tile(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
+hipError_t launch_phi_tile(const float2* X, float2* GT, int64_t planes, int ntc, int npc, int npp,
                            const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st) {
   constexpr int QT = 2, D = 1, BPC = 2;
   const int nqb = (npp + 64 * QT - 1) / (64 * QT);
-  const int64_t nrt = (rows + 31) / 32;
+  const int64_t nrt = (planes * gc_half_len(ntc) + 31) / 32;
   int64_t ng = std::max<int64_t>(1, 256LL * BPC / nqb);
   ng = std::min<int64_t>(ng, nrt);
   const int64_t used = (int64_t)nqb * ng;
   const int64_t nb = ((used + 7) / 8) * 8;
-  hipLaunchKernelGGL((phi_tile_kernel<NKS, QT, D>), dim3((unsigned)nb), dim3(256), 0, st, X, GT,
-                     (int)rows, ntc, npc, npp, gc_row_len(ntc),
+  hipLaunchKernelGGL((phi_tile_kernel<NKS, QT, D>), dim3((unsigned)nb), dim3(256), 0, st, X,
+                     reinterpret_cast<bf16x4*>(GT), (int)planes, ntc, npc, npp,
                      reinterpret_cast<const bf16x8*>(Mhi), reinterpret_cast<const bf16x8*>(Mlo),
                      kpad, interp_rows_pad(npp), nqb, (int)ng, used);
   return hipGetLastError();
 }
 
 hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
-                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad,
-                           hipStream_t st) {
+                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st) {
   if (rows <= 0) return hipSuccess;
-  if (rows > 0x7fffffffLL - 64 || ntc < 1 || npc < 1 || npp < 2 || (npp & 1) || kpad < npc ||
-      (kpad & 31))
+  if (ntc < 1 || npc < 2 || (npc & 1) || npc > kPhiMfmaMaxK || npp < 2 || (npp & 1) ||
+      kpad < npc || (kpad & 31) || rows % ntc)
     return hipErrorInvalidValue;
-  if ((int64_t)(npp / 2) * gc_row_len(ntc) >= (1LL << 31)) return hipErrorInvalidValue;
-  if ((int64_t)interp_rows_pad(npp) * kpad >= (1LL << 31)) return hipErrorInvalidValue;
-  if (!(npc & 1)) {
-    const int nks = (npc + 31) / 32;
-    if (nks <= 2) return launch_phi_tile<2>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
-    if (nks == 3) return launch_phi_tile<3>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
-    if (nks == 4) return launch_phi_tile<4>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
-    if (nks == 5) return launch_phi_tile<5>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
-    if (nks <= 7) return launch_phi_tile<7>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
-    if (nks <= 9) return launch_phi_tile<9>(X, GT, rows, ntc, npc, npp, Mhi, Mlo, kpad, st);
-  }
-  constexpr int MTW = 2;
-  const int nng = (npp + 63) / 64;
-  const int64_t nmg = (rows + 16 * MTW - 1) / (16 * MTW);
-  const int64_t ntasks = nmg * nng;
-  if (ntasks > 0x7fffffffLL - 4) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((phi_mfma_kernel<MTW>), dim3((unsigned)blocks_for(ntasks)), dim3(256), 0, st,
-                     X, GT, (int)rows, ntc, npc, npp, gc_row_len(ntc), reinterpret_cast<const bf16x8*>(Mhi),
-                     reinterpret_cast<const bf16x8*>(Mlo), kpad, nng, (int)ntasks);
-  return hipGetLastError();
+  const int64_t planes = rows / ntc;
+  if (planes * gc_half_len(ntc) > 0x7fffffffLL - 64 ||
+      (int64_t)(npp / 2) * gc_row_len(ntc) >= (1LL << 31) ||
+      (int64_t)interp_rows_pad(npp) * kpad >= (1LL << 31))
+    return hipErrorInvalidValue;
+  const int nks = (npc + 31) / 32;
+  if (nks <= 2) return launch_phi_tile<2>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks == 3) return launch_phi_tile<3>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks == 4) return launch_phi_tile<4>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks == 5) return launch_phi_tile<5>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks <= 7) return launch_phi_tile<7>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  return launch_phi_tile<9>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
 }
 
 hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
   if (a.nparents <= 0) return hipSuccess;
-  if (a.ntc < 1 || a.ntp < 1 || a.npp < 2 || (a.npp & 1) || a.kpad < 2 * a.ntc || (a.kpad & 31) ||
-      a.F < 1 || 2 * a.ntc > kThetaMfmaMaxK)
+  if (a.ntc < 1 || a.ntp < 1 || a.npp < 2 || (a.npp & 1) || a.kpad < gc_row_len(a.ntc) ||
+      (a.kpad & 31) || a.F < 1 || gc_row_len(a.ntc) > kThetaMfmaMaxK)
     return hipErrorInvalidValue;
   if (a.mirror != 2 * a.ntp - 1 && a.mirror != 2 * a.ntp) return hipErrorInvalidValue;
   if ((a.out_root != nullptr) == (a.out != nullptr)) return hipErrorInvalidValue;
@@ -849,7 +849,7 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
     return hipErrorInvalidValue;
   // tile by K steps (32 great-circle samples each), measured on the sphere
   // and fiber level shapes
-  const int nks = (2 * a.ntc + 31) / 32;
+  const int nks = (gc_row_len(a.ntc) + 31) / 32;
   if (a.out_root != nullptr) {
     if (nks <= 4) return launch_theta_t<2, 1, 4, 2, true, 1>(a, 2, st);
     if (nks <= 9) return launch_theta_t<2, 2, 9, 1, true, 1>(a, 2, st);
@@ -871,6 +871,34 @@ hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double t
   if (blocks < 1) return hipSuccess;
   hipLaunchKernelGGL(trans_ext_kernel, dim3((unsigned)blocks), dim3(256), 0, st, Text, ntp, npp,
                      mirror, theta0, h[0], h[1], h[2], k0 / WF_TWO_PI);
+  return hipGetLastError();
+}
+
+hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
+                              const double* cx, const double* cy, const double* cz, int nch,
+                              double k0, hipStream_t st) {
+  const int64_t tot = (int64_t)nch * (2 * ntp + 1) * (npp / 2);
+  int64_t blocks = (tot + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  if (blocks < 1) return hipSuccess;
+  hipLaunchKernelGGL(trans_child_kernel, dim3((unsigned)blocks), dim3(256), 0, st, T, ntp, npp,
+                     mirror, theta0, cx, cy, cz, nch, k0 / WF_TWO_PI);
+  return hipGetLastError();
+}
+
+hipError_t launch_child_offsets(const uint32_t* keys, int n, int r, const double* hx,
+                                const double* hy, const double* hz, double* cx, double* cy,
+                                double* cz, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (r < 1 || r > 10) return hipErrorInvalidValue;
+  LevelOffsets H;
+  for (int l = 0; l < 16; ++l) {
+    H.hx[l] = l < r ? hx[l] : 0.0;
+    H.hy[l] = l < r ? hy[l] : 0.0;
+    H.hz[l] = l < r ? hz[l] : 0.0;
+  }
+  hipLaunchKernelGGL(child_offsets_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     keys, n, r, H, cx, cy, cz);
   return hipGetLastError();
 }
 

@@ -57,6 +57,7 @@ struct LevelDev {
   int32_t* cb = nullptr;     // children range in the next level's list
   int32_t* ce = nullptr;
   int32_t* parent = nullptr; // index of the parent in the previous level's list
+  int32_t* span = nullptr;   // {0, count}: the whole level as one child range
 };
 
 struct FftEntry {
@@ -251,10 +252,12 @@ wf_status get_fft(wf_handle* h, int n, FftPlanDev* out) {
 
 // Interpolation matrix from nc samples to np samples with output origin
 // shifted by `shift` (radians of the period-2pi parameter), cached per handle.
-wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out) {
+// Interpolation matrix nc -> np samples (gc: the theta matrix of the MFMA
+// pass, nc = 2 ntc, columns in the phi pass's great-circle row order).
+wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out, bool gc = false) {
   uint64_t sb;
   memcpy(&sb, &shift, sizeof sb);
-  const auto key = std::make_pair(std::make_pair(nc, np), sb);
+  const auto key = std::make_pair(std::make_pair(gc ? -nc : nc, np), sb);
   auto it = h->mats.find(key);
   if (it != h->mats.end()) {
     *out = it->second;
@@ -262,7 +265,10 @@ wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out) {
   }
   std::vector<uint16_t> hi, lo;
   MatEntry e;
-  wf::build_interp_matrix(nc, np, shift, hi, lo, e.rows_pad, e.kpad);
+  if (gc)
+    wf::build_theta_matrix_gc(nc / 2, np, shift, hi, lo, e.rows_pad, e.kpad);
+  else
+    wf::build_interp_matrix(nc, np, shift, hi, lo, e.rows_pad, e.kpad);
   WF_HIP(hipMalloc(&e.hi, sizeof(uint16_t) * hi.size()));
   WF_HIP(hipMalloc(&e.lo, sizeof(uint16_t) * lo.size()));
   WF_HIP(hipMemcpy(e.hi, hi.data(), sizeof(uint16_t) * hi.size(), hipMemcpyHostToDevice));
@@ -760,10 +766,19 @@ wf_status wf_set_geometry(wf_handle* h, const wf_surface* s, int32_t depth, void
   int hc[16] = {0};
   WF_HIP(hipMemcpyAsync(hc, cnt_dev, sizeof(int) * (depth + 1), hipMemcpyDeviceToHost, st));
   WF_HIP(hipStreamSynchronize(st));
+  int32_t* spans = nullptr;
+  WF_TRY(dalloc_t(h, "lvl_span", (size_t)2 * (depth + 1), &spans));
+  std::vector<int32_t> hspan(2 * (depth + 1));
   for (int l = 0; l <= depth; ++l) {
     h->levels[l].count = hc[l];
+    h->levels[l].span = spans + 2 * l;
+    hspan[2 * l] = 0;
+    hspan[2 * l + 1] = hc[l];
     WF_HIP(wf::octree_level_finish(h->levels[l].begin, hc[l], (int32_t)n, h->levels[l].end, st));
   }
+  WF_HIP(hipMemcpyAsync(spans, hspan.data(), sizeof(int32_t) * hspan.size(), hipMemcpyHostToDevice,
+                        st));
+  WF_HIP(hipStreamSynchronize(st));
   for (int l = 0; l < depth; ++l)
     WF_HIP(wf::octree_links(h->levels[l + 1].begin, hc[l + 1], h->levels[l].begin, hc[l],
                             h->levels[l + 1].parent,
@@ -826,7 +841,8 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
   const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
   const bool mfma_ok = h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
-                       2 * ntc <= wf::kThetaMfmaMaxK && npp % 2 == 0;
+                       wf::gc_row_len(ntc) <= wf::kThetaMfmaMaxK && npc <= wf::kPhiMfmaMaxK &&
+                       npc % 2 == 0 && npp % 2 == 0;
   const bool fft2_ok = h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
                        is_smooth5(npc) && is_smooth5(npp) && is_smooth5(2 * ntc) &&
                        is_smooth5(2 * ntp);
@@ -863,7 +879,7 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   if (mfma_ok) {
     MatEntry mp, mt;
     WF_TRY(get_mat(h, npc, npp, 0.0, &mp));
-    WF_TRY(get_mat(h, 2 * ntc, 2 * ntp, off_tp - off_tc, &mt));
+    WF_TRY(get_mat(h, 2 * ntc, 2 * ntp, off_tp - off_tc, &mt, true));
     WF_HIP(wf::launch_phi_mfma((const float2*)child, G, (int64_t)count * ntc, ntc, npc, npp, mp.hi,
                                mp.lo, mp.kpad, st));
     wf::ThetaMfmaArgs ta;
@@ -973,6 +989,8 @@ struct GroupCtx {
   std::vector<int64_t> toff;
   float2* Tx = nullptr;  // extended tables of the MFMA level transition
   std::vector<int64_t> txoff;
+  int root_from = 1;         // level whose patterns the root evaluates
+  float2* Troot = nullptr;   // per-child table of that transition (root_from > 1)
 };
 
 double theta_origin(int l, int nt) {
@@ -1027,7 +1045,11 @@ int choose_depth(const wf_handle* h, double lam, int NT, int NP) {
   return best;
 }
 
-wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP, GroupCtx& gc) {
+// Grids and translation tables of one wavelength group.  allow_skip: the
+// root may read the first level whose grid is no larger than the output grid
+// directly (forward solves; the adjoint keeps every level).
+wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP, GroupCtx& gc,
+                        bool allow_skip = true) {
   const int depth = h->depth;
   gc.lam = lam;
   gc.k0 = 2.0 * WF_PI / lam;
@@ -1077,6 +1099,41 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
       WF_TRY(timed(h, st, WF_K_TRANS, 8.0 * 8 * (2 * gl.nt + 1) * (gl.np / 2), [&] {
         return wf::launch_trans_ext(Tl, gl.nt, gl.np, mirror, th0, hc, k0, st);
       }));
+    }
+  }
+  gc.root_from = 1;
+  gc.Troot = nullptr;
+  if (allow_skip && h->tune.agg_kernel == 0 && depth >= 2) {
+    int r = 1;
+    while (r < depth && (gc.g[r].nt > NT || gc.g[r].np > NP)) ++r;
+    const Grid gr = gc.g[r];
+    const double off_p = theta_origin(0, NT);
+    const int mirror = mirror_of(NT, off_p);
+    const bool mfma_ok = wf::gc_row_len(gr.nt) <= wf::kThetaMfmaMaxK &&
+                         gr.np <= wf::kPhiMfmaMaxK && gr.np % 2 == 0 && NP % 2 == 0;
+    const bool fft2_ok = is_smooth5(NP) && is_smooth5(2 * NT) && is_smooth5(gr.np) &&
+                         is_smooth5(2 * gr.nt);
+    const int nr = h->levels[r].count;
+    const double tbytes = 8.0 * nr * (2.0 * NT + 1) * (NP / 2);
+    if (r > 1 && (mfma_ok || fft2_ok) && (mirror == 2 * NT - 1 || mirror == 2 * NT) &&
+        tbytes <= 2.0 * 1024 * 1024 * 1024) {
+      double hx[16], hy[16], hz[16];
+      for (int l = 0; l < r; ++l) {
+        hx[l] = 0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1);
+        hy[l] = 0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1);
+        hz[l] = 0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1);
+      }
+      double* coff = nullptr;
+      WF_TRY(dalloc_t(h, "root_offsets", (size_t)3 * nr, &coff));
+      WF_HIP(wf::launch_child_offsets(h->levels[r].key, nr, r, hx, hy, hz, coff, coff + nr,
+                                      coff + 2 * nr, st));
+      WF_TRY(dalloc_t(h, "trans_root", (size_t)nr * (2 * NT + 1) * (NP / 2), &gc.Troot));
+      const double k0 = gc.k0;
+      WF_TRY(timed(h, st, WF_K_TRANS, tbytes, [&] {
+        return wf::launch_trans_child(gc.Troot, NT, NP, mirror, off_p, coff, coff + nr,
+                                      coff + 2 * nr, nr, k0, st);
+      }));
+      gc.root_from = r;
     }
   }
   for (int l = 0; l <= depth && l < 16; ++l) {
@@ -1138,7 +1195,8 @@ wf_status tree_buffers(wf_handle* h, const GroupCtx& gc, int F, int NP, TreeBufs
   WF_TRY(dalloc_t(h, "patA", pat_max, &tb.patA));
   WF_TRY(dalloc_t(h, "patB", pat_max, &tb.patB));
   WF_TRY(dalloc_t(h, "G", g_max, &tb.G));
-  WF_TRY(dalloc_t(h, "G0", g_elems(h->levels[1].count, gc.g[1].nt, NP), &tb.G0));
+  const int r = gc.root_from;
+  WF_TRY(dalloc_t(h, "G0", g_elems(h->levels[r].count, gc.g[r].nt, NP), &tb.G0));
   return WF_OK;
 }
 
@@ -1168,22 +1226,33 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     return wf::launch_leaf_mfma(la, h->opts.precision == WF_PREC_BF16X3, st);
   }));
   h->stats.leaf_pairs += (int64_t)(dleaf * n);
+  // the root reads level root_from directly when the levels above it have
+  // grids larger than the output grid (their transitions would only cost)
+  const int r = gc.root_from;
   for (int l = depth - 1; l >= 0; --l) {
+    if (l > 0 && l < r) continue;
     if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
-    const Grid gcg = gc.g[l + 1], gp = gc.g[l];
+    const int lc = (l == 0) ? r : l + 1;  // child level
+    const bool direct_root = (l == 0 && r > 1);
+    const int32_t* pcb = direct_root ? h->levels[lc].span : h->levels[l].cb;
+    const int32_t* pce = direct_root ? h->levels[lc].span + 1 : h->levels[l].ce;
+    // extended tables exist only for the MFMA / LDS FFT kernels (agg_kernel 0)
+    const float2* Tx_l = direct_root ? gc.Troot
+                                     : (gc.txoff.empty() ? nullptr : gc.Tx + gc.txoff[l]);
+    const Grid gcg = gc.g[lc], gp = gc.g[l];
     FftPlanDev pc, pp, tc, tp;
     WF_TRY(get_fft(h, gcg.np, &pc));
     WF_TRY(get_fft(h, gp.np, &pp));
     WF_TRY(get_fft(h, 2 * gcg.nt, &tc));
     WF_TRY(get_fft(h, 2 * gp.nt, &tp));
-    const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
+    const int nchild = h->levels[lc].count, npar = h->levels[l].count;
     const double planes = (double)nchild * F;
     float2* G = (l == 0) ? tb.G0 : tb.G;
-    const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
+    const double off_c = theta_origin(lc, gcg.nt), off_p = theta_origin(l, gp.nt);
     const int mirror = mirror_of(gp.nt, off_p);
     const float2* Tl = gc.T + gc.toff[l];
-    const bool mfma_ok = h->tune.agg_kernel == 0 && 2 * gcg.nt <= wf::kThetaMfmaMaxK &&
-                         gp.np % 2 == 0;
+    const bool mfma_ok = h->tune.agg_kernel == 0 && wf::gc_row_len(gcg.nt) <= wf::kThetaMfmaMaxK &&
+                         gcg.np <= wf::kPhiMfmaMaxK && gcg.np % 2 == 0 && gp.np % 2 == 0;
     const bool fft2_ok = h->tune.agg_kernel == 0 && is_smooth5(gp.np) && is_smooth5(2 * gp.nt) &&
                          is_smooth5(gcg.np) && is_smooth5(2 * gcg.nt);
     if (fft2_ok && (2 * gp.nt > h->tune.agg_fft_rows || !mfma_ok)) {
@@ -1208,10 +1277,11 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       ta.npp = gp.np;
       ta.ntp = gp.nt;
       ta.nparents = npar;
-      ta.cb = h->levels[l].cb;
-      ta.ce = h->levels[l].ce;
-      ta.child_key = h->levels[l + 1].key;
-      ta.Text = gc.Tx + gc.txoff[l];
+      ta.cb = pcb;
+      ta.ce = pce;
+      ta.child_key = h->levels[lc].key;
+      ta.Text = Tx_l;
+      ta.text_per_child = direct_root ? 1 : 0;
       ta.mirror = mirror;
       ta.pc = f_tc;
       ta.pp = f_tp;
@@ -1229,7 +1299,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       // translation + child sum -> parent (FP64 at the root)
       MatEntry mp, mt;
       WF_TRY(get_mat(h, gcg.np, gp.np, 0.0, &mp));
-      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt));
+      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true));
       const int64_t rows = (int64_t)nchild * F * gcg.nt;
       WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
         return wf::launch_phi_mfma(cur, G, rows, gcg.nt, gcg.np, gp.np, mp.hi, mp.lo, mp.kpad, st);
@@ -1242,10 +1312,11 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       ta.npp = gp.np;
       ta.ntp = gp.nt;
       ta.nparents = npar;
-      ta.cb = h->levels[l].cb;
-      ta.ce = h->levels[l].ce;
-      ta.child_key = h->levels[l + 1].key;
-      ta.Text = gc.Tx + gc.txoff[l];
+      ta.cb = pcb;
+      ta.ce = pce;
+      ta.child_key = h->levels[lc].key;
+      ta.Text = Tx_l;
+      ta.text_per_child = direct_root ? 1 : 0;
       ta.Mhi = mt.hi;
       ta.Mlo = mt.lo;
       ta.kpad = mt.kpad;
@@ -1585,7 +1656,7 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
       gc.lam = inc[order[i0]].lambda_m;
       gc.k0 = 2.0 * WF_PI / gc.lam;
     } else {
-      WF_TRY(prepare_group(h, st, inc[order[i0]].lambda_m, NT, NP, gc));
+      WF_TRY(prepare_group(h, st, inc[order[i0]].lambda_m, NT, NP, gc, false));
     }
     for (int c0 = i0; c0 < i1; c0 += max_solves) {
       const int S = std::min(max_solves, i1 - c0);
@@ -1949,7 +2020,8 @@ wf_status wf_batch_add(wf_batch* b, const wf_surface* s, int32_t depth, int32_t*
   const int64_t n = h->n;
   size_t nodes = 0;
   for (const LevelDev& L : h->levels) nodes += (size_t)L.count;
-  const size_t bytes = sizeof(float) * 7 * n + 2 * sizeof(int32_t) * n + 6 * sizeof(int32_t) * nodes;
+  const size_t bytes = sizeof(float) * 7 * n + 2 * sizeof(int32_t) * n + 6 * sizeof(int32_t) * nodes +
+                       2 * sizeof(int32_t) * h->levels.size();
   WF_HIP(hipMalloc(&g.mem, bytes));
   char* p = static_cast<char*>(g.mem);
   auto take = [&](size_t nb) {
@@ -1980,6 +2052,8 @@ wf_status wf_batch_add(wf_batch* b, const wf_surface* s, int32_t depth, int32_t*
     G.cb = reinterpret_cast<int32_t*>(take(4 * c));
     G.ce = reinterpret_cast<int32_t*>(take(4 * c));
     G.parent = reinterpret_cast<int32_t*>(take(4 * c));
+    G.span = reinterpret_cast<int32_t*>(take(8));
+    cp(G.span, L.span, 8);
     cp(G.key, L.key, 4 * c);
     cp(G.begin, L.begin, 4 * c);
     cp(G.end, L.end, 4 * c);

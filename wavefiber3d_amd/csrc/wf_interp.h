@@ -43,30 +43,40 @@ void build_interp_matrix(int nc, int np, double shift, std::vector<uint16_t>& hi
 int interp_rows_pad(int rows);
 int interp_kpad(int k);
 
-// phi pass: X [rows = nodes*F*ntc][npc] -> GT in great-circle order:
-// GT[nodes*F][npp/2][gc_row_len(ntc)], row k holds column k at theta j
-// (j < ntc), then column k + npp/2 at theta 2 ntc - 1 - j (ntc <= j < 2 ntc),
-// then zeros up to the padded length.
+// phi pass: X [rows = nodes*F*ntc][npc] -> GT, pre-split great-circle rows:
+// for each plane cf = (node, field) and column pair k < npp/2 one row of
+// gc_row_len(ntc) = 2 gc_half_len(ntc) samples, j < gc_half_len: column k at
+// theta j, then column k + npp/2 at theta j - gc_half_len (both ascending),
+// zero where theta >= ntc.  Each group of 8 samples is stored as 4 planes of
+// 8 BF16 (re hi, re lo, im hi, im lo; hi + lo = the FP32 value), i.e.
+// 64 B per group and 8 B per sample.  npc <= kPhiMfmaMaxK, npp even.
 hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
                            const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st);
-// padded great-circle row length: 2 ntc rounded up to a multiple of 8
-inline int gc_row_len(int ntc) { return ((2 * ntc + 7) / 8) * 8; }
-// elements of the phi-pass output for `planes` = nodes*F
+inline int gc_half_len(int ntc) { return ((ntc + 7) / 8) * 8; }
+inline int gc_row_len(int ntc) { return 2 * gc_half_len(ntc); }
+// samples (8 B each) of the phi-pass output for `planes` = nodes*F
 inline int64_t gc_elems(int64_t planes, int ntc, int npp) {
   return planes * (int64_t)(npp / 2) * gc_row_len(ntc);
 }
-// largest 2 ntc the theta MFMA pass accepts (its matrix fragments stay in
-// registers for all K steps)
+// the theta matrix in the same column order: [rows_pad >= 2 ntp][kpad >=
+// gc_row_len(ntc)], column j < gc_half_len -> great-circle sample j, column
+// gc_half_len + t -> sample 2 ntc - 1 - t, zero for padding
+void build_theta_matrix_gc(int ntc, int n2p, double shift, std::vector<uint16_t>& hi,
+                           std::vector<uint16_t>& lo, int& rows_pad, int& kpad);
+// largest npc of the phi MFMA pass and largest gc_row_len(ntc) of the theta
+// MFMA pass (their matrix fragments stay in registers for all K steps)
+constexpr int kPhiMfmaMaxK = 288;
 constexpr int kThetaMfmaMaxK = 512;
 
 struct ThetaMfmaArgs {
-  const float2* GT;  // phi_mfma output: [child][F][npp/2][gc_row_len(ntc)]
+  const float2* GT;  // phi_mfma output (pre-split great-circle rows, see above)
   int F, ntc, npp, ntp, nparents;
   const int32_t* cb;  // per-parent child range (null: parent p <- child p)
   const int32_t* ce;
   const uint32_t* child_key;  // octant = key & 7 (nullable)
   const float2* Text;         // [8][2 ntp + 1][npp/2] (null: no translation)
-  const uint16_t* Mhi;        // theta matrix [rows_pad >= 2 ntp][kpad >= 2 ntc]
+  int text_per_child = 0;     // 1: Text is [child][2 ntp + 1][npp/2] (launch_trans_child)
+  const uint16_t* Mhi;        // build_theta_matrix_gc matrix
   const uint16_t* Mlo;
   int kpad;
   int mirror;  // 2 ntp - 1 (half-sample grid) or 2 ntp (sample at the pole)
@@ -77,6 +87,15 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
 
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
                             const double h[3], double k0, hipStream_t st);
+// per-child extended table for children at offsets (cx, cy, cz)[c] from the
+// parent centre; offsets of level-r nodes from the root centre from their keys
+// (hx/hy/hz[l]: the child-centre offset magnitudes of the level-l transition)
+hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
+                              const double* cx, const double* cy, const double* cz, int nch,
+                              double k0, hipStream_t st);
+hipError_t launch_child_offsets(const uint32_t* keys, int n, int r, const double* hx,
+                                const double* hy, const double* hz, double* cx, double* cy,
+                                double* cz, hipStream_t st);
 
 // FFT level transition for large grids (wf_agg_fft.hip).  fac tables hold
 // w_f exp(i f shift) / nc for f = -(nc/2) .. band top (nc + 1 entries).
@@ -90,6 +109,7 @@ struct ThetaFftArgs {
   const int32_t* ce;
   const uint32_t* child_key;
   const float2* Text;  // [8][2 ntp + 1][npp/2] (nullable)
+  int text_per_child = 0;  // 1: Text is [child][2 ntp + 1][npp/2]
   int mirror;
   Fft2Plan pc;         // length 2 ntc
   Fft2Plan pp;         // length 2 ntp
