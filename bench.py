@@ -75,6 +75,11 @@ def workload(name: str):
                          "(N=200000), cuticle 5um/3deg, noise seed 0, IOR 1.55+0.02i, theta idx 90 "
                          "TM, lambda 0.4um, 180x360; loss = sum (log I - log T)^2, T = I_fp64 "
                          "x exp(0.3 N(0,1)) seed 12345")
+    if name == "heightfield_fiber400um":
+        return dict(heightfield=True,
+                    desc="fiber heightfield 40x30um ellipse, length 400um, 0.1um spacing, "
+                         "seed 0 (2210 arc x 4000 rings = 8.84M samples) -> points, normals, "
+                         "areas (util/geometry.h:78-178)")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -157,6 +162,8 @@ def main():
     print(f"[bench] workload {args.workload}: {W['desc']}", file=sys.stderr, flush=True)
     if W.get("batch"):
         return bench_batch(args, W)
+    if W.get("heightfield"):
+        return bench_heightfield(args, W)
     if W.get("adjoint"):
         return bench_adjoint(args, W)
     surf = W["surface"]()
@@ -353,6 +360,70 @@ def bench_batch(args, W):
         print(f"[bench] accuracy {errs}", file=sys.stderr, flush=True)
     b.close()
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_heightfield(args, W):
+    """Geometry front-end: heightfield -> surface samples on the GPU, inputs and
+    outputs resident in HBM.  One step = one wf_heightfield_points call (it
+    returns after its stream finished, for the zero-vertex check); HIP events
+    on the call's stream bracket each call.  Algorithmic bytes: the heightfield
+    read once (8 B per vertex) + 28 B written per sample."""
+    import torch
+
+    import wavefiber3d_amd as wf
+
+    length, spacing = 400e-6, 0.1e-6
+    hx, hy, nv = wf.geometry.fiber_heightfield(40e-6, 30e-6, length, spacing, seed=0)
+    numel = hx.shape[0]
+    h = np.concatenate([hx.astype(np.float32).ravel(), hy.astype(np.float32).ravel()])
+    unitz = length / nv
+    n = numel * nv
+    hd = torch.from_numpy(h).cuda()
+    for _ in range(max(1, args.warmup)):
+        wf.heightfield_samples(hd, numel, nv, length, unitz)
+    st = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(st)
+        out = wf.heightfield_samples(hd, numel, nv, length, unitz)
+        b.record(st)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    dev_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    bytes_ = 8.0 * numel * nv + 28.0 * n
+    achieved = bytes_ / (dev_ms * 1e-3) / 1e9
+    res = {"metric": "heightfield samples/s (auxiliary workload)", "value": n / (wall_ms * 1e-3),
+           "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall_ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": args.workload, "description": W["desc"], "numel": numel,
+                      "numelvert": nv, "samples": n},
+           "roofline": {"bound": "hbm", "kernel": "heightfield_points (call: flag init, kernel, "
+                        "4 B readback)", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "work_per_launch": bytes_, "avg_launch_ms": dev_ms}}
+    if not args.no_cpu:
+        # CPU leg: the C oracle on the host (timed) and, as the checker, the
+        # GPU result of the same heightfield compared bit for bit
+        from oracle import oracle
+
+        got = [t.cpu().numpy() for t in wf.heightfield_samples(hd, numel, nv, length,
+                                                              float(np.float32(unitz)))]
+        t0 = time.perf_counter()
+        ref, _ = oracle.heightfield_points(h, numel, nv, length, np.float32(unitz))
+        cs = time.perf_counter() - t0
+        res["parity_bit_identical"] = bool(all(np.array_equal(a, b) for a, b in zip(got, ref)))
+        res["cpu_baseline"] = {"value": n / cs, "unit": "samples/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole heightfield ({n} samples), C oracle, {cs:.2f} s"}
+    del out
+    print(f"[bench] heightfield: {wall_ms:.3f} ms/call wall, {dev_ms:.3f} ms device, "
+          f"{achieved:.0f} GB/s, parity {res.get('parity_bit_identical')}", file=sys.stderr,
+          flush=True)
+    print(json.dumps(res), flush=True)
     return 0
 
 

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 2
+#define WF_ABI_VERSION 3
 
 /* depth argument of wf_set_geometry: build the octree to the deepest useful
  * depth (about log8 n) and let each wavelength group of a solve use the
@@ -292,6 +292,30 @@ wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf
  * last wf_batch_solve; arrays of length n_segments, nullable. */
 wf_status wf_batch_info(wf_batch* b, int32_t* n_segments, int32_t* depth, double* ms,
                         int64_t* n_points);
+
+/* Geometry front-end: fiber heightfield -> surface samples
+ * (util/geometry.h:78-178, set_sourcepoint with the arc window [start, end];
+ * the windowless form util/geometry.h:3-72 is start = 0, end = numel - 1).
+ * height: [2][numel][numelvert] float32 -- the x block, then the y block;
+ * vertex (arc j, ring i) at j*numelvert + i.  Output: n = (end - start + 1) *
+ * numelvert samples as 7 float32 arrays [x | y | z | nx | ny | nz | area] of
+ * length n (out_soa holds 7 n floats); sample (j, i) at i*(end - start + 1) +
+ * j - start is the quad (j, i), (j + 1 mod numel, i), (j, i'), (j + 1 mod
+ * numel, i') with i' = min(i + 1, numelvert - 1) at heights z_i = i*unitz -
+ * period/2 and z_i + unitz: centroid, mean of its two triangle areas and the
+ * normalised sum of their area vectors, in the reference's float arithmetic
+ * (no fused multiply-add).  A vertex with x = y = 0 is an error (the
+ * reference prints it and exits); wf_last_error names the first such sample
+ * in (i, j) order.  memspace: height and out_soa both host or both device;
+ * the call returns after the stream has finished the conversion. */
+wf_status wf_heightfield_points(const float* height, int32_t numel, int32_t numelvert, float period,
+                                float unitz, int32_t start, int32_t end, float* out_soa,
+                                int32_t memspace, void* stream);
+
+/* Reads a fiber heightfield file (treePO.cu:79-88: raw float32, 2 * numel *
+ * numelvert values in host byte order) into host memory; a missing or short
+ * file is an error. */
+wf_status wf_read_heightfield(const char* path, int32_t numel, int32_t numelvert, float* out);
 
 #ifdef __cplusplus
 }

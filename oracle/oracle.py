@@ -248,3 +248,23 @@ def sphere_interp(pat, ntp: int, npp: int, off_tp: float):
     lib().orc_sphere_interp(ntc, npc, off_tc, _d(pat.view(np.float64)), ntp, npp, off_tp,
                             _d(out.view(np.float64)))
     return out
+
+
+def heightfield_points(height, numel: int, numelvert: int, period: float, unitz: float,
+                       start: int = 0, end: int | None = None):
+    """orc_heightfield_points (util/geometry.h:78-178): returns (7 float32
+    arrays x, y, z, nx, ny, nz, area, first_bad) with first_bad = -1 or the
+    loop-order index i*numel + j of the first sample with a zero vertex."""
+    if end is None:
+        end = numel - 1
+    h = np.ascontiguousarray(np.asarray(height, dtype=np.float32).reshape(-1))
+    n = (end - start + 1) * numelvert
+    out = [np.zeros(n, dtype=np.float32) for _ in range(7)]
+    L = lib()
+    fn = L.orc_heightfield_points
+    fn.restype = ct.c_long
+    fn.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_float, ct.c_float, ct.c_int, ct.c_int] + \
+        [ct.c_void_p] * 7
+    r = fn(h.ctypes.data_as(ct.c_void_p), numel, numelvert, period, unitz, start, end,
+           *[a.ctypes.data_as(ct.c_void_p) for a in out])
+    return out, (int(r) - 1 if r else -1)

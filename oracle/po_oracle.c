@@ -451,3 +451,62 @@ void orc_sphere_interp(int ntc, int npc, double off_tc, const double* in, int nt
   free(line);
   free(g);
 }
+
+/* ---------------------------------------------------------------------------
+ * Geometry front-end: util/geometry.h:78-178 (set_sourcepoint with the arc
+ * window [start, end]; util/geometry.h:3-72 is start = 0, end = numel - 1).
+ * Float arithmetic as there: Float3 components are float, subtract / add /
+ * crossproduct (util/util.h:339-345) / vecnorm (util.h:306-309) / normalize
+ * (util.h:312-320) work component-wise in float, sums run left to right, the
+ * ring height i*unitz - period*0.5 and the /4.0 and 0.5* factors go through
+ * double; the build disables contraction.  Returns 0, or 1 + i*numel + j of
+ * the first sample in (i, j) loop order with a vertex at x = y = 0 (the
+ * reference prints it and exits). */
+long orc_heightfield_points(const float* h, int numel, int numelvert, float period, float unitz,
+                            int start, int end, float* x, float* y, float* z, float* nx,
+                            float* ny, float* nz, float* area) {
+  const float* hy = h + (size_t)numel * numelvert;
+  const int W = end - start + 1;
+  for (int i = 0; i < numelvert; ++i) {
+    const float zi = (float)(i * unitz - period * 0.5);
+    const float zn = (float)((i + 1) * unitz - period * 0.5);
+    for (int j = 0; j < numel; ++j) {
+      if (j < start || j > end) continue;
+      const int jp = (j < numel - 1) ? j + 1 : 0;
+      const int ip = (i < numelvert - 1) ? i + 1 : i;
+      const float p1[3] = {h[(size_t)j * numelvert + i], hy[(size_t)j * numelvert + i], zi};
+      const float p2[3] = {h[(size_t)jp * numelvert + i], hy[(size_t)jp * numelvert + i], zi};
+      const float p3[3] = {h[(size_t)j * numelvert + ip], hy[(size_t)j * numelvert + ip], zn};
+      const float p4[3] = {h[(size_t)jp * numelvert + ip], hy[(size_t)jp * numelvert + ip], zn};
+      if ((p1[0] == 0 && p1[1] == 0) || (p2[0] == 0 && p2[1] == 0) ||
+          (p3[0] == 0 && p3[1] == 0) || (p4[0] == 0 && p4[1] == 0))
+        return 1 + (long)i * numel + j;
+      const size_t o = (size_t)i * W + (j - start);
+      x[o] = (float)((p1[0] + p2[0] + p3[0] + p4[0]) / 4.0);
+      y[o] = (float)((p1[1] + p2[1] + p3[1] + p4[1]) / 4.0);
+      z[o] = (float)((p1[2] + p2[2] + p3[2] + p4[2]) / 4.0);
+      float u[3], v[3], w[3], q[3], a1[3], a2[3], b[3];
+      for (int c = 0; c < 3; ++c) {
+        u[c] = p2[c] - p1[c];
+        v[c] = p3[c] - p1[c];
+        w[c] = p4[c] - p2[c];
+        q[c] = p3[c] - p2[c];
+      }
+      a1[0] = u[1] * v[2] - v[1] * u[2];
+      a1[1] = u[2] * v[0] - v[2] * u[0];
+      a1[2] = v[1] * u[0] - u[1] * v[0];
+      a2[0] = w[1] * q[2] - q[1] * w[2];
+      a2[1] = w[2] * q[0] - q[2] * w[0];
+      a2[2] = q[1] * w[0] - w[1] * q[0];
+      const float n1 = sqrtf(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+      const float n2 = sqrtf(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+      area[o] = (float)(0.5 * (n1 + n2));
+      for (int c = 0; c < 3; ++c) b[c] = a1[c] + a2[c];
+      const float nb = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      nx[o] = b[0] / nb;
+      ny[o] = b[1] / nb;
+      nz[o] = b[2] / nb;
+    }
+  }
+  return 0;
+}

@@ -101,6 +101,11 @@ void orc_trig_interp(int nc, const double* in, double off_c, int np, double off_
  * (ntc x npc, npc even) -> values at the parent grid theta_j = off_tp + pi j/ntp,
  * phi_k = 2 pi k/npp.  Brute force via orc_trig_interp along phi, then along the
  * great circle formed by columns phi and phi + pi. */
+/* util/geometry.h:78-178 heightfield -> samples (float arithmetic, no FMA);
+ * 0 or 1 + i*numel + j of the first sample with a zero vertex. */
+long orc_heightfield_points(const float* h, int numel, int numelvert, float period, float unitz,
+                            int start, int end, float* x, float* y, float* z, float* nx,
+                            float* ny, float* nz, float* area);
 void orc_sphere_interp(int ntc, int npc, double off_tc, const double* in,
                        int ntp, int npp, double off_tp, double* out);
 

@@ -5,7 +5,14 @@ intensity / complex E grids out, computed by hand-written HIP kernels for
 gfx950 behind the C-ABI in include/wf_po.h (libwfpo.so).
 """
 from ._lib import WfError, lib  # noqa: F401
-from .geometry import SurfaceSamples, fibonacci_sphere, fiber_segment, read_soa_mesh  # noqa: F401
+from .geometry import (  # noqa: F401
+    SurfaceSamples,
+    fibonacci_sphere,
+    fiber_segment,
+    heightfield_samples,
+    read_heightfield,
+    read_soa_mesh,
+)
 from .po import (  # noqa: F401
     Batch,
     Incidence,
@@ -18,6 +25,7 @@ from .po import (  # noqa: F401
 
 __all__ = [
     "WfError", "lib", "SurfaceSamples", "fibonacci_sphere", "fiber_segment", "read_soa_mesh",
+    "heightfield_samples", "read_heightfield",
     "Batch", "Incidence", "POSolver", "output_directions", "reference_incidence_phi",
     "reference_incidence_theta", "reference_lambda",
 ]

@@ -30,7 +30,8 @@ EXPORTS = (
     "wf_set_geometry", "wf_solve", "wf_po_solve", "wf_po_solve_fp64_direct", "wf_currents",
     "wf_octree_nodes", "wf_interp", "wf_get_stats", "wf_po_vjp", "wf_geometry_info",
     "wf_get_tune", "wf_set_tune", "wf_autotune", "wf_batch_create", "wf_batch_destroy",
-    "wf_batch_add", "wf_batch_solve", "wf_batch_info",
+    "wf_batch_add", "wf_batch_solve", "wf_batch_info", "wf_heightfield_points",
+    "wf_read_heightfield",
 )
 WF_DEPTH_AUTO = -1
 
@@ -163,6 +164,9 @@ def lib(path: str | None = None):
     L.wf_batch_add.argtypes = [vp, ct.POINTER(Surface), ct.c_int32, ct.POINTER(ct.c_int32)]
     L.wf_batch_solve.argtypes = [vp, ct.POINTER(Incidence), ct.c_int32, Grid, vp, ct.c_int32, vp]
     L.wf_batch_info.argtypes = [vp, ct.POINTER(ct.c_int32), vp, vp, vp]
+    L.wf_heightfield_points.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_float, ct.c_float,
+                                        ct.c_int32, ct.c_int32, vp, ct.c_int32, vp]
+    L.wf_read_heightfield.argtypes = [ct.c_char_p, ct.c_int32, ct.c_int32, vp]
     for name in EXPORTS:
         if name not in ("wf_abi_version", "wf_last_error", "wf_default_opts", "wf_destroy",
                         "wf_batch_destroy"):

@@ -2170,6 +2170,71 @@ wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf
   return WF_OK;
 }
 
+wf_status wf_heightfield_points(const float* height, int32_t numel, int32_t numelvert, float period,
+                                float unitz, int32_t start, int32_t end, float* out_soa,
+                                int32_t memspace, void* stream) {
+  if (!height || !out_soa) return set_err(WF_ERR_INVALID_ARG, "height or out_soa is null");
+  if (numel < 2 || numelvert < 1)
+    return set_err(WF_ERR_INVALID_ARG, "heightfield needs numel >= 2 and numelvert >= 1");
+  if (start < 0 || end < start || end >= numel)
+    return set_err(WF_ERR_INVALID_ARG, "arc window [%d, %d] outside [0, %d)", start, end, numel);
+  if ((int64_t)numel * numelvert >= (1LL << 31) - 1)
+    return set_err(WF_ERR_INVALID_ARG, "heightfield too large");
+  if (!(std::isfinite(period) && std::isfinite(unitz)))
+    return set_err(WF_ERR_INVALID_ARG, "period and unitz must be finite");
+  if (memspace != WF_MEM_HOST && memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "memspace unknown");
+  hipStream_t st = as_stream(stream);
+  const int64_t nh = 2LL * numel * numelvert;
+  const int64_t n = (int64_t)(end - start + 1) * numelvert;
+  // temporaries are stream-ordered (no device-wide synchronisation)
+  float* dh = const_cast<float*>(height);
+  float* dout = out_soa;
+  int* bad = nullptr;
+  hipError_t e = hipSuccess;
+  if (memspace == WF_MEM_HOST) {
+    dh = nullptr;
+    dout = nullptr;
+    e = hipMallocAsync(reinterpret_cast<void**>(&dh), sizeof(float) * nh, st);
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&dout), sizeof(float) * 7 * n, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dh, height, sizeof(float) * nh, hipMemcpyHostToDevice, st);
+  }
+  if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&bad), sizeof(int), st);
+  int hbad = 0x7fffffff;
+  if (e == hipSuccess)
+    e = wf::launch_heightfield_points(dh, numel, numelvert, period, unitz, start, end, dout, bad,
+                                      st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && memspace == WF_MEM_HOST)
+    e = hipMemcpyAsync(out_soa, dout, sizeof(float) * 7 * n, hipMemcpyDeviceToHost, st);
+  if (bad) (void)hipFreeAsync(bad, st);
+  if (memspace == WF_MEM_HOST) {
+    if (dh) (void)hipFreeAsync(dh, st);
+    if (dout) (void)hipFreeAsync(dout, st);
+  }
+  const hipError_t es = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = es;
+  if (e != hipSuccess) return set_err(WF_ERR_HIP, "wf_heightfield_points: %s", hipGetErrorString(e));
+  if (hbad != 0x7fffffff)
+    return set_err(WF_ERR_INVALID_ARG, "heightfield vertex at x = y = 0 (sample j = %d, i = %d)",
+                   hbad % numel, hbad / numel);
+  return WF_OK;
+}
+
+wf_status wf_read_heightfield(const char* path, int32_t numel, int32_t numelvert, float* out) {
+  if (!path || !out) return set_err(WF_ERR_INVALID_ARG, "path or out is null");
+  if (numel < 1 || numelvert < 1) return set_err(WF_ERR_INVALID_ARG, "numel, numelvert >= 1");
+  FILE* f = fopen(path, "rb");
+  if (!f) return set_err(WF_ERR_INVALID_ARG, "cannot open %s", path);
+  const size_t want = 2 * (size_t)numel * (size_t)numelvert;
+  const size_t got = fread(out, sizeof(float), want, f);
+  fclose(f);
+  if (got != want)
+    return set_err(WF_ERR_INVALID_ARG, "%s holds %zu of %zu float32 values", path, got, want);
+  return WF_OK;
+}
+
 wf_status wf_batch_info(wf_batch* b, int32_t* n_segments, int32_t* depth, double* ms,
                         int64_t* n_points) {
   if (!b) return set_err(WF_ERR_INVALID_ARG, "batch is null");

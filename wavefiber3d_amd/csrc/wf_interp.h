@@ -90,6 +90,11 @@ hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double t
 // per-child extended table for children at offsets (cx, cy, cz)[c] from the
 // parent centre; offsets of level-r nodes from the root centre from their keys
 // (hx/hy/hz[l]: the child-centre offset magnitudes of the level-l transition)
+// heightfield -> 7 SoA arrays (wf_geometry.hip); *first_bad receives the
+// smallest i * numel + j of a sample with a zero vertex (INT_MAX if none)
+hipError_t launch_heightfield_points(const float* height, int numel, int numelvert, float period,
+                                     float unitz, int start, int end, float* out, int* first_bad,
+                                     hipStream_t st);
 hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
                               const double* cx, const double* cy, const double* cz, int nch,
                               double k0, hipStream_t st);

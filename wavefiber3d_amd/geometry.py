@@ -15,6 +15,7 @@ areas (compareMie/sphere.cu:92-125).  These helpers produce such samples:
 """
 from __future__ import annotations
 
+import ctypes as ct
 import math
 from dataclasses import dataclass
 
@@ -106,6 +107,59 @@ def quad_samples_from_heightfield(hx: np.ndarray, hy: np.ndarray, period: float,
     nv = nv.reshape(-1, 3)
     return SurfaceSamples(*_f32(c[:, 0], c[:, 1], c[:, 2], nv[:, 0], nv[:, 1], nv[:, 2],
                                 area.reshape(-1)))
+
+
+def read_heightfield(path: str, numel: int, numelvert: int) -> np.ndarray:
+    """Fiber heightfield file (treePO.cu:79-88: raw float32, x block then y
+    block, vertex (j, i) at j*numelvert + i) through wf_read_heightfield;
+    returns float32 [2 * numel * numelvert]."""
+    from ._lib import check, lib
+
+    out = np.empty(2 * numel * numelvert, dtype=np.float32)
+    check(lib().wf_read_heightfield(str(path).encode(), numel, numelvert,
+                                    out.ctypes.data_as(ct.c_void_p)))
+    return out
+
+
+def heightfield_samples(height, numel: int, numelvert: int, period: float,
+                        unitz: float | None = None, start: int = 0, end: int | None = None,
+                        stream=None):
+    """Surface samples of a fiber heightfield on the GPU (wf_heightfield_points;
+    util/geometry.h:78-178 set_sourcepoint, arc window [start, end]).
+
+    height: [2 * numel * numelvert] float32 in the file layout (x block, then
+    y block), a numpy array (host path; returns SurfaceSamples) or a CUDA
+    tensor (device path; returns the 7 float32 CUDA tensors x, y, z, nx, ny,
+    nz, area).  unitz defaults to period / numelvert (treePO.cu:59).  A vertex
+    at x = y = 0 raises WfError naming the first sample (the reference exits).
+    """
+    from ._lib import WF_MEM_DEVICE, WF_MEM_HOST, check, lib
+
+    if end is None:
+        end = numel - 1
+    if unitz is None:
+        unitz = period / numelvert
+    n = (end - start + 1) * numelvert
+    if hasattr(height, "is_cuda") and height.is_cuda:
+        import torch
+
+        h = height.detach().to(torch.float32).contiguous()
+        if h.numel() != 2 * numel * numelvert:
+            raise ValueError("height must hold 2 * numel * numelvert values")
+        out = torch.empty((7, n), dtype=torch.float32, device=h.device)
+        st = stream if stream is not None else torch.cuda.current_stream(h.device).cuda_stream
+        check(lib().wf_heightfield_points(ct.c_void_p(h.data_ptr()), numel, numelvert, period,
+                                          unitz, start, end, ct.c_void_p(out.data_ptr()),
+                                          WF_MEM_DEVICE, ct.c_void_p(st)))
+        return [out[c] for c in range(7)]
+    h = np.ascontiguousarray(np.asarray(height, dtype=np.float32).reshape(-1))
+    if h.size != 2 * numel * numelvert:
+        raise ValueError("height must hold 2 * numel * numelvert values")
+    out = np.empty((7, n), dtype=np.float32)
+    check(lib().wf_heightfield_points(h.ctypes.data_as(ct.c_void_p), numel, numelvert, period,
+                                      unitz, start, end, out.ctypes.data_as(ct.c_void_p),
+                                      WF_MEM_HOST, None))
+    return SurfaceSamples(*[out[c] for c in range(7)])
 
 
 def fiber_heightfield(r1: float, r2: float, length: float, spacing: float,
