@@ -80,6 +80,10 @@ def workload(name: str):
                     desc="fiber heightfield 40x30um ellipse, length 400um, 0.1um spacing, "
                          "seed 0 (2210 arc x 4000 rings = 8.84M samples) -> points, normals, "
                          "areas (util/geometry.h:78-178)")
+    if name == "average_25x7200x1440":
+        return dict(average=True,
+                    desc="block average of 25 intensity grids 7200x1440 -> 180x360 "
+                         "(util/fields.h:206-234)")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -164,6 +168,8 @@ def main():
         return bench_batch(args, W)
     if W.get("heightfield"):
         return bench_heightfield(args, W)
+    if W.get("average"):
+        return bench_average(args, W)
     if W.get("adjoint"):
         return bench_adjoint(args, W)
     surf = W["surface"]()
@@ -423,6 +429,56 @@ def bench_heightfield(args, W):
     print(f"[bench] heightfield: {wall_ms:.3f} ms/call wall, {dev_ms:.3f} ms device, "
           f"{achieved:.0f} GB/s, parity {res.get('parity_bit_identical')}", file=sys.stderr,
           flush=True)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_average(args, W):
+    """Output-grid block average on the GPU (inputs resident in HBM, device
+    path, asynchronous): HIP events on the launch stream around each call.
+    Algorithmic bytes: every input sample read once + every output written."""
+    import torch
+
+    import wavefiber3d_amd as wf
+
+    g = torch.rand((25, 7200, 1440), dtype=torch.float32, device="cuda")
+    for _ in range(max(1, args.warmup)):
+        wf.average_grid(g, 180, 360)
+    st = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(st)
+        out = wf.average_grid(g, 180, 360)
+        b.record(st)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    dev_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    bytes_ = 4.0 * g.numel() + 4.0 * out.numel()
+    achieved = bytes_ / (dev_ms * 1e-3) / 1e9
+    res = {"metric": "grid average input samples/s (auxiliary workload)",
+           "value": g.numel() / (wall_ms * 1e-3), "unit": "samples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "config": {"workload": args.workload, "description": W["desc"]},
+           "roofline": {"bound": "hbm", "kernel": "average_grid_kernel", "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": None, "work_per_launch": bytes_, "avg_launch_ms": dev_ms}}
+    if not args.no_cpu:
+        from oracle import oracle
+
+        got = out[0].cpu().numpy()
+        x = g[0].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.average_grid(x, 180, 360)
+        cs = time.perf_counter() - t0
+        res["parity_bit_identical"] = bool(np.array_equal(got, ref))
+        res["cpu_baseline"] = {"value": x.size / cs, "unit": "samples/s", "cores": 1, "kind": "port",
+                               "sample": f"one 7200x1440 grid, C oracle, {cs:.3f} s"}
+    print(f"[bench] average: {dev_ms:.3f} ms/call device, {achieved:.0f} GB/s, parity "
+          f"{res.get('parity_bit_identical')}", file=sys.stderr, flush=True)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -312,6 +312,18 @@ wf_status wf_heightfield_points(const float* height, int32_t numel, int32_t nume
                                 float unitz, int32_t start, int32_t end, float* out_soa,
                                 int32_t memspace, void* stream);
 
+/* Block average of output grids (util/fields.h:206-234, average_cpu): in
+ * [n_grids][n_theta][n_phi] float32 -> out [n_grids][n_theta_out][n_phi_out]
+ * with rt = n_theta / n_theta_out and rp = n_phi / n_phi_out (integer
+ * quotients; rows and columns beyond rt * n_theta_out, rp * n_phi_out are
+ * ignored, as there), averaging along phi first and then along theta, each
+ * term divided before it is added, in float -- bit-identical to the
+ * reference's loop.  memspace: in and out both host (returns with the result)
+ * or both device (asynchronous on stream). */
+wf_status wf_average_grid(const float* in, int32_t n_grids, int32_t n_theta, int32_t n_phi,
+                          int32_t n_theta_out, int32_t n_phi_out, float* out, int32_t memspace,
+                          void* stream);
+
 /* Reads a fiber heightfield file (treePO.cu:79-88: raw float32, 2 * numel *
  * numelvert values in host byte order) into host memory; a missing or short
  * file is an error. */

@@ -268,3 +268,15 @@ def heightfield_points(height, numel: int, numelvert: int, period: float, unitz:
     r = fn(h.ctypes.data_as(ct.c_void_p), numel, numelvert, period, unitz, start, end,
            *[a.ctypes.data_as(ct.c_void_p) for a in out])
     return out, (int(r) - 1 if r else -1)
+
+
+def average_grid(grid, nto: int, npo: int) -> np.ndarray:
+    """orc_average_grid (util/fields.h:206-234) of one [nt][np] float32 grid."""
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.float32))
+    nt, npp = g.shape
+    out = np.empty((nto, npo), dtype=np.float32)
+    fn = lib().orc_average_grid
+    fn.restype = None
+    fn.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
+    fn(g.ctypes.data_as(ct.c_void_p), nt, npp, nto, npo, out.ctypes.data_as(ct.c_void_p))
+    return out

@@ -510,3 +510,20 @@ long orc_heightfield_points(const float* h, int numel, int numelvert, float peri
   }
   return 0;
 }
+
+/* util/fields.h:206-234 (average_cpu): phi pass then theta pass, each term
+ * divided by the (integer) ratio before it is added, in float. */
+void orc_average_grid(const float* in, int nt, int np, int nto, int npo, float* out) {
+  const int rp = np / npo, rt = nt / nto;
+  float* tmp = (float*)calloc((size_t)nt * npo, sizeof(float));
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < npo; ++j)
+      for (int k = 0; k < rp; ++k) tmp[(size_t)i * npo + j] += in[(size_t)i * np + j * rp + k] / (float)rp;
+  for (int i = 0; i < nto; ++i)
+    for (int j = 0; j < npo; ++j) {
+      out[(size_t)i * npo + j] = 0.f;
+      for (int k = 0; k < rt; ++k)
+        out[(size_t)i * npo + j] += tmp[(size_t)(i * rt + k) * npo + j] / (float)rt;
+    }
+  free(tmp);
+}

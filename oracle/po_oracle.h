@@ -106,6 +106,8 @@ void orc_trig_interp(int nc, const double* in, double off_c, int np, double off_
 long orc_heightfield_points(const float* h, int numel, int numelvert, float period, float unitz,
                             int start, int end, float* x, float* y, float* z, float* nx,
                             float* ny, float* nz, float* area);
+/* util/fields.h:206-234 average_cpu on one grid [nt][np] -> [nto][npo]. */
+void orc_average_grid(const float* in, int nt, int np, int nto, int npo, float* out);
 void orc_sphere_interp(int ntc, int npc, double off_tc, const double* in,
                        int ntp, int npp, double off_tp, double* out);
 

@@ -17,6 +17,7 @@ from .po import (  # noqa: F401
     Batch,
     Incidence,
     POSolver,
+    average_grid,
     output_directions,
     reference_incidence_phi,
     reference_incidence_theta,
@@ -25,7 +26,7 @@ from .po import (  # noqa: F401
 
 __all__ = [
     "WfError", "lib", "SurfaceSamples", "fibonacci_sphere", "fiber_segment", "read_soa_mesh",
-    "heightfield_samples", "read_heightfield",
+    "heightfield_samples", "read_heightfield", "average_grid",
     "Batch", "Incidence", "POSolver", "output_directions", "reference_incidence_phi",
     "reference_incidence_theta", "reference_lambda",
 ]

@@ -33,9 +33,10 @@ FLAGS = [
 ]
 
 
-# per-file flags appended after FLAGS: the geometry front-end reproduces the
-# reference's float arithmetic bit for bit, so nothing may be fused there
-FILE_FLAGS = {"wf_geometry.hip": ["-ffp-contract=off"]}
+# per-file flags appended after FLAGS: the geometry front-end and the grid
+# average reproduce the reference's float arithmetic bit for bit, so nothing
+# may be fused there
+FILE_FLAGS = {"wf_geometry.hip": ["-ffp-contract=off"], "wf_grid.hip": ["-ffp-contract=off"]}
 
 
 def hipcc() -> str:

@@ -1,0 +1,137 @@
+"""Reference-compatible command-line driver (row F2).
+
+``python -m wavefiber3d_amd.cli treepo LSTART LEND THETAI PHII FIBER DEPTH START END``
+takes the positional arguments of fibersim/treePO.cu:27-34 with the same
+meaning -- wavelength indices [LSTART, LEND), incident theta and phi indices,
+fiber index into the built-in tables (treePO.cu:45-52), tree depth, and the
+arc window [START, END] of each ring -- reads ``mesh/fiber<FIBER>.binary``
+(treePO.cu:79-88), converts the heightfield on the GPU, and writes, per
+wavelength, ``output/fiber_<FIBER>_lambda<l>_TM_depth<DEPTH>.binary``: the
+float32 intensity grid [thetao][phio] of the TM incidence, as
+treePO.cu:955-957.  The incidence, wavelength and IOR follow treePO.cu:501-604
+(lambda = 400 + 300/25 l nm, thetai = -pi/2 + 1e-10 + pi/180 t, phii =
+2 pi/360 p, all rounded to float32 as there; IOR 1.55; Gaussian beam of the
+fiber's sigma; D = 1 m) and the octree box is +-1.1 (r1, r2, period/2)
+(treePO.cu:94-99).  The tree's own sampling rule and kernels are this
+library's (the output grid and file layout are the reference's).
+
+Options: ``--mesh-dir`` and ``--output-dir`` (defaults ``mesh``, ``output``),
+``--final`` also writes ``..._final.binary``, the 180 x 360 block average of
+util/fields.h:206-234 (average_cpu) that the reference allocates for each
+incidence; ``--fiber-table FILE`` replaces the built-in table with a JSON list
+of {period, r1, r2, sigma, numel, numelvert, thetao, phio}; ``--device``.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+# treePO.cu:45-52
+FIBER_TABLE = [
+    dict(period=20e-6, r1=20e-6, r2=20e-6, sigma=10e-6, numel=3000, numelvert=478, thetao=450,
+         phio=880),
+    dict(period=50e-6, r1=40e-6, r2=40e-6, sigma=10e-6, numel=5026, numelvert=1000, thetao=900,
+         phio=1440),
+    dict(period=400e-6, r1=40e-6, r2=35e-6, sigma=150e-6, numel=4420, numelvert=8000, thetao=7200,
+         phio=1440),
+    dict(period=400e-6, r1=40e-6, r2=35e-6, sigma=150e-6, numel=4720, numelvert=8000, thetao=7200,
+         phio=1440),
+]
+
+
+def f32(x: float) -> float:
+    return float(np.float32(x))
+
+
+def wavelength(index: int) -> float:
+    """treePO.cu:504-505: float arithmetic, then *= 1e-9 through double."""
+    w = np.float32(np.float32(400.0) + np.float32(300.0) / np.float32(25.0) * np.float32(index))
+    return f32(float(w) * 1e-9)
+
+
+def incidence_angles(theta_index: int, phi_index: int) -> tuple[float, float]:
+    """treePO.cu:560, 564 (computed in double, stored float)."""
+    import math
+
+    return (f32(-math.pi / 2 + 1e-10 + math.pi / 180 * theta_index),
+            f32(2 * math.pi / 360 * phi_index))
+
+
+def output_name(outdir: str, fiber: int, lam: int, mode: str, depth: int, final: bool = False) -> str:
+    """treePO.cu:955."""
+    tail = "_final" if final else ""
+    return os.path.join(outdir, f"fiber_{fiber}_lambda{lam}_T{mode}_depth{depth}{tail}.binary")
+
+
+def treepo(argv) -> int:
+    ap = argparse.ArgumentParser(prog="python -m wavefiber3d_amd.cli treepo",
+                                 description=__doc__.split("\n\n")[0])
+    for a in ("lambdaindexstart", "lambdaindexend", "thetaindexstart", "phiiindexstart",
+              "fiberindex", "max_depth", "startindex", "endindex"):
+        ap.add_argument(a, type=int)
+    ap.add_argument("--mesh-dir", default="mesh")
+    ap.add_argument("--output-dir", default="output")
+    ap.add_argument("--final", action="store_true")
+    ap.add_argument("--fiber-table", default=None)
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    table = FIBER_TABLE
+    if a.fiber_table:
+        with open(a.fiber_table) as fh:
+            table = json.load(fh)
+    if not 0 <= a.fiberindex < len(table):
+        print(f"fiber index {a.fiberindex} outside the table (0..{len(table) - 1})", file=sys.stderr)
+        return 2
+    p = table[a.fiberindex]
+
+    import torch
+
+    import wavefiber3d_amd as wf
+
+    numel, numelvert = int(p["numel"]), int(p["numelvert"])
+    period = f32(p["period"])
+    unitz = f32(period / numelvert)
+    height = wf.read_heightfield(os.path.join(a.mesh_dir, f"fiber{a.fiberindex}.binary"), numel,
+                                 numelvert)
+    torch.cuda.set_device(a.device)
+    surf = wf.heightfield_samples(torch.from_numpy(height).cuda(), numel, numelvert, period, unitz,
+                                  a.startindex, a.endindex)
+    # treePO.cu:94-97: float enlargeratio = 1.1; r * enlargeratio in float,
+    # period * 0.5 * enlargeratio through double
+    enl = np.float32(1.1)
+    xr = float(np.float32(p["r1"]) * enl)
+    yr = float(np.float32(p["r2"]) * enl)
+    zr = f32(period * 0.5 * float(enl))
+    os.makedirs(a.output_dir, exist_ok=True)
+    thetai, phii = incidence_angles(a.thetaindexstart, a.phiiindexstart)
+    grid = (int(p["thetao"]), int(p["phio"]))
+    with wf.POSolver(device=a.device, box=(-xr, -yr, -zr, xr, yr, zr)) as solver:
+        solver.set_geometry(surf, a.max_depth)
+        for lam in range(a.lambdaindexstart, a.lambdaindexend):
+            inc = wf.Incidence(thetai, phii, wavelength(lam), "M", 1.55 + 0j, f32(p["sigma"]))
+            I = solver.solve([inc], grid, out="torch")[0]
+            I.cpu().numpy().astype(np.float32).tofile(
+                output_name(a.output_dir, a.fiberindex, lam, "M", a.max_depth))
+            if a.final:
+                wf.average_grid(I, 180, 360).cpu().numpy().tofile(
+                    output_name(a.output_dir, a.fiberindex, lam, "M", a.max_depth, final=True))
+            print(f"lambdaindex {lam} done", flush=True)
+    return 0
+
+
+def main(argv=None) -> int:
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if not argv or argv[0] not in ("treepo",):
+        print("usage: python -m wavefiber3d_amd.cli treepo LSTART LEND THETAI PHII FIBER DEPTH "
+              "START END [--mesh-dir D] [--output-dir D] [--final] [--fiber-table F] [--device N]",
+              file=sys.stderr)
+        return 2
+    return treepo(argv[1:])
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -2222,6 +2222,45 @@ wf_status wf_heightfield_points(const float* height, int32_t numel, int32_t nume
   return WF_OK;
 }
 
+wf_status wf_average_grid(const float* in, int32_t n_grids, int32_t n_theta, int32_t n_phi,
+                          int32_t n_theta_out, int32_t n_phi_out, float* out, int32_t memspace,
+                          void* stream) {
+  if (!in || !out) return set_err(WF_ERR_INVALID_ARG, "in or out is null");
+  if (n_grids < 1 || n_theta_out < 1 || n_phi_out < 1 || n_theta < n_theta_out ||
+      n_phi < n_phi_out)
+    return set_err(WF_ERR_INVALID_ARG, "average grid %dx%d -> %dx%d: output must not be finer",
+                   n_theta, n_phi, n_theta_out, n_phi_out);
+  if (memspace != WF_MEM_HOST && memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "memspace unknown");
+  hipStream_t st = as_stream(stream);
+  const int64_t nin = (int64_t)n_grids * n_theta * n_phi;
+  const int64_t nout = (int64_t)n_grids * n_theta_out * n_phi_out;
+  const float* din = in;
+  float* dout = out;
+  hipError_t e = hipSuccess;
+  float* tin = nullptr;
+  float* tout = nullptr;
+  if (memspace == WF_MEM_HOST) {
+    e = hipMallocAsync(reinterpret_cast<void**>(&tin), sizeof(float) * nin, st);
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&tout), sizeof(float) * nout, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(tin, in, sizeof(float) * nin, hipMemcpyHostToDevice, st);
+    din = tin;
+    dout = tout;
+  }
+  if (e == hipSuccess)
+    e = wf::launch_average_grid(din, n_grids, n_theta, n_phi, n_theta_out, n_phi_out, dout, st);
+  if (memspace == WF_MEM_HOST) {
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out, tout, sizeof(float) * nout, hipMemcpyDeviceToHost, st);
+    if (tin) (void)hipFreeAsync(tin, st);
+    if (tout) (void)hipFreeAsync(tout, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+  }
+  if (e != hipSuccess) return set_err(WF_ERR_HIP, "wf_average_grid: %s", hipGetErrorString(e));
+  return WF_OK;
+}
+
 wf_status wf_read_heightfield(const char* path, int32_t numel, int32_t numelvert, float* out) {
   if (!path || !out) return set_err(WF_ERR_INVALID_ARG, "path or out is null");
   if (numel < 1 || numelvert < 1) return set_err(WF_ERR_INVALID_ARG, "numel, numelvert >= 1");

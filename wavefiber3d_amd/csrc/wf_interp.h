@@ -95,6 +95,9 @@ hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double t
 hipError_t launch_heightfield_points(const float* height, int numel, int numelvert, float period,
                                      float unitz, int start, int end, float* out, int* first_bad,
                                      hipStream_t st);
+// block average of [ngrids][nt][np] onto [ngrids][nto][npo] (wf_grid.hip)
+hipError_t launch_average_grid(const float* in, int ngrids, int nt, int np, int nto, int npo,
+                               float* out, hipStream_t st);
 hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
                               const double* cx, const double* cy, const double* cz, int nch,
                               double k0, hipStream_t st);

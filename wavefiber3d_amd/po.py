@@ -68,6 +68,34 @@ def _incs(incs) -> tuple:
     return arr, len(incs)
 
 
+def average_grid(I, n_theta_out: int = 180, n_phi_out: int = 360, stream=None):
+    """Block average of intensity grids (wf_average_grid; util/fields.h:206-234
+    average_cpu): I [..., n_theta, n_phi] float32, numpy (host path) or CUDA
+    tensor (device path, asynchronous on the current stream) ->
+    [..., n_theta_out, n_phi_out] with integer ratios n_theta // n_theta_out,
+    n_phi // n_phi_out, bit-identical to the reference's float loop."""
+    if hasattr(I, "is_cuda") and I.is_cuda:
+        import torch
+
+        x = I.detach().to(torch.float32).contiguous()
+        nt, npp = x.shape[-2], x.shape[-1]
+        ng = x.numel() // (nt * npp)
+        out = torch.empty(x.shape[:-2] + (n_theta_out, n_phi_out), dtype=torch.float32,
+                          device=x.device)
+        st = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+        check(lib().wf_average_grid(ct.c_void_p(x.data_ptr()), ng, nt, npp, n_theta_out,
+                                    n_phi_out, ct.c_void_p(out.data_ptr()), WF_MEM_DEVICE,
+                                    ct.c_void_p(st)))
+        return out
+    x = np.ascontiguousarray(np.asarray(I, dtype=np.float32))
+    nt, npp = x.shape[-2], x.shape[-1]
+    ng = x.size // (nt * npp)
+    out = np.empty(x.shape[:-2] + (n_theta_out, n_phi_out), dtype=np.float32)
+    check(lib().wf_average_grid(x.ctypes.data_as(ct.c_void_p), ng, nt, npp, n_theta_out, n_phi_out,
+                                out.ctypes.data_as(ct.c_void_p), WF_MEM_HOST, None))
+    return out
+
+
 def output_directions(ntheta: int, nphi: int) -> np.ndarray:
     """Output grid directions (treePO.cu:378-391), float64 [ntheta*nphi, 3]."""
     t = -np.pi / 2 + 1e-10 + np.pi / ntheta * np.arange(ntheta)
