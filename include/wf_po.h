@@ -293,6 +293,17 @@ wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf
 wf_status wf_batch_info(wf_batch* b, int32_t* n_segments, int32_t* depth, double* ms,
                         int64_t* n_points);
 
+/* Finite-distance validator (compareBEM3d/ellipsoid_simple.cu:229-261 and
+ * :384-589): the FP64 direct sum with the exact Green's function,
+ * E = -i w mu0/(4 pi) sum_n J_n exp(-i k |x_n - ob|)/|x_n - ob| (H likewise)
+ * at observation points ob = distance_m * s of the grid directions s about
+ * the origin, then the far-field projection and intensity of the other
+ * solves (the reference's validator uses distance 1 m).  Outputs as
+ * wf_po_solve_fp64_direct. */
+wf_status wf_po_solve_fp64_finite(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
+                                  int32_t n_inc, wf_grid grid, double distance_m,
+                                  double* intensity, double* E_cplx, void* stream);
+
 /* Geometry front-end: fiber heightfield -> surface samples
  * (util/geometry.h:78-178, set_sourcepoint with the arc window [start, end];
  * the windowless form util/geometry.h:3-72 is start = 0, end = numel - 1).

@@ -72,6 +72,12 @@ def lib():
     L.orc_solve_direct_dirs.argtypes = [ct.c_int64] + [fp] * 7 + [ct.POINTER(Incidence), ct.c_int,
                                                                   ct.c_int64, dp, dp, dp, dp, ct.c_int]
     L.orc_solve_direct_dirs.restype = ct.c_int
+    L.orc_direct_sum_finite.argtypes = [ct.c_int64, fp, fp, fp, ct.c_double, ct.c_int, dp,
+                                        ct.c_int64, dp, ct.c_double, dp, ct.c_int]
+    L.orc_solve_finite_dirs.argtypes = [ct.c_int64] + [fp] * 7 + [ct.POINTER(Incidence), ct.c_int,
+                                                                  ct.c_int64, dp, ct.c_double, dp,
+                                                                  dp, ct.c_int]
+    L.orc_solve_finite_dirs.restype = ct.c_int
     L.orc_octree.argtypes = [ct.c_int64, fp, fp, fp, fp, ct.c_int, ct.POINTER(ct.c_int64),
                              ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), fp]
     L.orc_octree.restype = ct.c_int
@@ -163,6 +169,39 @@ def direct_sum(xyz, C, k: float, dirs, centre=(0.0, 0.0, 0.0), nthreads: int = 0
     lib().orc_direct_sum(n, _f(x), _f(y), _f(z), _d(cen), k, S, _d(C.view(np.float64)),
                          dirs.shape[0], _d(dirs), _d(F.view(np.float64)), nthreads)
     return F
+
+
+def direct_sum_finite(xyz, C, k: float, dirs, dist: float, nthreads: int = 0):
+    """ellipsoid_simple.cu:229-261 at ob = dist * s (about the origin), with the
+    far-field factor of the currents (D0 = 1 m) removed.  C: [S, n, 6] -> F [S, D, 6]."""
+    x, y, z = [_f32(v) for v in xyz]
+    C = np.ascontiguousarray(C, dtype=np.complex128)
+    if C.ndim == 2:
+        C = C[None]
+    S, n = C.shape[0], C.shape[1]
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+    F = np.zeros((S, dirs.shape[0], 6), dtype=np.complex128)
+    lib().orc_direct_sum_finite(n, _f(x), _f(y), _f(z), k, S, _d(C.view(np.float64)),
+                                dirs.shape[0], _d(dirs), dist, _d(F.view(np.float64)), nthreads)
+    return F
+
+
+def solve_finite(surf, incs, ntheta: int, nphi: int, dist: float, nthreads: int = 0, dirs=None):
+    """The finite-distance validator (ellipsoid_simple.cu:384-589 semantics) on the
+    output grid (or ``dirs``): (I [n_inc, D], E [n_inc, 3, D])."""
+    x, y, z, nx, ny, nz, a = [_f32(v) for v in surf]
+    n = x.shape[0]
+    arr = make_incidences(incs)
+    dirs = output_dirs(ntheta, nphi) if dirs is None else np.ascontiguousarray(dirs, np.float64)
+    D = dirs.shape[0]
+    I = np.zeros((len(incs), D))
+    E = np.zeros((len(incs), 3, D), dtype=np.complex128)
+    rc = lib().orc_solve_finite_dirs(n, _f(x), _f(y), _f(z), _f(nx), _f(ny), _f(nz), _f(a), arr,
+                                     len(incs), D, _d(dirs), dist, _d(I), _d(E.view(np.float64)),
+                                     nthreads)
+    if rc != 0:
+        raise MemoryError("oracle solve failed")
+    return I, E
 
 
 def intensity(dirs, F):

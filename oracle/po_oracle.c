@@ -233,6 +233,53 @@ void orc_direct_sum(int64_t n, const float* x, const float* y, const float* z,
   }
 }
 
+/* Finite-distance sum (compareBEM3d/ellipsoid_simple.cu:229-261 and its
+ * observation loop :547-555): ob = dist * s_d, green = exp(-i k |x - ob|)/|x - ob|
+ * with the distance formed from the coordinate differences as there.  The
+ * currents C carry the far-field factor exp(-i k D0)/D0 with D0 = 1 m
+ * (treePO.cu:504-509); the reference's validator applies -i w mu0/(4 pi) alone,
+ * so the sum is multiplied by D0 exp(+i k D0) to undo it. */
+void orc_direct_sum_finite(int64_t n, const float* x, const float* y, const float* z, double k,
+                           int S, const double* C, int64_t ndir, const double* dirs, double dist,
+                           double* F, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const cplx undo = cos(k) + I * sin(k);
+#pragma omp parallel
+  {
+    cplx* acc = (cplx*)malloc(sizeof(cplx) * 6 * (size_t)S);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t d = 0; d < ndir; ++d) {
+      const double* s = dirs + 3 * d;
+      const double ob[3] = {dist * s[0], dist * s[1], dist * s[2]};
+      for (int q = 0; q < 6 * S; ++q) acc[q] = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        double dx = (double)x[i] - ob[0];
+        double dy = (double)y[i] - ob[1];
+        double dz = (double)z[i] - ob[2];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        cplx g = (cos(k * r) - I * sin(k * r)) / r;
+        for (int sv = 0; sv < S; ++sv) {
+          const double* c = C + ((size_t)sv * n + i) * 12;
+          cplx* a = acc + 6 * sv;
+          for (int q = 0; q < 6; ++q) a[q] += (c[2 * q] + I * c[2 * q + 1]) * g;
+        }
+      }
+      for (int sv = 0; sv < S; ++sv)
+        for (int q = 0; q < 6; ++q) {
+          cplx v = acc[6 * sv + q] * undo;
+          double* f = F + (((size_t)sv * ndir + d) * 6 + q) * 2;
+          f[0] = creal(v);
+          f[1] = cimag(v);
+        }
+    }
+    free(acc);
+  }
+}
+
 /* ---------------------------------------------------------------- A13 --- */
 void orc_intensity(int64_t ndir, const double* dirs, const double* F, double* Iout, double* E) {
   const double Z0 = orc_z0();
@@ -259,22 +306,43 @@ void orc_intensity(int64_t ndir, const double* dirs, const double* F, double* Io
 }
 
 /* ----------------------------------------------------- full depth-0 ----- */
-int orc_solve_direct_dirs(int64_t n, const float* x, const float* y, const float* z,
-                          const float* nx, const float* ny, const float* nz, const float* area,
-                          const orc_incidence* inc, int n_inc, int64_t ndir, const double* dirs,
-                          const double centre[3], double* Iout, double* E, int nthreads) {
+static int solve_dirs(int64_t n, const float* x, const float* y, const float* z,
+                      const float* nx, const float* ny, const float* nz, const float* area,
+                      const orc_incidence* inc, int n_inc, int64_t ndir, const double* dirs,
+                      const double centre[3], double dist, double* Iout, double* E,
+                      int nthreads) {
   double* C = (double*)malloc(sizeof(double) * 12 * (size_t)n);
   double* F = (double*)malloc(sizeof(double) * 12 * (size_t)ndir);
   if (!C || !F) { free(C); free(F); return 1; }
   for (int i = 0; i < n_inc; ++i) {
     orc_currents(n, x, y, z, nx, ny, nz, area, inc + i, C);
     double k0 = 2.0 * PI / inc[i].lambda_m;
-    orc_direct_sum(n, x, y, z, centre, k0, 1, C, ndir, dirs, F, nthreads);
+    if (dist > 0)
+      orc_direct_sum_finite(n, x, y, z, k0, 1, C, ndir, dirs, dist, F, nthreads);
+    else
+      orc_direct_sum(n, x, y, z, centre, k0, 1, C, ndir, dirs, F, nthreads);
     orc_intensity(ndir, dirs, F, Iout + (size_t)i * ndir, E ? E + (size_t)i * 6 * ndir : NULL);
   }
   free(C);
   free(F);
   return 0;
+}
+
+int orc_solve_direct_dirs(int64_t n, const float* x, const float* y, const float* z,
+                          const float* nx, const float* ny, const float* nz, const float* area,
+                          const orc_incidence* inc, int n_inc, int64_t ndir, const double* dirs,
+                          const double centre[3], double* Iout, double* E, int nthreads) {
+  return solve_dirs(n, x, y, z, nx, ny, nz, area, inc, n_inc, ndir, dirs, centre, 0.0, Iout, E,
+                    nthreads);
+}
+
+int orc_solve_finite_dirs(int64_t n, const float* x, const float* y, const float* z,
+                          const float* nx, const float* ny, const float* nz, const float* area,
+                          const orc_incidence* inc, int n_inc, int64_t ndir, const double* dirs,
+                          double dist, double* Iout, double* E, int nthreads) {
+  const double origin[3] = {0.0, 0.0, 0.0};
+  return solve_dirs(n, x, y, z, nx, ny, nz, area, inc, n_inc, ndir, dirs, origin, dist, Iout, E,
+                    nthreads);
 }
 
 int orc_solve_direct(int64_t n, const float* x, const float* y, const float* z,

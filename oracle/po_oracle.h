@@ -58,6 +58,14 @@ void orc_direct_sum(int64_t n, const float* x, const float* y, const float* z,
                     const double centre[3], double k, int S, const double* C,
                     int64_t ndir, const double* dirs, double* F, int nthreads);
 
+/* compareBEM3d/ellipsoid_simple.cu:229-261, :547-555: the sum above with the
+ * exact Green's function exp(-i k |x - ob|)/|x - ob| at ob = dist * s_d about
+ * the origin, times D0 exp(+i k D0) (D0 = 1 m) to remove the far-field factor
+ * the currents carry. */
+void orc_direct_sum_finite(int64_t n, const float* x, const float* y, const float* z, double k,
+                           int S, const double* C, int64_t ndir, const double* dirs, double dist,
+                           double* F, int nthreads);
+
 /* util/fields.h:172-204: E'' = E_t - Z0 s x H_t, I = |E''|^2.
  * F[d][6][2] -> I[d], and (optional) E[3][d][2]. */
 void orc_intensity(int64_t ndir, const double* dirs, const double* F, double* I, double* E);
@@ -74,6 +82,13 @@ int orc_solve_direct_dirs(int64_t n, const float* x, const float* y, const float
                           const float* nx, const float* ny, const float* nz, const float* area,
                           const orc_incidence* inc, int n_inc, int64_t ndir, const double* dirs,
                           const double centre[3], double* I, double* E, int nthreads);
+
+/* orc_solve_direct_dirs with orc_direct_sum_finite at distance dist
+ * (ellipsoid_simple.cu:384-589 semantics). */
+int orc_solve_finite_dirs(int64_t n, const float* x, const float* y, const float* z,
+                          const float* nx, const float* ny, const float* nz, const float* area,
+                          const orc_incidence* inc, int n_inc, int64_t ndir, const double* dirs,
+                          double dist, double* Iout, double* E, int nthreads);
 
 /* util/tree.h:289-715 (+ treePO.cu:219-259): full octree to `depth` over the
  * float box {min x,y,z, max x,y,z}, recursive stable 8-way partition with the

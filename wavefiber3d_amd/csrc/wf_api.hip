@@ -908,9 +908,9 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   return WF_OK;
 }
 
-wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
+static wf_status po_solve_fp64_impl(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
                                   int32_t n_inc, wf_grid grid, double* intensity, double* E_cplx,
-                                  void* stream) {
+                                  void* stream, double dist) {
   if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
   if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
   if (grid.n_theta < 1 || grid.n_phi < 1) return set_err(WF_ERR_INVALID_ARG, "grid must be >= 1x1");
@@ -955,7 +955,7 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
     WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
       return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, 6 * S, centre,
                                     2.0 * WF_PI / inc[order[i0]].lambda_m, grid.n_theta,
-                                    grid.n_phi, F0, part, part_elems, st);
+                                    grid.n_phi, F0, part, part_elems, st, dist);
     }));
     for (int q = 0; q < S; ++q) {
       const int slot = order[i0 + q];
@@ -975,6 +975,20 @@ wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_in
     WF_HIP(hipStreamSynchronize(st));
   }
   return WF_OK;
+}
+
+wf_status wf_po_solve_fp64_direct(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
+                                  int32_t n_inc, wf_grid grid, double* intensity, double* E_cplx,
+                                  void* stream) {
+  return po_solve_fp64_impl(h, s, inc, n_inc, grid, intensity, E_cplx, stream, 0.0);
+}
+
+wf_status wf_po_solve_fp64_finite(wf_handle* h, const wf_surface* s, const wf_incidence* inc,
+                                  int32_t n_inc, wf_grid grid, double distance_m,
+                                  double* intensity, double* E_cplx, void* stream) {
+  if (!(distance_m > 0 && std::isfinite(distance_m)))
+    return set_err(WF_ERR_INVALID_ARG, "distance must be positive and finite");
+  return po_solve_fp64_impl(h, s, inc, n_inc, grid, intensity, E_cplx, stream, distance_m);
 }
 
 }  // extern "C"

@@ -77,11 +77,13 @@ hipError_t launch_currents(const float* x, const float* y, const float* z, const
 // Number of point chunks the direct kernel splits into for D directions.
 int direct_split(int64_t D);
 // scratch: partial sums of the point-split grid (up to 32 x F x D double2;
-// fewer splits when smaller, none when null).
+// fewer splits when smaller, none when null).  dist > 0: observation points
+// at distance dist (m) from the origin, exact Green's function (the centre is
+// then unused); dist <= 0: far field.
 hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, int64_t n,
                               const double2* C, int F, const double* centre, double k0,
                               int ntheta, int nphi, double2* Fout, double2* scratch,
-                              int64_t scratch_elems, hipStream_t st);
+                              int64_t scratch_elems, hipStream_t st, double dist = 0.0);
 
 hipError_t launch_intensity(const double2* F0, int S, int ntheta, int nphi, float* I, float* E,
                             double* Id, double* Ed, hipStream_t st);

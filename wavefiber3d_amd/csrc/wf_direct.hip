@@ -9,7 +9,16 @@
 // all S solves of one wavelength share the phase (F = 6 S complex columns).
 // Phase k0 s.(x - c) is formed in FP64 in revolutions and reduced exactly
 // before the FP64 sincos.
+//
+// Finite distance (compareBEM3d/ellipsoid_simple.cu:229-261, current_green):
+// the observation point is ob = dist * s about the origin and each point
+// contributes C exp(-i k0 |x - ob|) / |x - ob|, relative to the e^{-i k0 D0}
+// / D0 (D0 = 1 m) already in the currents: factor (1/r) exp(-i k0 (r - D0)).
+// r - dist is formed without cancellation as (|x|^2 - 2 dist s.x) / (r +
+// dist) and the common phase k0 (dist - D0) is reduced once on the host, so
+// the per-point phase stays accurate at any distance.
 #include <algorithm>
+#include <cmath>
 
 #include "wf_common.h"
 
@@ -30,12 +39,12 @@ __device__ inline void grid_dir(int64_t d, int nphi, int ntheta, double s[3]) {
   s[2] = st;
 }
 
-template <int F>
+template <int F, bool FINITE>
 __global__ __launch_bounds__(kDirThreads) void direct_fp64_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int64_t n,
     const double2* __restrict__ C, double cx, double cy, double cz, double k_rev, int ntheta,
-    int nphi, double2* __restrict__ Fout) {
-  __shared__ double sr[kPtTile][3];
+    int nphi, double dist, double rev0, double2* __restrict__ Fout) {
+  __shared__ double sr[kPtTile][FINITE ? 4 : 3];
   __shared__ double2 sc[kPtTile][F];
   const int64_t D = (int64_t)ntheta * nphi;
   const int64_t d = (int64_t)blockIdx.x * kDirThreads + threadIdx.x;
@@ -52,10 +61,19 @@ __global__ __launch_bounds__(kDirThreads) void direct_fp64_kernel(
     __syncthreads();
     for (int t = threadIdx.x; t < cnt; t += kDirThreads) {
       const int64_t i = base + t;
-      // scaled local coordinates: phase in revolutions = s . r
-      sr[t][0] = ((double)x[i] - cx) * k_rev;
-      sr[t][1] = ((double)y[i] - cy) * k_rev;
-      sr[t][2] = ((double)z[i] - cz) * k_rev;
+      if (FINITE) {
+        // absolute coordinates (m) and |x|^2
+        const double xa = x[i], ya = y[i], za = z[i];
+        sr[t][0] = xa;
+        sr[t][1] = ya;
+        sr[t][2] = za;
+        sr[t][3] = xa * xa + ya * ya + za * za;
+      } else {
+        // scaled local coordinates: phase in revolutions = s . r
+        sr[t][0] = ((double)x[i] - cx) * k_rev;
+        sr[t][1] = ((double)y[i] - cy) * k_rev;
+        sr[t][2] = ((double)z[i] - cz) * k_rev;
+      }
     }
     for (int t = threadIdx.x; t < cnt * F; t += kDirThreads) {
       const int p = t / F, f = t - p * F;
@@ -63,8 +81,20 @@ __global__ __launch_bounds__(kDirThreads) void direct_fp64_kernel(
     }
     __syncthreads();
     for (int p = 0; p < cnt; ++p) {
-      const double rev = s[0] * sr[p][0] + s[1] * sr[p][1] + s[2] * sr[p][2];
-      const double2 g = zcis_rev(rev);
+      double2 g;
+      if (FINITE) {
+        const double sx = s[0] * sr[p][0] + s[1] * sr[p][1] + s[2] * sr[p][2];
+        const double q = sr[p][3] - 2.0 * dist * sx;  // r^2 - dist^2
+        const double r = sqrt(dist * dist + q);
+        const double dr = q / (r + dist);  // r - dist
+        g = zcis_rev(-(dr * k_rev + rev0));
+        const double ir = 1.0 / r;
+        g.x *= ir;
+        g.y *= ir;
+      } else {
+        const double rev = s[0] * sr[p][0] + s[1] * sr[p][1] + s[2] * sr[p][2];
+        g = zcis_rev(rev);
+      }
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         const double2 c = sc[p][f];
@@ -142,7 +172,7 @@ int direct_split(int64_t D) {
 hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, int64_t n,
                               const double2* C, int F, const double* centre, double k0,
                               int ntheta, int nphi, double2* Fout, double2* scratch,
-                              int64_t scratch_elems, hipStream_t st) {
+                              int64_t scratch_elems, hipStream_t st, double dist) {
   const int64_t D = (int64_t)ntheta * nphi;
   const int64_t dblocks = (D + kDirThreads - 1) / kDirThreads;
   int P = direct_split(D);
@@ -150,10 +180,21 @@ hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, in
   dim3 grid((unsigned)dblocks, (unsigned)P);
   double2* dst = (P > 1) ? scratch : Fout;
   const double k_rev = k0 / WF_TWO_PI;
-#define WF_DIRECT_CASE(FF)                                                                 \
-  case FF:                                                                                 \
-    hipLaunchKernelGGL(direct_fp64_kernel<FF>, grid, dim3(kDirThreads), 0, st, x, y, z, n, \
-                       C, centre[0], centre[1], centre[2], k_rev, ntheta, nphi, dst);      \
+  // the currents carry e^{-i k0 D0}/D0 (D0 = 1 m): the common phase
+  // k0 (dist - D0) in revolutions, reduced once here so per-point phases
+  // keep full precision at any distance
+  const double rev0_full = (dist - 1.0) * k_rev;
+  const double rev0 = rev0_full - std::nearbyint(rev0_full);
+#define WF_DIRECT_CASE(FF)                                                                  \
+  case FF:                                                                                  \
+    if (dist > 0.0)                                                                         \
+      hipLaunchKernelGGL((direct_fp64_kernel<FF, true>), grid, dim3(kDirThreads), 0, st, x, \
+                         y, z, n, C, centre[0], centre[1], centre[2], k_rev, ntheta, nphi,  \
+                         dist, rev0, dst);                                                  \
+    else                                                                                    \
+      hipLaunchKernelGGL((direct_fp64_kernel<FF, false>), grid, dim3(kDirThreads), 0, st,   \
+                         x, y, z, n, C, centre[0], centre[1], centre[2], k_rev, ntheta,     \
+                         nphi, 0.0, 0.0, dst);                                              \
     break;
   switch (F) {
     WF_DIRECT_CASE(6)

@@ -288,26 +288,40 @@ class POSolver:
 
     def solve_fp64_direct(self, surface, incidences, grid, return_E: bool = False):
         """FP64 plain method on the GPU (sphere.cu:592-658 with depth 0)."""
+        return self._fp64(surface, incidences, grid, return_E, None)
+
+    def solve_fp64_finite(self, surface, incidences, grid, distance: float = 1.0,
+                          return_E: bool = False):
+        """FP64 direct sum with the exact Green's function exp(-ik|x-ob|)/|x-ob|
+        at observation points ob = distance * s (compareBEM3d/ellipsoid_simple.cu:
+        229-261, 384-589; the reference uses 1 m).  Same outputs as
+        solve_fp64_direct; for distance -> infinity the two agree."""
+        return self._fp64(surface, incidences, grid, return_E, float(distance))
+
+    def _fp64(self, surface, incidences, grid, return_E, distance):
         c, arrs, dev = self._surface(surface)
         arr, k = _incs(incidences)
         nt, np_ = grid
         g = _lib.Grid(nt, np_)
+        if distance is None:
+            call, dist = lib().wf_po_solve_fp64_direct, ()
+        else:
+            call, dist = lib().wf_po_solve_fp64_finite, (distance,)
         if dev:
             import torch
 
             I = torch.empty((k, nt, np_), dtype=torch.float64, device=arrs[0].device)
             E = torch.empty((k, 3, nt, np_, 2), dtype=torch.float64, device=arrs[0].device) if return_E else None
-            check(lib().wf_po_solve_fp64_direct(self._h, ct.byref(c), arr, k, g, I.data_ptr(),
-                                                E.data_ptr() if E is not None else None,
-                                                _stream_ptr(arrs)))
+            check(call(self._h, ct.byref(c), arr, k, g, *dist, I.data_ptr(),
+                       E.data_ptr() if E is not None else None, _stream_ptr(arrs)))
             self.depth = 0
             if return_E:
                 return I, torch.view_as_complex(E)
             return I
         I = np.empty((k, nt, np_), dtype=np.float64)
         E = np.empty((k, 3, nt, np_, 2), dtype=np.float64) if return_E else None
-        check(lib().wf_po_solve_fp64_direct(self._h, ct.byref(c), arr, k, g, I.ctypes.data,
-                                            E.ctypes.data if E is not None else None, None))
+        check(call(self._h, ct.byref(c), arr, k, g, *dist, I.ctypes.data,
+                   E.ctypes.data if E is not None else None, None))
         self.depth = 0
         if return_E:
             return I, E.view(np.complex128)[..., 0]
