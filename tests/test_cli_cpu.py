@@ -40,3 +40,17 @@ def test_average_grid_oracle_pins():
     g2 = rng.random((725, 1443)).astype(np.float32)
     a2 = oracle.average_grid(g2, 180, 360)
     assert np.array_equal(a2, oracle.average_grid(np.ascontiguousarray(g2[:720, :1440]), 180, 360))
+
+
+def test_mie_command_writes_notebook_files(tmp_path):
+    import os
+
+    from wavefiber3d_amd import mie
+
+    assert cli.main(["mie", "--radius", "1000", "--output-dir", str(tmp_path)]) == 0
+    _, SL, SR, SU = mie.scattering_function(1.55 + 0.1j, 400, 2000, min_angle=-180, max_angle=180,
+                                            angular_resolution=0.25, normalization="t")
+    for name, v in (("TM", SL), ("TE", SR), ("TEM", SU)):
+        got = np.fromfile(os.path.join(tmp_path, f"mie_1um_{name}.bin"), np.float64)
+        assert np.array_equal(got, v)
+    assert got.shape == (1441,)

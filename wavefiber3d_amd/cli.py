@@ -15,7 +15,10 @@ fiber's sigma; D = 1 m) and the octree box is +-1.1 (r1, r2, period/2)
 (treePO.cu:94-99).  The tree's own sampling rule and kernels are this
 library's (the output grid and file layout are the reference's).
 
-Options: ``--mesh-dir`` and ``--output-dir`` (defaults ``mesh``, ``output``),
+``python -m wavefiber3d_amd.cli mie`` writes the Mie curves of
+compareMie/mie.ipynb cells 2-3 (see ``mie_curves``).
+
+Options of treepo: ``--mesh-dir`` and ``--output-dir`` (defaults ``mesh``, ``output``),
 ``--final`` also writes ``..._final.binary``, the 180 x 360 block average of
 util/fields.h:206-234 (average_cpu) that the reference allocates for each
 incidence; ``--fiber-table FILE`` replaces the built-in table with a JSON list
@@ -123,14 +126,43 @@ def treepo(argv) -> int:
     return 0
 
 
+def mie_curves(argv) -> int:
+    """compareMie/mie.ipynb cells 2-3: the Mie scattering function of a sphere on
+    the -180..180 degree grid, written as float64 to
+    ``<output-dir>/mie_<radius/1000>um_{TM,TE,TEM}.bin`` (|S1|^2, |S2|^2 and their
+    mean; normalisation "t")."""
+    ap = argparse.ArgumentParser(prog="python -m wavefiber3d_amd.cli mie")
+    ap.add_argument("--wavelength", type=float, default=400.0, help="nm")
+    ap.add_argument("--radius", type=float, default=10000.0, help="nm")
+    ap.add_argument("--ior", type=complex, default=1.55 + 0.1j)
+    ap.add_argument("--resolution", type=float, default=0.25, help="degrees")
+    ap.add_argument("--output-dir", default="output")
+    a = ap.parse_args(argv)
+    from wavefiber3d_amd import mie
+
+    _, SL, SR, SU = mie.scattering_function(a.ior, a.wavelength, 2.0 * a.radius, min_angle=-180,
+                                            max_angle=180, angular_resolution=a.resolution,
+                                            normalization="t")
+    os.makedirs(a.output_dir, exist_ok=True)
+    tag = f"{a.radius / 1000:g}um"
+    for name, v in (("TM", SL), ("TE", SR), ("TEM", SU)):
+        v.astype(np.float64).tofile(os.path.join(a.output_dir, f"mie_{tag}_{name}.bin"))
+    print(f"wrote mie_{tag}_{{TM,TE,TEM}}.bin ({SL.shape[0]} angles)", flush=True)
+    return 0
+
+
+COMMANDS = {"treepo": treepo, "mie": mie_curves}
+
+
 def main(argv=None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
-    if not argv or argv[0] not in ("treepo",):
+    if not argv or argv[0] not in COMMANDS:
         print("usage: python -m wavefiber3d_amd.cli treepo LSTART LEND THETAI PHII FIBER DEPTH "
-              "START END [--mesh-dir D] [--output-dir D] [--final] [--fiber-table F] [--device N]",
-              file=sys.stderr)
+              "START END [--mesh-dir D] [--output-dir D] [--final] [--fiber-table F] [--device N]\n"
+              "       python -m wavefiber3d_amd.cli mie [--wavelength NM] [--radius NM] [--ior C] "
+              "[--resolution DEG] [--output-dir D]", file=sys.stderr)
         return 2
-    return treepo(argv[1:])
+    return COMMANDS[argv[0]](argv[1:])
 
 
 if __name__ == "__main__":
