@@ -335,6 +335,30 @@ wf_status wf_average_grid(const float* in, int32_t n_grids, int32_t n_theta, int
                           int32_t n_theta_out, int32_t n_phi_out, float* out, int32_t memspace,
                           void* stream);
 
+/* 2D cross-section PO (compareBEM2d/rough2d.cu:237-585): a closed polyline
+ * x[numel], y[numel] (metres, counter-clockwise; sample i at vertex i with the
+ * length and normal (dy, -dx)/|d| of segment i -> i+1 mod numel) lit by a
+ * plane wave per incidence; intensity [n_inc][n_phi] float64 at azimuths
+ * phi_k = 2 pi k / n_phi in the incidence's elevation plane:
+ * |E''|^2 distance_m / (2 radius_m) with the 2D far-field factor
+ * sqrt(2i/(pi k_rho D)) exp(-i k_rho D), k_rho = k cos theta (the reference:
+ * theta 0, phi pi, D = 1 m, radius 10 um, n_phi 3600).  FP64 throughout.
+ * memspace: x, y and intensity all host (returns with the result) or all
+ * device (the call still returns after the stream has finished, to report
+ * a zero-length segment, which is an error). */
+typedef struct wf_rough2d_incidence {
+  double theta;          /* elevation of incidence and observation (rough2d.cu:464) */
+  double phi;            /* incidence azimuth (rough2d.cu:471: pi) */
+  double lambda_m;       /* vacuum wavelength (rough2d.cu:467-468) */
+  double eta_re, eta_im; /* complex IOR, outside medium 1 (rough2d.cu:466) */
+  char pol;              /* 'M' = TM (E along z at theta 0), 'E' = TE (rough2d.cu:269-277) */
+} wf_rough2d_incidence;
+
+wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
+                           const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
+                           double distance_m, double radius_m, double* intensity,
+                           int32_t memspace, void* stream);
+
 /* Reads a fiber heightfield file (treePO.cu:79-88: raw float32, 2 * numel *
  * numelvert values in host byte order) into host memory; a missing or short
  * file is an error. */

@@ -78,6 +78,8 @@ def lib():
                                                                   ct.c_int64, dp, ct.c_double, dp,
                                                                   dp, ct.c_int]
     L.orc_solve_finite_dirs.restype = ct.c_int
+    L.orc_rough2d.argtypes = [ct.c_int, dp, dp, ct.c_double, ct.c_double, ct.c_double, ct.c_char,
+                              ct.c_double, ct.c_double, ct.c_int, ct.c_double, ct.c_double, dp, dp]
     L.orc_octree.argtypes = [ct.c_int64, fp, fp, fp, fp, ct.c_int, ct.POINTER(ct.c_int64),
                              ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), fp]
     L.orc_octree.restype = ct.c_int
@@ -319,3 +321,18 @@ def average_grid(grid, nto: int, npo: int) -> np.ndarray:
     fn.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
     fn(g.ctypes.data_as(ct.c_void_p), nt, npp, nto, npo, out.ctypes.data_as(ct.c_void_p))
     return out
+
+
+def rough2d(x, y, theta: float, phii: float, lambda_m: float, pol: str, eta: complex,
+            nphi: int = 3600, dist: float = 1.0, radius: float = 10e-6, currents: bool = False):
+    """compareBEM2d/rough2d.cu:237-585 for one incidence -> scattering [nphi]
+    (and the unscaled currents [n, 6] complex when ``currents``)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = x.shape[0]
+    out = np.zeros(nphi)
+    C = np.zeros((n, 6), dtype=np.complex128) if currents else None
+    eta = complex(eta)
+    lib().orc_rough2d(n, _d(x), _d(y), theta, phii, lambda_m, pol.encode(), eta.real, eta.imag,
+                      nphi, dist, radius, _d(out), _d(C.view(np.float64)) if currents else None)
+    return (out, C) if currents else out

@@ -595,3 +595,118 @@ void orc_average_grid(const float* in, int nt, int np, int nto, int npo, float* 
     }
   free(tmp);
 }
+
+/* ---------------------------------------------------------------- F4 ---- */
+/* compareBEM2d/rough2d.cu:237-362 (current_fresnel_oblique, eta_in = 1), the
+ * sample lengths :477-490, the 2D Green's function :58-76, the sum :520-560 and
+ * the projection and normalisation :562-577, for one incidence. */
+static void r2_cross(const cplx a[3], const cplx b[3], cplx r[3]) { /* rough2d.cu:172-176 */
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = -(a[0] * b[2] - a[2] * b[0]);
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static cplx r2_dot(const cplx a[3], const cplx b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static cplx r2_rs(double t, cplx nt) { /* rough2d.cu:210-215 with n0 = 1 */
+  cplx p0 = 1.0 * cos_internal(1.0, t);
+  cplx pt = nt * cos_internal(nt / 1.0, t);
+  return (p0 - pt) / (p0 + pt);
+}
+static cplx r2_rp(double t, cplx nt) { /* rough2d.cu:217-222 with n0 = 1 */
+  cplx q0 = cos_internal(1.0, t) / 1.0;
+  cplx qt = cos_internal(nt, t) / nt;
+  return (q0 - qt) / (q0 + qt);
+}
+
+void orc_rough2d(int n, const double* x, const double* y, double theta, double phii,
+                 double lambda_m, char mode, double eta_re, double eta_im, int nphi, double Dis,
+                 double radius, double* scattering, double* C /* [n][6][2] or NULL */) {
+  const double Z0 = orc_z0();
+  const cplx eta = eta_re + I * eta_im;
+  const double k = 2 * PI / lambda_m, krho = k * cos(theta), omega = k * ORC_C0;
+  cplx w[3] = {cos(theta) * cos(phii), cos(theta) * sin(phii), sin(theta)}; /* :264-266 */
+  cplx Ed[3], Hd[3];
+  const cplx v[3] = {-sin(theta) * cos(phii), -sin(theta) * sin(phii), cos(theta)};
+  if (mode == 'M') { /* :268-272, :292 */
+    for (int c = 0; c < 3; ++c) Ed[c] = v[c];
+    r2_cross(w, Ed, Hd);
+  } else { /* :273-277, :296 */
+    for (int c = 0; c < 3; ++c) Hd[c] = v[c];
+    r2_cross(Hd, w, Ed);
+  }
+  cplx* cur = (cplx*)calloc((size_t)n * 6, sizeof(cplx));
+  double* len = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    double xn = (i < n - 1) ? x[i + 1] : x[0], yn = (i < n - 1) ? y[i + 1] : y[0];
+    len[i] = sqrt((x[i] - xn) * (x[i] - xn) + (y[i] - yn) * (y[i] - yn)); /* :486-487 */
+    cplx nrm[3] = {yn - y[i], -(xn - x[i]), 0.0};                          /* :303-304 */
+    double nn = sqrt(creal(nrm[0] * conj(nrm[0]) + nrm[1] * conj(nrm[1])));
+    for (int c = 0; c < 3; ++c) nrm[c] *= 1.0 / nn; /* :305 */
+    double phase = creal(k * (w[0] * x[i] + w[1] * y[i]));       /* :286-287 */
+    cplx Ei0 = cexp(-I * phase), Hi0 = cexp(-I * phase) / Z0;    /* :288-289 */
+    cplx Ei[3], Hi[3], Es[3], Hs[3];
+    for (int c = 0; c < 3; ++c) { Ei[c] = Ed[c] * Ei0; Hi[c] = Hd[c] * Hi0; }
+    double dotvalue = creal(r2_dot(w, nrm)); /* :307 */
+    if (dotvalue <= 0) {
+      if (fabs(fabs(dotvalue) - 1) < 1e-15) { /* :311-315 */
+        cplx R = r2_rs(0, eta);
+        for (int c = 0; c < 3; ++c) { Es[c] = Ei[c] * R; Hs[c] = Hi[c] * R; }
+      } else { /* :316-341 */
+        cplx q[3], Eperp[3], Epar[3], Rdir[3], Esp[3], Hsp[3], Hip[3], Hsq[3], Esq[3];
+        r2_cross(w, nrm, q);
+        double qn = sqrt(creal(q[0] * conj(q[0]) + q[1] * conj(q[1]) + q[2] * conj(q[2])));
+        for (int c = 0; c < 3; ++c) q[c] *= 1.0 / qn;
+        cplx f = r2_dot(Ei, q);
+        for (int c = 0; c < 3; ++c) { Eperp[c] = q[c] * f; Epar[c] = Ei[c] - Eperp[c]; }
+        double lphi = acos(cabs(r2_dot(w, nrm)));
+        cplx R = r2_rs(lphi, eta);
+        cplx f2 = 2.0 * r2_dot(w, nrm);
+        for (int c = 0; c < 3; ++c) { Rdir[c] = w[c] - nrm[c] * f2; Esp[c] = Eperp[c] * R; }
+        r2_cross(Rdir, Esp, Hsp);
+        for (int c = 0; c < 3; ++c) Hsp[c] *= 1.0 / Z0;
+        R = r2_rp(lphi, eta);
+        r2_cross(w, Epar, Hip);
+        for (int c = 0; c < 3; ++c) { Hip[c] *= 1.0 / Z0; Hsq[c] = Hip[c] * R; }
+        r2_cross(Hsq, Rdir, Esq);
+        for (int c = 0; c < 3; ++c) { Esq[c] *= Z0; Es[c] = Esp[c] + Esq[c]; Hs[c] = Hsp[c] + Hsq[c]; }
+      }
+    } else { /* :344-346 */
+      for (int c = 0; c < 3; ++c) { Es[c] = -Ei[c]; Hs[c] = -Hi[c]; }
+    }
+    cplx Et[3], Ht[3], M[3], J[3];
+    for (int c = 0; c < 3; ++c) { Et[c] = Es[c] + Ei[c]; Ht[c] = Hs[c] + Hi[c]; }
+    r2_cross(nrm, Et, M); /* :348-350 */
+    for (int c = 0; c < 3; ++c) M[c] = -M[c];
+    r2_cross(nrm, Ht, J);
+    for (int c = 0; c < 3; ++c) { cur[6 * i + c] = J[c]; cur[6 * i + 3 + c] = M[c]; }
+  }
+  if (C)
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < 6; ++c) {
+        C[12 * i + 2 * c] = creal(cur[6 * i + c]);
+        C[12 * i + 2 * c + 1] = cimag(cur[6 * i + c]);
+      }
+  for (int p = 0; p < nphi; ++p) {
+    double phi = (double)p / (double)nphi * 2 * PI; /* :521 */
+    double dir[3] = {cos(theta) * cos(phi), cos(theta) * sin(phi), sin(theta)};
+    cplx E[3] = {0, 0, 0}, H[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      double d = x[i] * dir[0] + y[i] * dir[1]; /* :37-41 */
+      cplx g = csqrt(2.0 * I / (PI * krho * Dis)) * cexp(-I * krho * Dis) * cexp(I * krho * d); /* :69 */
+      for (int c = 0; c < 3; ++c) {
+        E[c] += cur[6 * i + c] * (-omega * ORC_MU0 / 4.0 * g) * len[i];     /* :89, :114 */
+        H[c] += cur[6 * i + 3 + c] * (-omega * ORC_EPS0 / 4.0 * g) * len[i];
+      }
+    }
+    cplx dv = E[0] * dir[0] + E[1] * dir[1] + E[2] * dir[2]; /* :562-569 */
+    for (int c = 0; c < 3; ++c) E[c] -= dv * dir[c];
+    dv = H[0] * dir[0] + H[1] * dir[1] + H[2] * dir[2];
+    for (int c = 0; c < 3; ++c) H[c] -= dv * dir[c];
+    cplx Ex = E[0] - (dir[1] * H[2] - dir[2] * H[1]) * Z0; /* :571-573 */
+    cplx Ey = E[1] - (-(dir[0] * H[2] - dir[2] * H[0])) * Z0;
+    cplx Ez = E[2] - (dir[0] * H[1] - dir[1] * H[0]) * Z0;
+    double m2 = creal(Ex * conj(Ex)) + creal(Ey * conj(Ey)) + creal(Ez * conj(Ez));
+    scattering[p] = m2 * Dis / (2 * radius); /* :577 */
+  }
+  free(cur);
+  free(len);
+}

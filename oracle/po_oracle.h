@@ -126,6 +126,14 @@ void orc_average_grid(const float* in, int nt, int np, int nto, int npo, float* 
 void orc_sphere_interp(int ntc, int npc, double off_tc, const double* in,
                        int ntp, int npp, double off_tp, double* out);
 
+/* compareBEM2d/rough2d.cu:237-585 for one incidence: closed polyline x, y [n]
+ * (sample i at vertex i, segment i -> i+1 mod n), scattering [nphi] =
+ * |E''|^2 Dis/(2 radius) at phi_p = 2 pi p/nphi; C (nullable) receives the
+ * unscaled currents [n][J, M][2] (rough2d.cu:348-361). */
+void orc_rough2d(int n, const double* x, const double* y, double theta, double phii,
+                 double lambda_m, char mode, double eta_re, double eta_im, int nphi, double Dis,
+                 double radius, double* scattering, double* C);
+
 #ifdef __cplusplus
 }
 #endif

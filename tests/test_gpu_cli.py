@@ -69,3 +69,29 @@ def test_average_grid_device_and_host_bit_identical():
         ref = oracle.average_grid(g[k].cpu().numpy(), 180, 360)
         assert np.array_equal(dev[k], ref)
         assert np.array_equal(host[k], ref)
+
+
+def test_rough2d_command_writes_reference_files(tmp_path):
+    import os
+    import shutil
+
+    from oracle import oracle
+    from wavefiber3d_amd import cli
+    from wavefiber3d_amd import rough2d as r2
+
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    xs = os.path.join(tmp_path, "crosssection")
+    os.makedirs(xs)
+    for f in ("circle_x.txt", "circle_y.txt", "ellipse_x.txt", "ellipse_y.txt"):
+        shutil.copy(os.path.join(g, f), xs)
+    out = os.path.join(tmp_path, "output")
+    print("case: cli rough2d circle TM 3..5, ellipse TE 0..1", flush=True)
+    assert cli.main(["rough2d", "0", "M", "3", "5", "--crosssection-dir", xs, "--output-dir", out]) == 0
+    assert cli.main(["rough2d", "1", "E", "0", "1", "--crosssection-dir", xs, "--output-dir", out]) == 0
+    x, y = r2.read_crosssection(os.path.join(xs, "circle_x.txt"), os.path.join(xs, "circle_y.txt"))
+    for l in (3, 4):
+        got = np.fromfile(os.path.join(out, f"circle_TM_{l}.binary"), np.float64)
+        ref = oracle.rough2d(x, y, 0.0, np.pi, r2.wavelength(l), "M", 1.55 + 0.1j)
+        assert got.shape == (3600,)
+        assert np.abs(got - ref).max() < 1e-11 * ref.max()
+    assert os.path.exists(os.path.join(out, "ellipse_TE_0.binary"))

@@ -23,10 +23,12 @@ from .po import (  # noqa: F401
     reference_incidence_theta,
     reference_lambda,
 )
+from .rough2d import Rough2dIncidence, read_crosssection, rough2d_solve  # noqa: F401
 
 __all__ = [
     "WfError", "lib", "SurfaceSamples", "fibonacci_sphere", "fiber_segment", "read_soa_mesh",
     "heightfield_samples", "read_heightfield", "average_grid",
     "Batch", "Incidence", "POSolver", "output_directions", "reference_incidence_phi",
-    "reference_incidence_theta", "reference_lambda",
+    "reference_incidence_theta", "reference_lambda", "Rough2dIncidence", "read_crosssection",
+    "rough2d_solve",
 ]

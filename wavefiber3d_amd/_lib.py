@@ -31,7 +31,7 @@ EXPORTS = (
     "wf_octree_nodes", "wf_interp", "wf_get_stats", "wf_po_vjp", "wf_geometry_info",
     "wf_get_tune", "wf_set_tune", "wf_autotune", "wf_batch_create", "wf_batch_destroy",
     "wf_batch_add", "wf_batch_solve", "wf_batch_info", "wf_heightfield_points",
-    "wf_read_heightfield", "wf_average_grid", "wf_po_solve_fp64_finite",
+    "wf_read_heightfield", "wf_average_grid", "wf_po_solve_fp64_finite", "wf_rough2d_solve",
 )
 WF_DEPTH_AUTO = -1
 
@@ -58,6 +58,14 @@ class Incidence(ct.Structure):
         ("pol", ct.c_char),
         ("eta_re", ct.c_double), ("eta_im", ct.c_double),
         ("beam_sigma_m", ct.c_double),
+    ]
+
+
+class Rough2dIncidence(ct.Structure):
+    _fields_ = [
+        ("theta", ct.c_double), ("phi", ct.c_double), ("lambda_m", ct.c_double),
+        ("eta_re", ct.c_double), ("eta_im", ct.c_double),
+        ("pol", ct.c_char),
     ]
 
 
@@ -167,6 +175,8 @@ def lib(path: str | None = None):
     L.wf_heightfield_points.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_float, ct.c_float,
                                         ct.c_int32, ct.c_int32, vp, ct.c_int32, vp]
     L.wf_read_heightfield.argtypes = [ct.c_char_p, ct.c_int32, ct.c_int32, vp]
+    L.wf_rough2d_solve.argtypes = [vp, vp, ct.c_int32, ct.POINTER(Rough2dIncidence), ct.c_int32,
+                                   ct.c_int32, ct.c_double, ct.c_double, vp, ct.c_int32, vp]
     L.wf_po_solve_fp64_finite.argtypes = [vp, ct.POINTER(Surface), ct.POINTER(Incidence),
                                           ct.c_int32, Grid, ct.c_double, vp, vp, vp]
     L.wf_average_grid.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32,

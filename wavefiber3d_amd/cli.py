@@ -16,7 +16,9 @@ fiber's sigma; D = 1 m) and the octree box is +-1.1 (r1, r2, period/2)
 library's (the output grid and file layout are the reference's).
 
 ``python -m wavefiber3d_amd.cli mie`` writes the Mie curves of
-compareMie/mie.ipynb cells 2-3 (see ``mie_curves``).
+compareMie/mie.ipynb cells 2-3 (see ``mie_curves``);
+``python -m wavefiber3d_amd.cli rough2d`` is compareBEM2d/rough2d.cu's driver
+(see ``rough2d``).
 
 Options of treepo: ``--mesh-dir`` and ``--output-dir`` (defaults ``mesh``, ``output``),
 ``--final`` also writes ``..._final.binary``, the 180 x 360 block average of
@@ -151,7 +153,55 @@ def mie_curves(argv) -> int:
     return 0
 
 
-COMMANDS = {"treepo": treepo, "mie": mie_curves}
+def rough2d(argv) -> int:
+    """compareBEM2d/rough2d.cu:393-583: ``rough2d TESTCASE MODE LSTART LEND``
+    (TESTCASE 0: circle, 1000 samples; otherwise the ellipse, 2000 samples;
+    MODE M or E; wavelength indices [LSTART, LEND) of 400 + 300/25 l nm) reads
+    ``<crosssection-dir>/circle_{x,y}.txt`` or ``ellipse_new_{x,y}.txt`` (falling
+    back to ``ellipse_{x,y}.txt``, the name the reference's data ships under),
+    and writes ``<output-dir>/<geo>_T<MODE>_<l>.binary``: 3600 float64 values,
+    the scattering at phi = 2 pi k / 3600 for incidence along -x (theta 0,
+    phi pi), IOR 1.55 + 0.1i, D = 1 m, normalised by 2 x 10 um."""
+    ap = argparse.ArgumentParser(prog="python -m wavefiber3d_amd.cli rough2d")
+    ap.add_argument("testcase", type=int)
+    ap.add_argument("mode", choices=["M", "E"])
+    ap.add_argument("lambdaindexstart", type=int)
+    ap.add_argument("lambdaindexend", type=int)
+    ap.add_argument("--crosssection-dir", default="crosssection")
+    ap.add_argument("--output-dir", default="output")
+    a = ap.parse_args(argv)
+    from wavefiber3d_amd import rough2d as r2
+
+    if a.testcase == 0:
+        geo, names, numel = "circle", ["circle"], 1000
+    else:
+        geo, names, numel = "ellipse", ["ellipse_new", "ellipse"], 2000
+    for nm in names:
+        xf = os.path.join(a.crosssection_dir, f"{nm}_x.txt")
+        yf = os.path.join(a.crosssection_dir, f"{nm}_y.txt")
+        if os.path.exists(xf) and os.path.exists(yf):
+            break
+    else:
+        print(f"no cross-section files for {geo} in {a.crosssection_dir}", file=sys.stderr)
+        return 2
+    x, y = r2.read_crosssection(xf, yf)
+    if x.shape[0] < numel:
+        print(f"{xf}: {x.shape[0]} values, the reference reads {numel}", file=sys.stderr)
+        return 2
+    x, y = x[:numel], y[:numel]
+    lams = list(range(a.lambdaindexstart, a.lambdaindexend))
+    if not lams:
+        return 0
+    incs = [r2.Rough2dIncidence(r2.wavelength(l), a.mode) for l in lams]
+    I = r2.rough2d_solve(x, y, incs)
+    os.makedirs(a.output_dir, exist_ok=True)
+    for i, l in enumerate(lams):
+        I[i].astype(np.float64).tofile(os.path.join(a.output_dir, f"{geo}_T{a.mode}_{l}.binary"))
+        print(f"wavelength index {l} done", flush=True)
+    return 0
+
+
+COMMANDS = {"treepo": treepo, "mie": mie_curves, "rough2d": rough2d}
 
 
 def main(argv=None) -> int:
@@ -160,7 +210,9 @@ def main(argv=None) -> int:
         print("usage: python -m wavefiber3d_amd.cli treepo LSTART LEND THETAI PHII FIBER DEPTH "
               "START END [--mesh-dir D] [--output-dir D] [--final] [--fiber-table F] [--device N]\n"
               "       python -m wavefiber3d_amd.cli mie [--wavelength NM] [--radius NM] [--ior C] "
-              "[--resolution DEG] [--output-dir D]", file=sys.stderr)
+              "[--resolution DEG] [--output-dir D]\n"
+              "       python -m wavefiber3d_amd.cli rough2d TESTCASE MODE LSTART LEND "
+              "[--crosssection-dir D] [--output-dir D]", file=sys.stderr)
         return 2
     return COMMANDS[argv[0]](argv[1:])
 

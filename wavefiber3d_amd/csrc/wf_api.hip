@@ -2236,6 +2236,110 @@ wf_status wf_heightfield_points(const float* height, int32_t numel, int32_t nume
   return WF_OK;
 }
 
+wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
+                           const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
+                           double distance_m, double radius_m, double* intensity,
+                           int32_t memspace, void* stream) {
+  if (!x || !y || !inc || !intensity) return set_err(WF_ERR_INVALID_ARG, "null argument");
+  if (numel < 2 || numel > (1 << 26)) return set_err(WF_ERR_INVALID_ARG, "numel must be in [2, 2^26]");
+  if (n_inc < 1 || n_inc > 65535) return set_err(WF_ERR_INVALID_ARG, "n_inc must be in [1, 65535]");
+  if (n_phi < 1 || n_phi > (1 << 24)) return set_err(WF_ERR_INVALID_ARG, "n_phi must be in [1, 2^24]");
+  if ((int64_t)numel * n_inc > (1LL << 28))
+    return set_err(WF_ERR_INVALID_ARG, "numel * n_inc too large");
+  if (!(distance_m > 0 && std::isfinite(distance_m)) || !(radius_m > 0 && std::isfinite(radius_m)))
+    return set_err(WF_ERR_INVALID_ARG, "distance and radius must be positive and finite");
+  if (memspace != WF_MEM_HOST && memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "memspace unknown");
+  const double mu0 = 4e-7 * WF_PI, eps0 = 8.8541878e-12, c0 = 299792458.0;  // rough2d.cu:25-28
+  std::vector<wf::Rough2dIncDev> hinc((size_t)n_inc);
+  for (int i = 0; i < n_inc; ++i) {
+    const wf_rough2d_incidence& a = inc[i];
+    if (!(a.lambda_m > 0 && std::isfinite(a.lambda_m)))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: wavelength must be positive", i);
+    if (!(std::isfinite(a.theta) && std::fabs(a.theta) < WF_PI / 2 && std::isfinite(a.phi)))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: need |theta| < pi/2 and finite phi", i);
+    if (a.pol != 'M' && a.pol != 'E')
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: pol must be 'M' or 'E'", i);
+    if (!(std::isfinite(a.eta_re) && std::isfinite(a.eta_im)))
+      return set_err(WF_ERR_INVALID_ARG, "incidence %d: eta must be finite", i);
+    wf::Rough2dIncDev& d = hinc[(size_t)i];
+    const double ct = std::cos(a.theta), st_ = std::sin(a.theta);
+    const double cp = std::cos(a.phi), sp = std::sin(a.phi);
+    d.w[0] = ct * cp;  // rough2d.cu:264-266
+    d.w[1] = ct * sp;
+    d.w[2] = st_;
+    const double v[3] = {-st_ * cp, -st_ * sp, ct};
+    double* A = (a.pol == 'M') ? d.Ed : d.Hd;  // rough2d.cu:268-277
+    double* B = (a.pol == 'M') ? d.Hd : d.Ed;
+    for (int c = 0; c < 3; ++c) A[c] = v[c];
+    if (a.pol == 'M') {  // H = w x E (:292)
+      B[0] = d.w[1] * v[2] - d.w[2] * v[1];
+      B[1] = d.w[2] * v[0] - d.w[0] * v[2];
+      B[2] = d.w[0] * v[1] - d.w[1] * v[0];
+    } else {  // E = H x w (:296)
+      B[0] = v[1] * d.w[2] - v[2] * d.w[1];
+      B[1] = v[2] * d.w[0] - v[0] * d.w[2];
+      B[2] = v[0] * d.w[1] - v[1] * d.w[0];
+    }
+    d.k = 2.0 * WF_PI / a.lambda_m;
+    d.krho = d.k * ct;  // :474
+    d.ct = ct;
+    d.st = st_;
+    d.z0 = std::sqrt(mu0 / eps0);
+    d.eta_re = a.eta_re;
+    d.eta_im = a.eta_im;
+    const double omega = d.k * c0;
+    d.fe = -omega * mu0 / 4.0;  // :82-90
+    d.fh = -omega * eps0 / 4.0;
+  }
+  hipStream_t st = as_stream(stream);
+  const int64_t nI = (int64_t)n_inc * n_phi;
+  double* dx = const_cast<double*>(x);
+  double* dy = const_cast<double*>(y);
+  double* dI = intensity;
+  wf::Rough2dIncDev* dinc = nullptr;
+  double2* C = nullptr;
+  int* bad = nullptr;
+  hipError_t e = hipSuccess;
+  if (memspace == WF_MEM_HOST) {
+    dx = dy = dI = nullptr;
+    e = hipMallocAsync(reinterpret_cast<void**>(&dx), sizeof(double) * numel, st);
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&dy), sizeof(double) * numel, st);
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&dI), sizeof(double) * nI, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(double) * numel, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, sizeof(double) * numel, hipMemcpyHostToDevice, st);
+  }
+  if (e == hipSuccess)
+    e = hipMallocAsync(reinterpret_cast<void**>(&dinc), sizeof(wf::Rough2dIncDev) * n_inc, st);
+  if (e == hipSuccess)
+    e = hipMallocAsync(reinterpret_cast<void**>(&C), sizeof(double2) * 6 * (size_t)numel * n_inc, st);
+  if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&bad), sizeof(int), st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dinc, hinc.data(), sizeof(wf::Rough2dIncDev) * n_inc,
+                       hipMemcpyHostToDevice, st);
+  int hbad = 0x7fffffff;
+  if (e == hipSuccess)
+    e = wf::launch_rough2d(dx, dy, numel, dinc, n_inc, n_phi, distance_m, radius_m, C, dI, bad, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && memspace == WF_MEM_HOST)
+    e = hipMemcpyAsync(intensity, dI, sizeof(double) * nI, hipMemcpyDeviceToHost, st);
+  if (bad) (void)hipFreeAsync(bad, st);
+  if (C) (void)hipFreeAsync(C, st);
+  if (dinc) (void)hipFreeAsync(dinc, st);
+  if (memspace == WF_MEM_HOST) {
+    if (dx) (void)hipFreeAsync(dx, st);
+    if (dy) (void)hipFreeAsync(dy, st);
+    if (dI) (void)hipFreeAsync(dI, st);
+  }
+  const hipError_t es = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = es;
+  if (e != hipSuccess) return set_err(WF_ERR_HIP, "wf_rough2d_solve: %s", hipGetErrorString(e));
+  if (hbad != 0x7fffffff)
+    return set_err(WF_ERR_INVALID_ARG, "cross-section segment %d has zero length or a non-finite vertex",
+                   hbad);
+  return WF_OK;
+}
+
 wf_status wf_average_grid(const float* in, int32_t n_grids, int32_t n_theta, int32_t n_phi,
                           int32_t n_theta_out, int32_t n_phi_out, float* out, int32_t memspace,
                           void* stream) {

@@ -87,9 +87,6 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
 
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
                             const double h[3], double k0, hipStream_t st);
-// per-child extended table for children at offsets (cx, cy, cz)[c] from the
-// parent centre; offsets of level-r nodes from the root centre from their keys
-// (hx/hy/hz[l]: the child-centre offset magnitudes of the level-l transition)
 // heightfield -> 7 SoA arrays (wf_geometry.hip); *first_bad receives the
 // smallest i * numel + j of a sample with a zero vertex (INT_MAX if none)
 hipError_t launch_heightfield_points(const float* height, int numel, int numelvert, float period,
@@ -98,6 +95,22 @@ hipError_t launch_heightfield_points(const float* height, int numel, int numelve
 // block average of [ngrids][nt][np] onto [ngrids][nto][npo] (wf_grid.hip)
 hipError_t launch_average_grid(const float* in, int ngrids, int nt, int np, int nto, int npo,
                                float* out, hipStream_t st);
+// one incidence of the 2D cross-section solve (wf_rough2d.hip), FP64, host-made
+struct Rough2dIncDev {
+  double w[3], Ed[3], Hd[3];  // propagation direction, E and H directions
+  double k, krho, ct, st;     // k0, k0 cos t, cos t, sin t (t: elevation)
+  double z0, eta_re, eta_im;
+  double fe, fh;              // -w mu0/4 and -w eps0/4
+};
+// 2D cross-section PO: currents C[n_inc][n][6] (scratch) and intensity
+// I[n_inc][nphi]; *bad receives the smallest index of a zero-length or
+// non-finite segment (must be INT_MAX on entry)
+hipError_t launch_rough2d(const double* x, const double* y, int n, const Rough2dIncDev* incs,
+                          int n_inc, int nphi, double dist, double radius, double2* C, double* I,
+                          int* bad, hipStream_t st);
+// per-child extended table for children at offsets (cx, cy, cz)[c] from the
+// parent centre; offsets of level-r nodes from the root centre from their keys
+// (hx/hy/hz[l]: the child-centre offset magnitudes of the level-l transition)
 hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
                               const double* cx, const double* cy, const double* cz, int nch,
                               double k0, hipStream_t st);
