@@ -84,6 +84,17 @@ def workload(name: str):
         return dict(average=True,
                     desc="block average of 25 intensity grids 7200x1440 -> 180x360 "
                          "(util/fields.h:206-234)")
+    if name == "finite_ellipsoid_ring":
+        return dict(finite=True,
+                    desc="finite-distance FP64 validator: ellipsoid 3x3x2um mesh (30322 samples), "
+                         "500 nm, IOR 1.55+0.3i, incidence +x TM, beam sigma 1 cm, D = 1 m, "
+                         "grid 2x3600 (the theta = 0 ring and the south pole ring) "
+                         "(compareBEM3d/ellipsoid_simple.cu:384-589)")
+    if name == "rough2d_sweep":
+        return dict(rough2d=True,
+                    desc="2D cross-section PO: reference ellipse (2000 samples), 25 wavelengths "
+                         "400-688 nm x TM+TE, incidence phi = pi, IOR 1.55+0.1i, 3600 azimuths, "
+                         "FP64 (compareBEM2d/rough2d.cu:393-583)")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -172,6 +183,10 @@ def main():
         return bench_average(args, W)
     if W.get("adjoint"):
         return bench_adjoint(args, W)
+    if W.get("finite"):
+        return bench_finite(args, W)
+    if W.get("rough2d"):
+        return bench_rough2d(args, W)
     surf = W["surface"]()
     incs = W["incs"]
     nt, np_ = W["grid"]
@@ -479,6 +494,132 @@ def bench_average(args, W):
                                "sample": f"one 7200x1440 grid, C oracle, {cs:.3f} s"}
     print(f"[bench] average: {dev_ms:.3f} ms/call device, {achieved:.0f} GB/s, parity "
           f"{res.get('parity_bit_identical')}", file=sys.stderr, flush=True)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def _aux_timing(call, steps):
+    """Wall and HIP-event (current stream) time per call, in ms."""
+    import torch
+
+    st = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for a, b in ev:
+        a.record(st)
+        out = call()
+        b.record(st)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    dev_ms = sum(a.elapsed_time(b) for a, b in ev) / steps
+    return out, wall_ms, dev_ms
+
+
+def bench_finite(args, W):
+    """Finite-distance FP64 validator (wf_po_solve_fp64_finite), surface in HBM.
+    One step = one solve of the reference's validator configuration.  The
+    dominant kernel is the FP64 direct sum: 48 useful FP64 flops per
+    point-direction pair (6 complex multiply-adds) against the FP64 peak (the
+    vector and matrix FP64 peaks of MI355X are both 78.6 TFLOP/s)."""
+    import torch
+
+    import wavefiber3d_amd as wf
+
+    g = os.path.join(ROOT, "tests", "golden")
+    s = wf.read_soa_mesh(os.path.join(g, "ellipsoid_3_3_2_float.binary"),
+                         os.path.join(g, "ellipsoid_3_3_2_normal_float.binary"),
+                         os.path.join(g, "ellipsoid_3_3_2_area_float.binary"))
+    inc = wf.Incidence(0.0, 0.0, 500e-9, "M", 1.55 + 0.3j, 1e-2)
+    grid = (2, 3600)
+    dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
+    solver = wf.POSolver(device=0)
+    for _ in range(max(1, args.warmup)):
+        solver.solve_fp64_finite(dev, [inc], grid, 1.0)
+    I, wall_ms, dev_ms = _aux_timing(lambda: solver.solve_fp64_finite(dev, [inc], grid, 1.0),
+                                     args.steps)
+    k = solver.stats()["kernels"]["direct_fp64"]
+    pairs = float(s.n) * grid[0] * grid[1]
+    kms = k["ms"] / k["launches"]
+    tflops = 48.0 * pairs / (kms * 1e-3) / 1e12
+    res = {"metric": "finite-distance validator point-direction pairs/s (auxiliary workload)",
+           "value": pairs / (wall_ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "reference ellipsoid mesh",
+           "config": {"workload": args.workload, "description": W["desc"], "n_points": s.n,
+                      "n_directions": grid[0] * grid[1]},
+           "roofline": {"bound": "mfma", "kernel": "direct_fp64_kernel<6, finite> (FP64 vector ALU)",
+                        "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                        "work_per_launch": 48.0 * pairs, "avg_launch_ms": kms},
+           "device_ms_per_call": dev_ms}
+    if not args.no_cpu:
+        from oracle import oracle
+
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        ref, _ = oracle.solve_finite(s.arrays(), [inc.as_tuple()], *grid, 1.0, nthreads=nthreads)
+        cs = time.perf_counter() - t0
+        got = I.cpu().numpy().reshape(-1)
+        res["parity_rel_l2"] = float(np.linalg.norm(got - ref[0]) / np.linalg.norm(ref[0]))
+        res["cpu_baseline"] = {"value": pairs / cs, "unit": "pairs/s", "cores": nthreads,
+                               "kind": "port",
+                               "sample": f"the whole workload, C oracle (OpenMP), {cs:.2f} s"}
+    solver.close()
+    print(f"[bench] finite: {wall_ms:.3f} ms/call wall, kernel {kms:.3f} ms, {tflops:.1f} TFLOP/s, "
+          f"parity {res.get('parity_rel_l2')}", file=sys.stderr, flush=True)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_rough2d(args, W):
+    """2D cross-section PO (wf_rough2d_solve), polyline in HBM: one step = one
+    call with all 50 incidences.  The dominant kernel is the far-field sum:
+    48 useful FP64 flops per sample-direction pair, FP64 peak 78.6 TFLOP/s."""
+    import torch
+
+    import wavefiber3d_amd as wf
+
+    g = os.path.join(ROOT, "tests", "golden")
+    x, y = wf.read_crosssection(os.path.join(g, "ellipse_x.txt"), os.path.join(g, "ellipse_y.txt"))
+    incs = [wf.Rough2dIncidence(wf.rough2d.wavelength(l), p) for p in "ME" for l in range(25)]
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    for _ in range(max(1, args.warmup)):
+        wf.rough2d_solve(xd, yd, incs)
+    I, wall_ms, dev_ms = _aux_timing(lambda: wf.rough2d_solve(xd, yd, incs), args.steps)
+    pairs = float(x.shape[0]) * 3600 * len(incs)
+    tflops = 48.0 * pairs / (dev_ms * 1e-3) / 1e12
+    res = {"metric": "2D cross-section PO sample-direction pairs/s (auxiliary workload)",
+           "value": pairs / (wall_ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "reference ellipse cross-section",
+           "config": {"workload": args.workload, "description": W["desc"], "n_samples": x.shape[0],
+                      "incidences": len(incs), "n_phi": 3600},
+           "roofline": {"bound": "mfma", "kernel": "rough2d call (currents + far field, FP64 "
+                        "vector ALU; HIP events around the call)", "achieved": tflops,
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                        "work_per_launch": 48.0 * pairs, "avg_launch_ms": dev_ms}}
+    if not args.no_cpu:
+        from oracle import oracle
+
+        got = I.cpu().numpy()
+        t0 = time.perf_counter()
+        errs = []
+        for i, t in enumerate(incs):
+            ref = oracle.rough2d(x, y, t.theta, t.phi, t.lambda_m, t.pol, t.eta)
+            errs.append(float(np.abs(got[i] - ref).max() / np.abs(ref).max()))
+        cs = time.perf_counter() - t0
+        res["parity_max_rel"] = max(errs)
+        res["cpu_baseline"] = {"value": pairs / cs, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"all {len(incs)} incidences, C oracle (one thread), "
+                                         f"{cs:.2f} s"}
+    print(f"[bench] rough2d: {wall_ms:.3f} ms/call wall, {dev_ms:.3f} ms device, "
+          f"{tflops:.2f} TFLOP/s, parity {res.get('parity_max_rel')}", file=sys.stderr, flush=True)
     print(json.dumps(res), flush=True)
     return 0
 
