@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 3
+#define WF_ABI_VERSION 4
 
 /* depth argument of wf_set_geometry: build the octree to the deepest useful
  * depth (about log8 n) and let each wavelength group of a solve use the
@@ -108,6 +108,11 @@ typedef struct wf_opts {
   int32_t use_graph;     /* 1: capture wf_solve into a hipGraph and replay it when the same
                             geometry, incidences, grid and outputs come back (needs a
                             non-null stream; the first call runs eagerly, then captures) */
+  int32_t solve_streams; /* wf_solve spreads its chunks of solves (wavelength groups, and
+                            the field chunks of one group) over this many streams, each
+                            with its own workspace, forked from and joined back into the
+                            caller's stream; results are identical to 1 (sequential).
+                            0 = default (4); graph capture and wf_autotune run sequentially */
 } wf_opts;
 
 typedef struct wf_handle wf_handle;

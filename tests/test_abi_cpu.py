@@ -31,11 +31,11 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_default_opts():
     L = wf.lib()
-    assert L.wf_abi_version() == 3
+    assert L.wf_abi_version() == 4
     o = _lib.Opts()
     L.wf_default_opts(ct.byref(o))
     assert o.struct_size == ct.sizeof(_lib.Opts)
-    assert o.device == -1 and o.beta == 4.0 and o.box_enlarge == pytest.approx(1.1)
+    assert o.device == -1 and o.beta == 4.0 and o.solve_streams == 0 and o.box_enlarge == pytest.approx(1.1)
 
 
 def _no_gpu():

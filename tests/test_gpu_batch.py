@@ -165,3 +165,29 @@ def test_batch_rejects_bad_grid_and_empty():
         b.add(wf.fibonacci_sphere(2000, 1e-6), 2)
         with pytest.raises(wf.WfError):
             b.solve(INCS, (32, 63))  # odd n_phi
+
+
+@pytest.mark.parametrize("field_chunk", [0, 12])
+def test_solve_streams_identical_to_sequential(field_chunk):
+    """wf_solve's concurrent chunks (opts.solve_streams) give bit-identical
+    results to the sequential order, on device and host outputs, and the same
+    launch counts in the merged statistics."""
+    s = wf.fibonacci_sphere(20000, 2e-6)
+    dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
+    incs = [wf.Incidence(0.3 * k, 0.5, 0.45e-6 + 0.02e-6 * (k // 2), "ME"[k % 2], 1.5 + 0.01j)
+            for k in range(12)]
+    grid = (36, 72)
+    out = {}
+    for K in (1, 3, 4):
+        print(f"case: solve_streams={K} field_chunk={field_chunk}", flush=True)
+        with wf.POSolver(device=0, solve_streams=K, field_chunk=field_chunk) as sv:
+            sv.set_geometry(dev, 3)
+            I = sv.solve(incs, grid, out="torch")
+            torch.cuda.synchronize()
+            st = sv.stats()
+            Ih = sv.solve(incs, grid)
+            out[K] = (I.cpu().numpy(), Ih, {k: v["launches"] for k, v in st["kernels"].items()})
+    for K in (3, 4):
+        assert np.array_equal(out[K][0], out[1][0])
+        assert np.array_equal(out[K][1], out[1][1])
+        assert out[K][2] == out[1][2]

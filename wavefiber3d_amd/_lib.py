@@ -88,6 +88,7 @@ class Opts(ct.Structure):
         ("cancel_flag", ct.POINTER(ct.c_int32)),
         ("field_chunk", ct.c_int32),
         ("use_graph", ct.c_int32),
+        ("solve_streams", ct.c_int32),
     ]
 
 

@@ -128,14 +128,17 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
   // with consecutive w read the translation table and write the parent grid
   // at consecutive addresses
   const uint32_t mag_nw = fmagic((uint32_t)nw);
-  acc_t acc[MAXS], acc2[MAXS];  // acc2: second sample fed by e = 0 (pole grids)
+  // acc2: the second parent sample fed by great-circle row e = 0 (pole
+  // grids).  Row 0 is slots s < nw, and nw <= W <= NT (launcher), so only
+  // slot m = 0 can hold it: one extra accumulator, not MAXS.
+  acc_t acc[MAXS], acc2;
 #pragma unroll
   for (int m = 0; m < MAXS; ++m) {
     acc[m].x = 0;
     acc[m].y = 0;
-    acc2[m].x = 0;
-    acc2[m].y = 0;
   }
+  acc2.x = 0;
+  acc2.y = 0;
   const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
   for (int c = c0; c < c1; ++c) {
     const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
@@ -163,10 +166,10 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
         const float2 tu = To ? cmul(To[(int64_t)e * half + k0 + w], u) : u;
         acc[m].x += tu.x;
         acc[m].y += tu.y;
-        if (dual && e == 0) {
+        if (m == 0 && dual && e == 0) {
           const float2 tu2 = To ? cmul(To[(int64_t)n2p * half + k0 + w], u) : u;
-          acc2[m].x += tu2.x;
-          acc2[m].y += tu2.y;
+          acc2.x += tu2.x;
+          acc2.y += tu2.y;
         }
       }
     }
@@ -191,12 +194,12 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
       else
         out[o] = make_float2((float)acc[m].x, (float)acc[m].y);
     }
-    if (dual && e == 0) {
+    if (m == 0 && dual && e == 0) {
       const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
       if (ROOT)
-        out_root[o2] = make_double2((double)acc2[m].x, (double)acc2[m].y);
+        out_root[o2] = make_double2((double)acc2.x, (double)acc2.y);
       else
-        out[o2] = make_float2((float)acc2[m].x, (float)acc2[m].y);
+        out[o2] = make_float2((float)acc2.x, (float)acc2.y);
     }
   }
 }
@@ -223,9 +226,23 @@ hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_
 #undef WF_THETA_FFT
 }
 
-// threads per block of the FFT kernels and the LDS they may use for lines
+// threads per block of the FFT kernels and the LDS they may use for lines.
+// Half the CU's 160 KB lets two 512-thread blocks share a CU, so one block's
+// FFT barriers overlap the other's work (fiber 663k root transition: -8 %
+// theta time against a 150 KB budget with one block per CU; 48 KB is slower
+// again, the lines per block get too few).
 int fft_threads() { return 512; }
-constexpr size_t kFftLds = 150 * 1024;
+constexpr size_t kFftLds = 75 * 1024;
+constexpr size_t kFftLdsMax = 150 * 1024;  // when 75 KB cannot hold the tables and one line pair
+
+// Lines of length lmax (two ping-pong buffers each) that fit beside the
+// tables: in kFftLds, or in kFftLdsMax when not even one fits there.
+int fft_lines(size_t tables, int lmax) {
+  const size_t per = 2 * sizeof(float2) * (size_t)lmax;
+  if (tables + per <= kFftLds) return (int)((kFftLds - tables) / per);
+  if (tables + per <= kFftLdsMax) return (int)((kFftLdsMax - tables) / per);
+  return 1;  // the launcher's 160 KB check rejects it
+}
 
 }  // namespace
 
@@ -239,7 +256,7 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
   if (lmax > kLineBuf || pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
   const int NT = fft_threads();
   const size_t tables = sizeof(float2) * ((size_t)pc.twn + pp.twn + npc + 1);
-  int RB = NT == 256 ? kLineBuf / lmax : (int)((kFftLds - tables) / (2 * sizeof(float2) * lmax));
+  int RB = NT == 256 ? kLineBuf / lmax : fft_lines(tables, lmax);
   RB = RB < 1 ? 1 : (RB > ntc ? ntc : RB);
   const size_t sh = sizeof(float2) * (size_t)2 * RB * lmax + tables;
   if (sh > 160 * 1024) return hipErrorInvalidValue;
@@ -270,8 +287,9 @@ hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st) {
   const int NT = fft_threads();
   if (lmax > kLineBuf || n2p > NT * 16) return hipErrorInvalidValue;
   const size_t tables = sizeof(float2) * ((size_t)a.pc.twn + a.pp.twn + n2c + 1);
-  int W = NT == 256 ? kLineBuf / lmax : (int)((kFftLds - tables) / (2 * sizeof(float2) * lmax));
+  int W = NT == 256 ? kLineBuf / lmax : fft_lines(tables, lmax);
   W = W < 1 ? 1 : (W > half ? half : W);
+  if (W > NT) W = NT;  // the kernel keeps row 0's second sample for slot 0 only
   const int max_slots = NT == 1024 ? 8 : 16;  // accumulator registers per thread
   while (W > 1 && W * n2p > NT * max_slots) --W;
   const int slots = (W * n2p + NT - 1) / NT;

@@ -144,6 +144,12 @@ struct wf_handle {
   uint64_t geom_gen = 0;
   uint64_t ws_gen = 1;  // bumped whenever a workspace buffer moves
   GraphCache gcache;
+  // concurrent chunks of wf_solve (opts.solve_streams): worker handles bound
+  // to this handle's geometry, each with its own workspace and stream
+  std::vector<wf_handle*> subs;
+  std::vector<hipStream_t> sub_st;
+  std::vector<hipEvent_t> sub_ev;  // [0]: fork on the caller's stream, [1 + k]: join of worker k
+  bool sequential = false;         // wf_autotune: clean per-kernel times
 };
 
 namespace {
@@ -505,6 +511,7 @@ void prof_reset(wf_handle* h) {
   h->ev_t0 = h->ev_t1 = -1;
   memset(&h->stats, 0, sizeof h->stats);
   h->stats_final = false;
+  for (wf_handle* w : h->subs) prof_reset(w);
 }
 
 // Runs `launch` (returning hipError_t) between two events and books it.
@@ -539,6 +546,16 @@ wf_status prof_finalize(wf_handle* h) {
     float ms = 0;
     WF_HIP(hipEventElapsedTime(&ms, h->evpool[h->ev_t0], h->evpool[h->ev_t1]));
     h->stats.ms_total = ms;
+  }
+  // launches the worker handles made for this handle's last solve
+  for (wf_handle* w : h->subs) {
+    WF_TRY(prof_finalize(w));
+    for (int k = 0; k < WF_K_COUNT; ++k) {
+      h->stats.kern_ms[k] += w->stats.kern_ms[k];
+      h->stats.kern_launches[k] += w->stats.kern_launches[k];
+      h->stats.kern_work[k] += w->stats.kern_work[k];
+    }
+    h->stats.leaf_pairs += w->stats.leaf_pairs;
   }
   h->stats_final = true;
   return WF_OK;
@@ -582,6 +599,8 @@ wf_status wf_create(wf_handle** out, const wf_opts* opts) {
     o = *opts;
   }
   if (!(o.beta >= 0 && o.beta < 100)) return set_err(WF_ERR_INVALID_ARG, "opts.beta out of range");
+  if (o.solve_streams < 0 || o.solve_streams > 16)
+    return set_err(WF_ERR_INVALID_ARG, "opts.solve_streams must be in [0, 16]");
   if (!(o.box_enlarge >= 1.0 && o.box_enlarge < 100))
     return set_err(WF_ERR_INVALID_ARG, "opts.box_enlarge must be >= 1");
   if (o.precision != WF_PREC_FP32 && o.precision != WF_PREC_BF16X3 && o.precision != WF_PREC_FP32_VALU)
@@ -623,6 +642,12 @@ void wf_destroy(wf_handle* h) {
   if (!h) return;
   hipSetDevice(h->device);
   hipDeviceSynchronize();
+  for (wf_handle* w : h->subs) {
+    w->has_geom = false;  // the geometry buffers belong to h
+    wf_destroy(w);
+  }
+  for (auto s : h->sub_st) hipStreamDestroy(s);
+  for (auto e : h->sub_ev) hipEventDestroy(e);
   for (auto& kv : h->bufs)
     if (kv.second.p) hipFree(kv.second.p);
   for (auto& kv : h->ffts)
@@ -1007,6 +1032,14 @@ struct GroupCtx {
   float2* Troot = nullptr;   // per-child table of that transition (root_from > 1)
 };
 
+// Which level-transition kernels can take child grid gcg -> parent grid gp:
+// the dense MFMA pair (phi_tile + theta_mfma) and/or the LDS FFT pair; with
+// neither, the older FFT kernels run (they read the plain translation tables).
+struct LevelPath {
+  bool mfma, fft2;
+};
+LevelPath level_path(const wf_handle* h, Grid gcg, Grid gp);
+
 double theta_origin(int l, int nt) {
   return (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * nt));
 }
@@ -1059,6 +1092,36 @@ int choose_depth(const wf_handle* h, double lam, int NT, int NP) {
   return best;
 }
 
+// Plain translation tables [level][8 octants][nt][np] (read by the older FFT
+// kernels and by the adjoint; the MFMA and LDS FFT transitions use the
+// extended tables).
+wf_status build_plain_tables(wf_handle* h, hipStream_t st, GroupCtx& gc, int depth) {
+  const int64_t tt = gc.toff[depth - 1] + 8LL * gc.g[depth - 1].nt * gc.g[depth - 1].np;
+  WF_TRY(dalloc_t(h, "trans", (size_t)tt, &gc.T));
+  for (int l = 0; l < depth; ++l) {
+    const double th0 =
+        (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * gc.g[l].nt));
+    const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
+                          0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
+                          0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
+    const Grid gl = gc.g[l];
+    float2* Tl = gc.T + gc.toff[l];
+    const double k0 = gc.k0;
+    WF_TRY(timed(h, st, WF_K_TRANS, 64.0 * gl.nt * gl.np,
+                 [&] { return wf::launch_trans_table(Tl, gl.nt, gl.np, th0, hc, k0, st); }));
+  }
+  return WF_OK;
+}
+
+LevelPath level_path(const wf_handle* h, Grid gcg, Grid gp) {
+  LevelPath p;
+  p.mfma = h->tune.agg_kernel == 0 && wf::gc_row_len(gcg.nt) <= wf::kThetaMfmaMaxK &&
+           gcg.np <= wf::kPhiMfmaMaxK && gcg.np % 2 == 0 && gp.np % 2 == 0;
+  p.fft2 = h->tune.agg_kernel == 0 && is_smooth5(gp.np) && is_smooth5(2 * gp.nt) &&
+           is_smooth5(gcg.np) && is_smooth5(2 * gcg.nt);
+  return p;
+}
+
 // Grids and translation tables of one wavelength group.  allow_skip: the
 // root may read the first level whose grid is no larger than the output grid
 // directly (forward solves; the adjoint keeps every level).
@@ -1080,19 +1143,7 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
     gc.toff[l] = tt;
     tt += 8LL * gc.g[l].nt * gc.g[l].np;
   }
-  WF_TRY(dalloc_t(h, "trans", (size_t)tt, &gc.T));
-  for (int l = 0; l < depth; ++l) {
-    const double th0 =
-        (l == 0) ? (-WF_PI / 2 + 1e-10) : (-WF_PI / 2 + WF_PI / (2.0 * gc.g[l].nt));
-    const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
-                          0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
-                          0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
-    const Grid gl = gc.g[l];
-    float2* Tl = gc.T + gc.toff[l];
-    const double k0 = gc.k0;
-    WF_TRY(timed(h, st, WF_K_TRANS, 64.0 * gl.nt * gl.np,
-                 [&] { return wf::launch_trans_table(Tl, gl.nt, gl.np, th0, hc, k0, st); }));
-  }
+  gc.T = nullptr;  // built below only when a kernel that reads it will run
   if (h->tune.agg_kernel == 0) {
     gc.txoff.assign(depth + 1, 0);
     int64_t tx = 0;
@@ -1150,6 +1201,15 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
       gc.root_from = r;
     }
   }
+  // the plain tables: for the adjoint, and for transitions that neither the
+  // MFMA nor the LDS FFT kernels take (the dispatch of forward_chunk)
+  bool need_plain = !allow_skip;
+  for (int l = depth - 1; l >= 0 && !need_plain; --l) {
+    if (l > 0 && l < gc.root_from) continue;
+    const LevelPath lp = level_path(h, gc.g[(l == 0) ? gc.root_from : l + 1], gc.g[l]);
+    if (!lp.mfma && !lp.fft2) need_plain = true;
+  }
+  if (need_plain) WF_TRY(build_plain_tables(h, st, gc, depth));
   for (int l = 0; l <= depth && l < 16; ++l) {
     h->stats.level_nt[l] = gc.g[l].nt;
     h->stats.level_np[l] = gc.g[l].np;
@@ -1264,11 +1324,9 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     float2* G = (l == 0) ? tb.G0 : tb.G;
     const double off_c = theta_origin(lc, gcg.nt), off_p = theta_origin(l, gp.nt);
     const int mirror = mirror_of(gp.nt, off_p);
-    const float2* Tl = gc.T + gc.toff[l];
-    const bool mfma_ok = h->tune.agg_kernel == 0 && wf::gc_row_len(gcg.nt) <= wf::kThetaMfmaMaxK &&
-                         gcg.np <= wf::kPhiMfmaMaxK && gcg.np % 2 == 0 && gp.np % 2 == 0;
-    const bool fft2_ok = h->tune.agg_kernel == 0 && is_smooth5(gp.np) && is_smooth5(2 * gp.nt) &&
-                         is_smooth5(gcg.np) && is_smooth5(2 * gcg.nt);
+    const float2* Tl = gc.T ? gc.T + gc.toff[l] : nullptr;
+    const LevelPath lpath = level_path(h, gcg, gp);
+    const bool mfma_ok = lpath.mfma, fft2_ok = lpath.fft2;
     if (fft2_ok && (2 * gp.nt > h->tune.agg_fft_rows || !mfma_ok)) {
       // large grids: LDS FFT (each sample read and written once)
       Fft2Plan f_pc, f_pp, f_tc, f_tp;
@@ -1442,63 +1500,155 @@ wf_status backward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const
 
 // The launch sequence of one wf_solve (replayable: it allocates only on first
 // use and records nothing but kernels while a graph is being captured).
+// One chunk of a solve: solves [c0, c0 + S) (in lambda order) of the
+// wavelength group [i0, i1), all at the group's depth.
+struct Chunk {
+  int i0, i1, c0, S;
+};
+
+// Runs chunks in order on handle w and stream st; a chunk whose group differs
+// from the previous one prepares that group's grids and tables first (in w's
+// workspace).
+wf_status run_chunks(wf_handle* w, hipStream_t st, const std::vector<Chunk>& cs,
+                     const wf_incidence* inc, int NT, int NP, const WfIncDev* dinc,
+                     const int* order, float* I_dev, float* E_dev) {
+  const int64_t n = w->n;
+  const int64_t D = (int64_t)NT * NP;
+  const float* px = w->pts;
+  const double centre[3] = {0.5 * (w->box[0] + w->box[3]), 0.5 * (w->box[1] + w->box[4]),
+                            0.5 * (w->box[2] + w->box[5])};
+  const double epi_bytes = (96.0 + 4.0 + (E_dev ? 24.0 : 0.0)) * D;
+  GroupCtx gc;
+  int cur_group = -1;
+  for (const Chunk& ch : cs) {
+    if (ch.i0 != cur_group) {
+      w->depth = choose_depth(w, inc[order[ch.i0]].lambda_m, NT, NP);
+      w->stats.depth = w->depth;
+      gc = GroupCtx();
+      WF_TRY(prepare_group(w, st, inc[order[ch.i0]].lambda_m, NT, NP, gc));
+      cur_group = ch.i0;
+    }
+    const int depth = w->depth;
+    const int c0 = ch.c0, S = ch.S;
+    const int F = 6 * S;
+    if (cancelled(w)) return set_err(WF_ERR_CANCELLED, "cancelled");
+    double2* F0 = nullptr;
+    WF_TRY(dalloc_t(w, "F0", (size_t)D * F, &F0));
+    if (depth == 0) {
+      double2 *C = nullptr, *part = nullptr;
+      WF_TRY(dalloc_t(w, "direct_C", (size_t)n * F, &C));
+      const int64_t part_elems = (int64_t)wf::direct_split(D) * F * D;
+      WF_TRY(dalloc_t(w, "direct_part", (size_t)part_elems, &part));
+      for (int q = 0; q < S; ++q)
+        WF_TRY(timed(w, st, WF_K_CURRENTS, (28.0 + 96.0) * n, [&] {
+          return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                     px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st);
+        }));
+      const double k0 = gc.k0;
+      WF_TRY(timed(w, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
+        return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0,
+                                      part, part_elems, st);
+      }));
+    } else {
+      TreeBufs tb;
+      WF_TRY(tree_buffers(w, gc, F, NP, tb));
+      WF_TRY(forward_chunk(w, st, gc, dinc, order, c0, S, NT, NP, tb, F0));
+    }
+    for (int q = 0; q < S; ++q) {
+      const int slot = order[c0 + q];
+      WF_TRY(timed(w, st, WF_K_EPILOGUE, epi_bytes, [&] {
+        return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
+                                    E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
+                                    nullptr, st);
+      }));
+    }
+  }
+  return WF_OK;
+}
+
+// Worker handles for concurrent chunks: created on first use with the
+// parent's options (sequential themselves), then bound to the parent's
+// geometry and tuning before every solve.
+wf_status ensure_workers(wf_handle* h, int K) {
+  while ((int)h->subs.size() < K - 1) {
+    wf_opts o = h->opts;
+    o.use_graph = 0;
+    o.solve_streams = 1;
+    wf_handle* w = nullptr;
+    WF_TRY(wf_create(&w, &o));
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+      wf_destroy(w);
+      return set_err(WF_ERR_HIP, "worker stream creation failed");
+    }
+    h->subs.push_back(w);
+    h->sub_st.push_back(s);
+  }
+  while ((int)h->sub_ev.size() < K) {
+    hipEvent_t e = nullptr;
+    WF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    h->sub_ev.push_back(e);
+  }
+  for (int k = 0; k < K - 1; ++k) {
+    wf_handle* w = h->subs[k];
+    w->n = h->n;
+    w->depth = h->depth;
+    w->depth_max = h->depth_max;
+    w->auto_depth = h->auto_depth;
+    w->force_depth = h->force_depth;
+    w->tune = h->tune;
+    w->tuned_gen = h->tuned_gen;
+    w->tuned_lam = h->tuned_lam;
+    w->tuned_depth = h->tuned_depth;
+    for (int q = 0; q < 6; ++q) {
+      w->boxf[q] = h->boxf[q];
+      w->box[q] = h->box[q];
+    }
+    w->pts = h->pts;
+    w->perm = h->perm;
+    w->keys = h->keys;
+    w->levels = h->levels;
+    w->geom_gen = h->geom_gen;
+    w->has_geom = true;
+  }
+  return WF_OK;
+}
+
 wf_status solve_core(wf_handle* h, hipStream_t st, const wf_incidence* inc, int n_inc, int NT,
                      int NP, const WfIncDev* dinc, const int* order, float* I_dev, float* E_dev) {
-  const int64_t n = h->n;
-  const int64_t D = (int64_t)NT * NP;
-  const float* px = h->pts;
-  const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
-                            0.5 * (h->box[2] + h->box[5])};
-  const double epi_bytes = (96.0 + 4.0 + (E_dev ? 24.0 : 0.0)) * D;
+  // chunks: per wavelength group, runs of at most max_solves solves
+  std::vector<Chunk> chunks;
   for (int i0 = 0; i0 < n_inc;) {
     int i1 = i0 + 1;
     while (i1 < n_inc && inc[order[i1]].lambda_m == inc[order[i0]].lambda_m) ++i1;
-    h->depth = choose_depth(h, inc[order[i0]].lambda_m, NT, NP);
-    h->stats.depth = h->depth;
-    const int depth = h->depth;
+    const int depth = choose_depth(h, inc[order[i0]].lambda_m, NT, NP);
     int max_solves = h->opts.field_chunk > 0 ? std::max(1, h->opts.field_chunk / 6)
                      : (h->tune.chunk_solves > 0 ? h->tune.chunk_solves : 8);
     max_solves = std::min(max_solves, 8);
     if (depth == 0) max_solves = std::min(max_solves, 4);
-    GroupCtx gc;
-    WF_TRY(prepare_group(h, st, inc[order[i0]].lambda_m, NT, NP, gc));
-    for (int c0 = i0; c0 < i1; c0 += max_solves) {
-      const int S = std::min(max_solves, i1 - c0);
-      const int F = 6 * S;
-      if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
-      double2* F0 = nullptr;
-      WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
-      if (depth == 0) {
-        double2 *C = nullptr, *part = nullptr;
-        WF_TRY(dalloc_t(h, "direct_C", (size_t)n * F, &C));
-        const int64_t part_elems = (int64_t)wf::direct_split(D) * F * D;
-        WF_TRY(dalloc_t(h, "direct_part", (size_t)part_elems, &part));
-        for (int q = 0; q < S; ++q)
-          WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 96.0) * n, [&] {
-            return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                       px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st);
-          }));
-        const double k0 = gc.k0;
-        WF_TRY(timed(h, st, WF_K_DIRECT, 48.0 * n * D * S, [&] {
-          return wf::launch_direct_fp64(px, px + n, px + 2 * n, n, C, F, centre, k0, NT, NP, F0,
-                                        part, part_elems, st);
-        }));
-      } else {
-        TreeBufs tb;
-        WF_TRY(tree_buffers(h, gc, F, NP, tb));
-        WF_TRY(forward_chunk(h, st, gc, dinc, order, c0, S, NT, NP, tb, F0));
-      }
-      for (int q = 0; q < S; ++q) {
-        const int slot = order[c0 + q];
-        WF_TRY(timed(h, st, WF_K_EPILOGUE, epi_bytes, [&] {
-          return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
-                                      E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
-                                      nullptr, st);
-        }));
-      }
-    }
+    for (int c0 = i0; c0 < i1; c0 += max_solves)
+      chunks.push_back(Chunk{i0, i1, c0, std::min(max_solves, i1 - c0)});
     i0 = i1;
   }
+  int K = h->opts.solve_streams > 0 ? h->opts.solve_streams : 4;
+  if (h->capturing || h->sequential || h->opts.use_graph) K = 1;  // graphs record one stream
+  K = std::min<int>(K, (int)chunks.size());
+  if (K <= 1) return run_chunks(h, st, chunks, inc, NT, NP, dinc, order, I_dev, E_dev);
+  WF_TRY(ensure_workers(h, K));
+  // chunk j goes to worker j mod K (worker 0 is h on the caller's stream);
+  // the workers start after everything the caller's stream holds so far
+  std::vector<std::vector<Chunk>> part(K);
+  for (size_t j = 0; j < chunks.size(); ++j) part[j % K].push_back(chunks[j]);
+  WF_HIP(hipEventRecord(h->sub_ev[0], st));
+  for (int k = 1; k < K; ++k) {
+    prof_reset(h->subs[k - 1]);  // a worker created by this call starts with no records
+    WF_HIP(hipStreamWaitEvent(h->sub_st[k - 1], h->sub_ev[0], 0));
+    WF_TRY(run_chunks(h->subs[k - 1], h->sub_st[k - 1], part[k], inc, NT, NP, dinc, order, I_dev,
+                      E_dev));
+    WF_HIP(hipEventRecord(h->sub_ev[k], h->sub_st[k - 1]));
+  }
+  WF_TRY(run_chunks(h, st, part[0], inc, NT, NP, dinc, order, I_dev, E_dev));
+  for (int k = 1; k < K; ++k) WF_HIP(hipStreamWaitEvent(st, h->sub_ev[k], 0));
   return WF_OK;
 }
 
@@ -1813,6 +1963,7 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
   // eager solve at a forced depth: one untimed pass (allocations), one timed
   auto run = [&](int d, Run& r) -> wf_status {
     h->force_depth = d;
+    h->sequential = true;  // per-kernel times without other streams' launches beside them
     for (int rep = 0; rep < 2; ++rep) {
       prof_reset(h);
       wf_status e = prof_event(h, st, &h->ev_t0);
@@ -1821,10 +1972,12 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
       if (e == WF_OK) e = prof_finalize(h);
       if (e != WF_OK) {
         h->force_depth = -1;
+        h->sequential = false;
         return e;
       }
     }
     h->force_depth = -1;
+    h->sequential = false;
     const double* k = h->stats.kern_ms;
     r.d = d;
     r.total = 1e-3 * h->stats.ms_total;
@@ -1986,6 +2139,7 @@ wf_status wf_batch_create(wf_batch** out, const wf_opts* opts, int32_t n_streams
       wf_batch_destroy(b);
       return e;
     }
+    h->opts.solve_streams = 1;  // the batch's own streams give the concurrency
     b->workers.push_back(h);
     if (w == 0) {
       b->opts = h->opts;

@@ -126,7 +126,8 @@ def _stream_ptr(arrays) -> int | None:
 def _make_opts(device: int = -1, precision: str = "bf16x3", beta: float = 4.0, box=None,
                box_enlarge: float = 1.1, lambda_sample_m: float = 0.0, strict_grid: bool = False,
                workspace_cap_bytes: int = 0, field_chunk: int = 0,
-               cancel_flag: ct.c_int32 | None = None, use_graph: bool = False) -> _lib.Opts:
+               cancel_flag: ct.c_int32 | None = None, use_graph: bool = False,
+               solve_streams: int = 0) -> _lib.Opts:
     o = _lib.Opts()
     lib().wf_default_opts(ct.byref(o))
     o.device = device
@@ -139,6 +140,7 @@ def _make_opts(device: int = -1, precision: str = "bf16x3", beta: float = 4.0, b
     o.workspace_cap_bytes = workspace_cap_bytes
     o.field_chunk = field_chunk
     o.use_graph = int(use_graph)
+    o.solve_streams = int(solve_streams)
     if box is not None:
         o.has_box = 1
         for i in range(6):
@@ -181,9 +183,9 @@ class POSolver:
                  box=None, box_enlarge: float = 1.1, lambda_sample_m: float = 0.0,
                  strict_grid: bool = False, workspace_cap_bytes: int = 0,
                  field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None,
-                 use_graph: bool = False):
+                 use_graph: bool = False, solve_streams: int = 0):
         o = _make_opts(device, precision, beta, box, box_enlarge, lambda_sample_m, strict_grid,
-                       workspace_cap_bytes, field_chunk, cancel_flag, use_graph)
+                       workspace_cap_bytes, field_chunk, cancel_flag, use_graph, solve_streams)
         self._cancel = cancel_flag
         self._h = ct.c_void_p()
         check(lib().wf_create(ct.byref(self._h), ct.byref(o)))
