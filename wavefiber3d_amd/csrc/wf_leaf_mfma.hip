@@ -193,10 +193,15 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_kernel(wf::LeafArgs a, i
 // give all four (the other two are their conjugates).  Lane l owns quad
 // tile*16 + l%16; the four directions are its four MFMA B operands.
 template <int MT>
-__global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs a, int KS) {
+__global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs a, int KS, int RB) {
   extern __shared__ float4 smem4[];
   float4* spt = smem4;
   bf16x8* scur = reinterpret_cast<bf16x8*>(smem4 + KS * 16);
+  // e^{iB} of every (direction row of this block, staged point): B = k s_z z
+  // depends on the direction only through its row, so it is formed once per
+  // (row, point) here instead of once per (direction, point) in the lanes
+  float2* seb = reinterpret_cast<float2*>(scur + KS * MT * 128);  // [RB][KS*16]
+  float* sbz = reinterpret_cast<float*>(seb + RB * KS * 16);       // [RB] sin(theta_row)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
   const int D = a.nt * a.np;
   const int nth = (a.nt + 1) / 2, nph = a.np / 2;
@@ -211,13 +216,23 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
   const int quad = (sup * 4 + wave) * 16 + col;
   const bool qvalid = quad < nquad;
   const int j = qvalid ? quad / nph : 0, k = qvalid ? quad - (quad / nph) * nph : 0;
+  // rows of the block's 64 quads: jlo .. jlo + nrow - 1 (nrow <= RB, launcher)
+  const int jlo = (sup * 64) / nph;
+  const int nrow = min(nth - 1, (sup * 64 + 63) / nph) - jlo + 1;
+  const int jrow = qvalid ? j - jlo : 0;
+  for (int r = threadIdx.x; r < nrow; r += kThreads) {
+    double st_, ct_;
+    sincos(-WF_PI / 2 + WF_PI * (jlo + r + 0.5) / a.nt, &st_, &ct_);
+    sbz[r] = (float)st_;
+  }
   const bool mirror_row = (a.nt - 1 - j) != j;
   int dd[4];
   dd[0] = j * a.np + k;
   dd[1] = j * a.np + k + nph;
   dd[2] = (a.nt - 1 - j) * a.np + k;
   dd[3] = (a.nt - 1 - j) * a.np + k + nph;
-  float ax = 0.f, ay = 0.f, bz = 0.f;  // A = ax x + ay y, B = bz z (revolutions per metre scaled)
+  // A = ax x + ay y (revolutions per metre scaled); B = s_z z comes from seb
+  float ax = 0.f, ay = 0.f;
   if (qvalid) {
     const double th = -WF_PI / 2 + WF_PI * (j + 0.5) / a.nt;
     const double ph = WF_TWO_PI * k / a.np;
@@ -226,7 +241,6 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
     sincos(ph, &sp, &cp);
     ax = (float)(ct * cp);
     ay = (float)(ct * sp);
-    bz = (float)st;
   }
   floatx4 acc[4][MT];
 #pragma unroll
@@ -239,6 +253,13 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
     const int ksteps = (cnt + 15) / 16;
     __syncthreads();
     stage_points(a, base, cnt, ksteps * 16, c, k_rev, spt);
+    // the same float z the staging forms, then the lanes' B phase and sincos
+    for (int t = threadIdx.x; t < nrow * ksteps * 16; t += kThreads) {
+      const int r = t / (ksteps * 16), p = t - r * (ksteps * 16);
+      const float z = (p < cnt) ? (float)(((double)a.pz[base + p] - c[2]) * k_rev) : 0.f;
+      const float fb = rev_reduce(sbz[r] * z);
+      seb[r * chunk + p] = make_float2(__builtin_amdgcn_cosf(fb), __builtin_amdgcn_sinf(fb));
+    }
     for (int idx = threadIdx.x; idx < ksteps * MT * 64; idx += kThreads) {
       const int ln = idx & 63, mt = (idx >> 6) % MT, ks = idx / (64 * MT);
       const int cc = ln & 15, gg = ln >> 4;
@@ -267,9 +288,9 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
       for (int q = 0; q < 4; ++q) {
         const float4 pt = spt[ks * 16 + 4 * g + q];
         const float fa = rev_reduce(ax * pt.x + ay * pt.y);
-        const float fb = rev_reduce(bz * pt.z);
         const float cA = __builtin_amdgcn_cosf(fa), sA = __builtin_amdgcn_sinf(fa);
-        const float cB = __builtin_amdgcn_cosf(fb), sB = __builtin_amdgcn_sinf(fb);
+        const float2 eB = seb[jrow * chunk + ks * 16 + 4 * g + q];
+        const float cB = eB.x, sB = eB.y;
         const float cc_ = cA * cB, ss = sA * sB, sc = sA * cB, cs = cA * sB;
         const float p0c = cc_ - ss, p0s = sc + cs;  // e^{i(A+B)}
         const float p2c = cc_ + ss, p2s = sc - cs;  // e^{i(A-B)}
@@ -420,9 +441,14 @@ hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
     const size_t per_ks = (size_t)MT * 128 * sizeof(bf16x8);
     int KS = (int)((48 * 1024) / per_ks);
     KS = KS < 1 ? 1 : (KS > 8 ? 8 : KS);
-    const size_t sh = (size_t)KS * 16 * sizeof(float4) + (size_t)KS * per_ks;
+    // direction rows one block's 64 consecutive quads can touch
+    const int nph = a.np / 2;
+    const int RB = 63 / nph + 2;
+    const size_t sh = (size_t)KS * 16 * sizeof(float4) + (size_t)KS * per_ks +
+                      (size_t)RB * KS * 16 * sizeof(float2) + (size_t)RB * sizeof(float);
+    if (sh > 64 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL((leaf_bf16x3_quad_kernel<MT>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
-                       a, KS);
+                       a, KS, RB);
     return hipGetLastError();
   }
   const int dtiles = (D + 15) / 16;
