@@ -303,6 +303,24 @@ def test_trig_interp_exact_on_band_limited():
     assert np.abs(out - f(2 * np.pi * np.arange(np_) / np_)).max() < 1e-12
 
 
+@pytest.mark.parametrize("nc, nup", [(9, 24), (12, 30), (15, 15)])
+def test_trig_interp_up_then_down_identity(nc, nup):
+    """Resampling an arbitrary nc-sample line onto a finer grid (nup >= nc) and
+    back recovers it (the finer grid holds every frequency of the coarse one),
+    including a shifted origin; the round trip through a coarser grid does not
+    (it drops frequencies), so the identity is not trivially met."""
+    rng = np.random.default_rng(nc)
+    v = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    off_c, off_u = 0.37, -0.21
+    up = O.trig_interp(v, off_c, nup, off_u)
+    back = O.trig_interp(up, off_u, nc, off_c)
+    assert np.abs(back - v).max() < 1e-12 * np.abs(v).max()
+    if nc > 4:
+        down = O.trig_interp(v, off_c, nc - 3, off_u)
+        again = O.trig_interp(down, off_u, nc, off_c)
+        assert np.abs(again - v).max() > 1e-3 * np.abs(v).max()
+
+
 def test_sphere_interp_point_sources():
     """Pattern of sources within radius R is band-limited (degree ~ kR): the 2 pi
     theta extension + phi interpolation must reproduce it on a new grid."""
