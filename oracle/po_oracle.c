@@ -24,7 +24,7 @@ static const double PI = 3.14159265358979323846;
 
 double orc_z0(void) { return sqrt(ORC_MU0 / ORC_EPS0); }
 
-/* ---------------------------------------------------------------- A2 ---- */
+/* --------------------------------------------------------- Fresnel --- */
 /* util/util.h:365-368 cos_internal_angle(n, t) = sqrt(1 - sin(t)/n * sin(t)/n)
  * (principal branch) and :370-382 rs/rp with n0 = eta_in = 1
  * (computeLocalTangentPlane is called with comThr(1,0), treePO.cu:604). */
@@ -50,7 +50,7 @@ void orc_fresnel(double theta_loc, double eta_re, double eta_im, double rs[2], d
   rp[0] = creal(b); rp[1] = cimag(b);
 }
 
-/* ---------------------------------------------------------------- A1 ---- */
+/* ------------------------------ incidence basis, radiation factors --- */
 /* fibersim/treePO.cu:580-601 (and compareMie/sphere.cu:557-579). */
 void orc_incidence_basis(double ti, double pi_, char pol, double w[3], double E[3], double H[3]) {
   w[0] = cos(ti) * cos(pi_);
@@ -80,7 +80,7 @@ void orc_radiation_factors(double lambda_m, double Ef[2], double Hf[2]) {
   Hf[0] = creal(hf); Hf[1] = cimag(hf);
 }
 
-/* ---------------------------------------------------------------- A3 ---- */
+/* -------------------------------------------------------- currents --- */
 static void cross_rr(const double a[3], const double b[3], double r[3]) {
   /* util.h:338-344 component formula */
   r[0] = a[1] * b[2] - b[1] * a[2];
@@ -180,7 +180,7 @@ void orc_currents(int64_t n, const float* x, const float* y, const float* z,
   }
 }
 
-/* ---------------------------------------------------------------- A8 ---- */
+/* ----------------------------------------------------- output grid --- */
 void orc_output_dirs(int ntheta, int nphi, double* dirs) {
   for (int j = 0; j < ntheta; ++j) {
     double t = -PI / 2 + 1e-10 + PI / ntheta * j; /* treePO.cu:380 */
@@ -194,7 +194,7 @@ void orc_output_dirs(int ntheta, int nphi, double* dirs) {
   }
 }
 
-/* ------------------------------------------------------------ A10/A14 --- */
+/* ------------------------------------------------------ direct sum --- */
 void orc_direct_sum(int64_t n, const float* x, const float* y, const float* z,
                     const double centre[3], double k, int S, const double* C,
                     int64_t ndir, const double* dirs, double* F, int nthreads) {
@@ -280,7 +280,7 @@ void orc_direct_sum_finite(int64_t n, const float* x, const float* y, const floa
   }
 }
 
-/* ---------------------------------------------------------------- A13 --- */
+/* ------------------------------------------------------- intensity --- */
 void orc_intensity(int64_t ndir, const double* dirs, const double* F, double* Iout, double* E) {
   const double Z0 = orc_z0();
   for (int64_t d = 0; d < ndir; ++d) {
@@ -359,7 +359,7 @@ int orc_solve_direct(int64_t n, const float* x, const float* y, const float* z,
   return rc;
 }
 
-/* ---------------------------------------------------------------- A5 ---- */
+/* ---------------------------------------------------------- octree --- */
 /* util/tree.h:289-715 restated: node = nodes[level offset + id]; children of id
  * m are 8m..8m+7 at the next level (tree.h:647-662); the centre is
  * 0.5f*(min+max) in float (tree.h:116-120); octants in the order of the eight
@@ -431,7 +431,7 @@ int orc_octree(int64_t n, const float* x, const float* y, const float* z, const 
   return 0;
 }
 
-/* ---------------------------------------------------------------- A7 ---- */
+/* -------------------------------------------------- level sampling --- */
 int orc_ref_level_count(int level, float extent, float beta) {
   /* treePO.cu:346-351: float d = pow(0.5, i) * extent; ceil(2 pi/0.4e-6 d + beta (..)^(1/3)) */
   float d = (float)(pow(0.5, level) * extent);
@@ -439,7 +439,7 @@ int orc_ref_level_count(int level, float extent, float beta) {
   return (int)ceil(kd + beta * pow(kd, 1.0 / 3.0));
 }
 
-/* --------------------------------------------------------------- A11 ---- */
+/* --------------------------------------------------- interpolation --- */
 /* Coefficients of the trigonometric interpolant (interpolation.h:17-37 and
  * :47-68: forward DFT, spectral zero-pad, Nyquist split when n is even). */
 static void trig_coeffs(int nc, const double* in, cplx* c) {
@@ -596,7 +596,7 @@ void orc_average_grid(const float* in, int nt, int np, int nto, int npo, float* 
   free(tmp);
 }
 
-/* ---------------------------------------------------------------- F4 ---- */
+/* ------------------------------------------------ 2D cross-section --- */
 /* compareBEM2d/rough2d.cu:237-362 (current_fresnel_oblique, eta_in = 1), the
  * sample lengths :477-490, the 2D Green's function :58-76, the sum :520-560 and
  * the projection and normalisation :562-577, for one incidence. */

@@ -17,7 +17,7 @@ LAM = 0.5e-6
 K = 2 * np.pi / LAM
 
 
-# ------------------------------------------------------------------ A2 -----
+# ------------------------------------------------------------- Fresnel -----
 @pytest.mark.parametrize("eta", [1.55, 1.55 + 0.1j, 1.3 + 0.02j, 2.4 + 1.0j])
 def test_fresnel_textbook(eta):
     for th in np.linspace(0, np.pi / 2 - 1e-6, 23):
@@ -44,7 +44,7 @@ def test_fresnel_special_angles():
     assert abs(rs + 1) < 1e-12 and abs(rp + 1) < 1e-12
 
 
-# ------------------------------------------------------------ A1/A3/A4 -----
+# ---------------------------------------------- incidence and currents -----
 def test_incidence_basis_orthonormal():
     for th, ph in [(0.3, 1.0), (-1.2, 4.0), (np.pi / 2, 0.0)]:
         for pol in "ME":
@@ -162,7 +162,7 @@ def test_beam_taper():
     assert np.abs(C1 - C0 * g[:, None]).max() < 1e-13 * np.abs(C0).max()
 
 
-# ----------------------------------------------------------- A10/A14 -------
+# ---------------------------------------------------------- direct sum -----
 def test_direct_sum_dirichlet_lattice():
     """m x m lattice, constant current: F = C D_m(k s_x h) D_m(k s_y h)."""
     m, h = 11, 0.37e-6
@@ -197,7 +197,7 @@ def test_direct_sum_translation_identity():
     assert np.abs(F1 - F0 * ph[:, None]).max() < 1e-12 * np.abs(F0).max()
 
 
-# ---------------------------------------------------------------- A13 ------
+# ----------------------------------------------------------- intensity -----
 def test_intensity_dipoles():
     dirs = O.output_dirs(9, 18)
     z0 = O.z0()
@@ -214,7 +214,7 @@ def test_intensity_dipoles():
     assert np.abs(np.einsum("cd,dc->d", E, dirs)).max() < 1e-9
 
 
-# ---------------------------------------------------------------- A5/A6 ----
+# -------------------------------------------------------------- octree -----
 @pytest.mark.parametrize("depth", [0, 1, 2, 4])
 def test_octree_partition_invariants_and_sort_restatement(depth):
     rng = np.random.default_rng(depth + 11)
@@ -259,7 +259,7 @@ def test_octree_partition_invariants_and_sort_restatement(depth):
         assert np.all((q >= bx[:3]) & (q <= bx[3:]))
 
 
-# ------------------------------------------------------------------ A7 -----
+# ------------------------------------------------------ level sampling -----
 def test_reference_level_counts_appendix_b():
     # sphere r = 10 um, rho = 1.3 (sphere.cu:346-350): extent 2 r rho
     assert [O.ref_level_count(i, np.float32(2 * 10e-6 * 1.3)) for i in range(1, 7)] == \
@@ -271,7 +271,7 @@ def test_reference_level_counts_appendix_b():
         [809, 421, 223, 122, 69, 41]
 
 
-# ----------------------------------------------------------------- A11 -----
+# ------------------------------------------------------- interpolation -----
 @pytest.mark.parametrize("nc,np_", [(7, 16), (8, 15), (12, 12), (16, 9), (9, 4)])
 def test_trig_interp_vs_numpy_fft(nc, np_):
     rng = np.random.default_rng(nc * 31 + np_)

@@ -1,4 +1,4 @@
-"""Reference-compatible command-line driver (row F2).
+"""Reference-compatible command-line drivers.
 
 ``python -m wavefiber3d_amd.cli treepo LSTART LEND THETAI PHII FIBER DEPTH START END``
 takes the positional arguments of fibersim/treePO.cu:27-34 with the same

@@ -1,4 +1,4 @@
-// wf_adjoint.hip -- reverse mode of the PO solve (row A16; no reference
+// wf_adjoint.hip -- reverse mode of the PO solve (no reference
 // counterpart).
 //
 // The solve is F = A C (currents C -> far-field components F, complex linear

@@ -1,5 +1,5 @@
 // wf_agg_fft.hip -- level transitions with large angular grids through the
-// LDS FFT of wf_fft2.h (rows A9, A11, A12).
+// LDS FFT of wf_fft2.h.
 //
 // For long lines the dense interpolation GEMM has to re-read either the data
 // or its (rows x columns) matrix many times; the FFT route reads each child

@@ -1,5 +1,5 @@
 // wf_agg_mfma.hip -- one level transition of the tree aggregation as dense
-// interpolation GEMMs on the MFMA units (rows A9, A11, A12).
+// interpolation GEMMs on the MFMA units.
 //
 // The reference interpolates with cuFFT plans per child and component
 // (interpolation.h:11-92), translates (fields.h:92-103) and sums children

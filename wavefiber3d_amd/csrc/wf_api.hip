@@ -1,5 +1,5 @@
 // wf_api.hip -- C-ABI of the MI355X PO solver: handle, geometry/octree cache,
-// per-wavelength solve orchestration (row B; drivers treePO.cu:21-992,
+// per-wavelength solve orchestration (drivers treePO.cu:21-992,
 // sphere.cu:21-966 restructured as a library).
 #include <hip/hip_runtime.h>
 
@@ -436,7 +436,7 @@ struct Grid {
   int nt, np;
 };
 
-// Level sampling (row A7).  The reference rule (treePO.cu:342-375) is
+// Level sampling.  The reference rule (treePO.cu:342-375) is
 // N = ceil(k d + beta (k d)^(1/3)) with d = 0.5^i * (extent * rho).  Here the
 // extent is the node's own bound: theta samples the great circle through the
 // poles (band limit k R, R = node half-diagonal, 2 pi extension -> Nt >= L),

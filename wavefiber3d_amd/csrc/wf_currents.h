@@ -1,4 +1,4 @@
-// wf_currents.h -- per-point tangent-plane Fresnel currents (rows A2-A4) as a
+// wf_currents.h -- per-point tangent-plane Fresnel currents as a
 // device function shared by the currents kernel and the adjoint.
 // Restates util/tangentplane.h:18-80, util/util.h:365-382 (n0 = 1) and
 // util/fields.h:106-150 in FP64.

@@ -1,4 +1,4 @@
-// wf_currents.hip -- tangent-plane Fresnel surface currents (rows A2-A4).
+// wf_currents.hip -- tangent-plane Fresnel surface currents.
 //
 // Restates util/tangentplane.h:18-80 (LocalTangentPlane), util/util.h:365-382
 // (cos_internal_angle, rs, rp with n0 = 1) and fuses util/fields.h:106-126

@@ -1,5 +1,5 @@
-// wf_direct.hip -- depth-0 "plain method" radiation sum in FP64 (row A14) and
-// the far-field intensity epilogue (row A13).
+// wf_direct.hip -- depth-0 "plain method" radiation sum in FP64 and
+// the far-field intensity epilogue.
 //
 // Direct sum: util/fields.h:8-38 (current_green_center) summed over all points
 // (treePO.cu:652-670 with depth 0, sphere.cu:592-658): for every output

@@ -1,4 +1,4 @@
-// wf_geometry.hip -- fiber heightfield -> surface samples (row F1; the
+// wf_geometry.hip -- fiber heightfield -> surface samples (the
 // reference's set_sourcepoint, util/geometry.h:78-178, and its windowless
 // form util/geometry.h:3-72).
 //

@@ -1,4 +1,4 @@
-// wf_grid.hip -- block average of output grids (row F2; the reference's
+// wf_grid.hip -- block average of output grids (the reference's
 // average_cpu, util/fields.h:206-234).
 //
 // Output sample (i, j) of grid g averages rows i*rt .. i*rt + rt - 1 and

@@ -1,4 +1,4 @@
-// wf_interp.hip -- upward aggregation of the PO tree (rows A9, A11, A12).
+// wf_interp.hip -- upward aggregation of the PO tree.
 //
 // Reference per (parent, active child, component): copyandinterp
 // (interpolation.h:82-92: fill, copy, 4 cuFFT C2C, 2 (Nt_c + Np_p) row

@@ -1,4 +1,4 @@
-// wf_leaf.hip -- leaf radiation sum (row A10).
+// wf_leaf.hip -- leaf radiation sum.
 //
 // Reference: current_times_green_center (util/fields.h:41-50, functor
 // :8-38) evaluates C_n exp(+i k s.(x_n - c_leaf(n))) over ALL N points for

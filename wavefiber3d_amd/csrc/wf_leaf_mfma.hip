@@ -1,4 +1,4 @@
-// wf_leaf_mfma.hip -- leaf radiation sum on the MFMA units (row A10).
+// wf_leaf_mfma.hip -- leaf radiation sum on the MFMA units.
 //
 // Per leaf L the pattern is the complex GEMM Y[d][f] = sum_n P[d][n] C[n][f],
 // P[d][n] = exp(+i k s_d.(x_n - c_L)) (util/fields.h:8-38 summed per leaf as

@@ -1,4 +1,4 @@
-// wf_octree.hip -- level-synchronous octree build (rows A5, A6).
+// wf_octree.hip -- level-synchronous octree build.
 //
 // Replaces the recursive CUDA-dynamic-parallelism build_octree_kernel
 // (util/tree.h:289-715, (8^d-1)/7 device launches of one 256-thread block per

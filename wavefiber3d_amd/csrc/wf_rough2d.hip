@@ -1,4 +1,4 @@
-// wf_rough2d.hip -- PO of a 2D cross-section (row F4; the reference's
+// wf_rough2d.hip -- PO of a 2D cross-section (the reference's
 // compareBEM2d/rough2d.cu:237-585).
 //
 // The surface is a closed polyline (x_i, y_i), i = 0 .. n-1, traversed
