@@ -191,3 +191,33 @@ def test_solve_streams_identical_to_sequential(field_chunk):
         assert np.array_equal(out[K][0], out[1][0])
         assert np.array_equal(out[K][1], out[1][1])
         assert out[K][2] == out[1][2]
+
+
+def test_solve_streams_cancel_then_resume():
+    """A cancellation flag set by the host stops a multi-stream solve with
+    WF_ERR_CANCELLED (every worker stops at its next chunk, the caller's stream
+    still joins them); cleared, the next solve on the same handle matches a
+    sequential handle bit for bit, complex E included."""
+    import ctypes as ct
+
+    s = wf.fibonacci_sphere(12000, 1.5e-6)
+    dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
+    incs = [wf.Incidence(0.2 * k, 0.1, 0.5e-6 + 0.03e-6 * (k // 2), "ME"[k % 2], 1.5 + 0.0j)
+            for k in range(8)]
+    grid = (30, 60)
+    flag = ct.c_int32(1)
+    print("case: cancelled multi-stream solve", flush=True)
+    with wf.POSolver(device=0, solve_streams=4, cancel_flag=flag) as sv:
+        sv.set_geometry(dev, 2)
+        with pytest.raises(wf.WfError, match="CANCELLED"):
+            sv.solve(incs, grid, out="torch")
+        torch.cuda.synchronize()
+        flag.value = 0
+        I, E = sv.solve(incs, grid, return_E=True, out="torch")
+        I, E = I.cpu().numpy(), E.cpu().numpy()
+    with wf.POSolver(device=0, solve_streams=1) as sq:
+        sq.set_geometry(dev, 2)
+        I1, E1 = sq.solve(incs, grid, return_E=True, out="torch")
+        I1, E1 = I1.cpu().numpy(), E1.cpu().numpy()
+    assert np.array_equal(I, I1)
+    assert np.array_equal(E, E1)

@@ -1640,16 +1640,28 @@ wf_status solve_core(wf_handle* h, hipStream_t st, const wf_incidence* inc, int 
   std::vector<std::vector<Chunk>> part(K);
   for (size_t j = 0; j < chunks.size(); ++j) part[j % K].push_back(chunks[j]);
   WF_HIP(hipEventRecord(h->sub_ev[0], st));
-  for (int k = 1; k < K; ++k) {
+  // the caller's stream waits for every worker that was given work, also when
+  // a worker or the handle itself stops early (error, cancellation)
+  wf_status err = WF_OK;
+  int started = 0;
+  for (int k = 1; k < K && err == WF_OK; ++k) {
     prof_reset(h->subs[k - 1]);  // a worker created by this call starts with no records
-    WF_HIP(hipStreamWaitEvent(h->sub_st[k - 1], h->sub_ev[0], 0));
-    WF_TRY(run_chunks(h->subs[k - 1], h->sub_st[k - 1], part[k], inc, NT, NP, dinc, order, I_dev,
-                      E_dev));
-    WF_HIP(hipEventRecord(h->sub_ev[k], h->sub_st[k - 1]));
+    if (hipStreamWaitEvent(h->sub_st[k - 1], h->sub_ev[0], 0) != hipSuccess) {
+      err = set_err(WF_ERR_HIP, "worker stream wait failed");
+      break;
+    }
+    started = k;
+    err = run_chunks(h->subs[k - 1], h->sub_st[k - 1], part[k], inc, NT, NP, dinc, order, I_dev,
+                     E_dev);
+    if (hipEventRecord(h->sub_ev[k], h->sub_st[k - 1]) != hipSuccess && err == WF_OK)
+      err = set_err(WF_ERR_HIP, "worker join event failed");
   }
-  WF_TRY(run_chunks(h, st, part[0], inc, NT, NP, dinc, order, I_dev, E_dev));
-  for (int k = 1; k < K; ++k) WF_HIP(hipStreamWaitEvent(st, h->sub_ev[k], 0));
-  return WF_OK;
+  if (err == WF_OK) err = run_chunks(h, st, part[0], inc, NT, NP, dinc, order, I_dev, E_dev);
+  for (int k = 1; k <= started; ++k) {
+    const hipError_t e = hipStreamWaitEvent(st, h->sub_ev[k], 0);
+    if (e != hipSuccess && err == WF_OK) err = set_err(WF_ERR_HIP, "join wait failed");
+  }
+  return err;
 }
 
 }  // namespace
