@@ -112,7 +112,9 @@ typedef struct wf_opts {
                             the field chunks of one group) over this many streams, each
                             with its own workspace, forked from and joined back into the
                             caller's stream; results are identical to 1 (sequential).
-                            0 = default (4); graph capture and wf_autotune run sequentially */
+                            0 = default (4); graph capture and wf_autotune run sequentially.
+                            Each extra stream keeps a workspace of its own (as large as the
+                            handle's; workspace_cap_bytes applies to each) */
 } wf_opts;
 
 typedef struct wf_handle wf_handle;
