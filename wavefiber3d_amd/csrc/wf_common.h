@@ -39,12 +39,41 @@ __device__ inline float2 cis_rev(double rev) {
   return make_float2(c, s);
 }
 
-// FP64 e^{2 pi i rev} with the same exact reduction.
+// FP64 e^{2 pi i rev} with the same exact reduction, then to the nearest
+// quarter turn: f = q/4 + r with |r| <= 1/8 (exact, Sterbenz), t = 2 pi r in
+// [-pi/4, pi/4], and the Taylor series of sin t to t^17 and cos t to t^16
+// (truncation below 1e-17 there), evaluated by Horner in t^2; the quarter
+// turn then rotates (cos t, sin t) by i^q.  No branches and no further range
+// reduction, unlike a general-purpose sincos.
 __device__ inline double2 zcis_rev(double rev) {
-  double f = rev - rint(rev);
-  double s, c;
-  sincos(f * WF_TWO_PI, &s, &c);
-  return make_double2(c, s);
+  const double f = rev - rint(rev);
+  const double q = rint(4.0 * f);
+  const double t = (f - 0.25 * q) * WF_TWO_PI;
+  const double t2 = t * t;
+  double sp = 1.0 / 355687428096000.0;  // 1/17!
+  sp = sp * t2 - 1.0 / 1307674368000.0;
+  sp = sp * t2 + 1.0 / 6227020800.0;
+  sp = sp * t2 - 1.0 / 39916800.0;
+  sp = sp * t2 + 1.0 / 362880.0;
+  sp = sp * t2 - 1.0 / 5040.0;
+  sp = sp * t2 + 1.0 / 120.0;
+  sp = sp * t2 - 1.0 / 6.0;
+  const double s = t + t * (t2 * sp);
+  double cp = 1.0 / 20922789888000.0;  // 1/16!
+  cp = cp * t2 - 1.0 / 87178291200.0;
+  cp = cp * t2 + 1.0 / 479001600.0;
+  cp = cp * t2 - 1.0 / 3628800.0;
+  cp = cp * t2 + 1.0 / 40320.0;
+  cp = cp * t2 - 1.0 / 720.0;
+  cp = cp * t2 + 1.0 / 24.0;
+  cp = cp * t2 - 0.5;
+  const double c = 1.0 + t2 * cp;
+  switch ((int)q & 3) {
+    case 1: return make_double2(-s, c);
+    case 2: return make_double2(-c, -s);
+    case 3: return make_double2(s, -c);
+    default: return make_double2(c, s);
+  }
 }
 
 // ----------------------------------------------------------- per-solve data --
