@@ -2480,16 +2480,23 @@ wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
   if (e == hipSuccess)
     e = hipMallocAsync(reinterpret_cast<void**>(&C), sizeof(double2) * 6 * (size_t)numel * n_inc, st);
   if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&bad), sizeof(int), st);
+  const int P = wf::rough2d_split(numel, n_inc, n_phi);
+  double2* part = nullptr;
+  if (e == hipSuccess)
+    e = hipMallocAsync(reinterpret_cast<void**>(&part), sizeof(double2) * 6 * (size_t)P * n_inc * n_phi,
+                       st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dinc, hinc.data(), sizeof(wf::Rough2dIncDev) * n_inc,
                        hipMemcpyHostToDevice, st);
   int hbad = 0x7fffffff;
   if (e == hipSuccess)
-    e = wf::launch_rough2d(dx, dy, numel, dinc, n_inc, n_phi, distance_m, radius_m, C, dI, bad, st);
+    e = wf::launch_rough2d(dx, dy, numel, dinc, n_inc, n_phi, distance_m, radius_m, C, part, P, dI,
+                           bad, st);
   if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && memspace == WF_MEM_HOST)
     e = hipMemcpyAsync(intensity, dI, sizeof(double) * nI, hipMemcpyDeviceToHost, st);
   if (bad) (void)hipFreeAsync(bad, st);
+  if (part) (void)hipFreeAsync(part, st);
   if (C) (void)hipFreeAsync(C, st);
   if (dinc) (void)hipFreeAsync(dinc, st);
   if (memspace == WF_MEM_HOST) {

@@ -102,12 +102,14 @@ struct Rough2dIncDev {
   double z0, eta_re, eta_im;
   double fe, fh;              // -w mu0/4 and -w eps0/4
 };
-// 2D cross-section PO: currents C[n_inc][n][6] (scratch) and intensity
+// 2D cross-section PO: currents C[n_inc][n][6] and partial sums
+// part[n_inc][P][6][nphi] (scratch, P = rough2d_split), intensity
 // I[n_inc][nphi]; *bad receives the smallest index of a zero-length or
-// non-finite segment (must be INT_MAX on entry)
+// non-finite segment
+int rough2d_split(int n, int n_inc, int nphi);
 hipError_t launch_rough2d(const double* x, const double* y, int n, const Rough2dIncDev* incs,
-                          int n_inc, int nphi, double dist, double radius, double2* C, double* I,
-                          int* bad, hipStream_t st);
+                          int n_inc, int nphi, double dist, double radius, double2* C,
+                          double2* part, int P, double* I, int* bad, hipStream_t st);
 // per-child extended table for children at offsets (cx, cy, cz)[c] from the
 // parent centre; offsets of level-r nodes from the root centre from their keys
 // (hx/hy/hz[l]: the child-centre offset magnitudes of the level-l transition)

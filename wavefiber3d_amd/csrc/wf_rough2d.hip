@@ -29,6 +29,8 @@
 // thread one (direction, incidence) and streams the samples through LDS.
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "wf_common.h"
 #include "wf_currents.h"
 #include "wf_interp.h"
@@ -152,27 +154,36 @@ __global__ __launch_bounds__(kR2Threads) void rough2d_currents_kernel(
   for (int c = 0; c < 6; ++c) dst[c] = out[c];
 }
 
-// One thread per (direction k, incidence s).
-__global__ __launch_bounds__(kR2Threads) void rough2d_farfield_kernel(
-    const double* __restrict__ X, const double* __restrict__ Y, int n,
-    const wf::Rough2dIncDev* __restrict__ incs, const double2* __restrict__ C, int nphi,
-    double dist, double norm, double* __restrict__ I) {
-  __shared__ double sx[kR2Tile], sy[kR2Tile];
-  __shared__ double2 sc[kR2Tile][6];
-  const int k = blockIdx.x * kR2Threads + threadIdx.x;
-  const int s = blockIdx.y;
-  const wf::Rough2dIncDev inc = incs[s];
+__device__ inline void r2_direction(const wf::Rough2dIncDev& inc, int k, int nphi, double dir[3]) {
   const double phi = (double)k / (double)nphi * 2.0 * WF_PI;  // :521
   double sphi, cphi;
   sincos(phi, &sphi, &cphi);
-  const double dir[3] = {inc.ct * cphi, inc.ct * sphi, inc.st};
+  dir[0] = inc.ct * cphi;
+  dir[1] = inc.ct * sphi;
+  dir[2] = inc.st;
+}
+
+// One thread per (direction k, incidence s, point split z): the six sums over
+// samples [n z / P, n (z + 1) / P) into part[s][z][6][nphi].
+__global__ __launch_bounds__(kR2Threads) void rough2d_partial_kernel(
+    const double* __restrict__ X, const double* __restrict__ Y, int n,
+    const wf::Rough2dIncDev* __restrict__ incs, const double2* __restrict__ C, int nphi,
+    double2* __restrict__ part) {
+  __shared__ double sx[kR2Tile], sy[kR2Tile];
+  __shared__ double2 sc[kR2Tile][6];
+  const int k = blockIdx.x * kR2Threads + threadIdx.x;
+  const int s = blockIdx.y, z = blockIdx.z, P = gridDim.z;
+  const wf::Rough2dIncDev inc = incs[s];
+  double dir[3];
+  r2_direction(inc, k, nphi, dir);
   const double krev = inc.krho / WF_TWO_PI;
   double2 acc[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) acc[c] = make_double2(0.0, 0.0);
   const double2* Cs = C + (int64_t)s * n * 6;
-  for (int base = 0; base < n; base += kR2Tile) {
-    const int cnt = min(kR2Tile, n - base);
+  const int p0 = (int)((int64_t)n * z / P), p1 = (int)((int64_t)n * (z + 1) / P);
+  for (int base = p0; base < p1; base += kR2Tile) {
+    const int cnt = min(kR2Tile, p1 - base);
     __syncthreads();
     for (int t = threadIdx.x; t < cnt; t += kR2Threads) {
       sx[t] = X[base + t];
@@ -187,6 +198,31 @@ __global__ __launch_bounds__(kR2Threads) void rough2d_farfield_kernel(
     }
   }
   if (k >= nphi) return;
+  double2* dst = part + ((int64_t)s * P + z) * 6 * nphi + k;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst[(int64_t)c * nphi] = acc[c];
+}
+
+// One thread per (direction k, incidence s): the splits summed in order, the
+// 2D far-field factor, the projection and the intensity.
+__global__ __launch_bounds__(kR2Threads) void rough2d_epilogue_kernel(
+    const wf::Rough2dIncDev* __restrict__ incs, const double2* __restrict__ part, int P, int nphi,
+    double dist, double norm, double* __restrict__ I) {
+  const int k = blockIdx.x * kR2Threads + threadIdx.x;
+  const int s = blockIdx.y;
+  if (k >= nphi) return;
+  const wf::Rough2dIncDev inc = incs[s];
+  double dir[3];
+  r2_direction(inc, k, nphi, dir);
+  const double krev = inc.krho / WF_TWO_PI;
+  double2 acc[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int z = 0; z < P; ++z) {
+    const double2* src = part + ((int64_t)s * P + z) * 6 * nphi + k;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] = zadd(acc[c], src[(int64_t)c * nphi]);
+  }
   // 2D far-field factor sqrt(2i/(pi k_rho D)) exp(-i k_rho D) (:69)
   const double2 G = zmul(zsqrt(make_double2(0.0, 2.0 / (WF_PI * inc.krho * dist))),
                          zcis_rev(-krev * dist));
@@ -220,17 +256,30 @@ __global__ void r2_set_int_kernel(int* p, int v) { *p = v; }
 
 namespace wf {
 
+int rough2d_split(int n, int n_inc, int nphi) {
+  // enough (direction, incidence, split) blocks to fill the chip, >= 64 samples per split
+  const int64_t kb = (nphi + kR2Threads - 1) / kR2Threads;
+  int64_t P = (2048 + kb * n_inc - 1) / (kb * n_inc);
+  P = std::min<int64_t>(P, 16);
+  P = std::min<int64_t>(P, std::max(1, n / 64));
+  return (int)std::max<int64_t>(P, 1);
+}
+
 hipError_t launch_rough2d(const double* x, const double* y, int n, const Rough2dIncDev* incs,
-                          int n_inc, int nphi, double dist, double radius, double2* C, double* I,
-                          int* bad, hipStream_t st) {
+                          int n_inc, int nphi, double dist, double radius, double2* C,
+                          double2* part, int P, double* I, int* bad, hipStream_t st) {
   hipLaunchKernelGGL(r2_set_int_kernel, dim3(1), dim3(1), 0, st, bad, 0x7fffffff);
   const dim3 gc((unsigned)((n + kR2Threads - 1) / kR2Threads), (unsigned)n_inc);
   hipLaunchKernelGGL(rough2d_currents_kernel, gc, dim3(kR2Threads), 0, st, x, y, n, incs, C, bad);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const dim3 gf((unsigned)((nphi + kR2Threads - 1) / kR2Threads), (unsigned)n_inc);
-  hipLaunchKernelGGL(rough2d_farfield_kernel, gf, dim3(kR2Threads), 0, st, x, y, n, incs, C, nphi,
-                     dist, dist / (2.0 * radius), I);
+  const unsigned kb = (unsigned)((nphi + kR2Threads - 1) / kR2Threads);
+  hipLaunchKernelGGL(rough2d_partial_kernel, dim3(kb, (unsigned)n_inc, (unsigned)P), dim3(kR2Threads),
+                     0, st, x, y, n, incs, C, nphi, part);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rough2d_epilogue_kernel, dim3(kb, (unsigned)n_inc), dim3(kR2Threads), 0, st,
+                     incs, part, P, nphi, dist, dist / (2.0 * radius), I);
   return hipGetLastError();
 }
 
