@@ -167,11 +167,13 @@ def test_batch_rejects_bad_grid_and_empty():
             b.solve(INCS, (32, 63))  # odd n_phi
 
 
+@pytest.mark.parametrize("depth", [0, 3])
 @pytest.mark.parametrize("field_chunk", [0, 12])
-def test_solve_streams_identical_to_sequential(field_chunk):
+def test_solve_streams_identical_to_sequential(field_chunk, depth):
     """wf_solve's concurrent chunks (opts.solve_streams) give bit-identical
-    results to the sequential order, on device and host outputs, and the same
-    launch counts in the merged statistics."""
+    results to the sequential order, on device and host outputs, for tree and
+    depth-0 (FP64 direct) geometries, and the same launch counts in the merged
+    statistics."""
     s = wf.fibonacci_sphere(20000, 2e-6)
     dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
     incs = [wf.Incidence(0.3 * k, 0.5, 0.45e-6 + 0.02e-6 * (k // 2), "ME"[k % 2], 1.5 + 0.01j)
@@ -179,9 +181,9 @@ def test_solve_streams_identical_to_sequential(field_chunk):
     grid = (36, 72)
     out = {}
     for K in (1, 3, 4):
-        print(f"case: solve_streams={K} field_chunk={field_chunk}", flush=True)
+        print(f"case: solve_streams={K} field_chunk={field_chunk} depth={depth}", flush=True)
         with wf.POSolver(device=0, solve_streams=K, field_chunk=field_chunk) as sv:
-            sv.set_geometry(dev, 3)
+            sv.set_geometry(dev, depth)
             I = sv.solve(incs, grid, out="torch")
             torch.cuda.synchronize()
             st = sv.stats()
