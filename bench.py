@@ -740,6 +740,17 @@ def bench_adjoint(args, W):
             g8 = (torch.stack([x[c].grad for c in range(3)]).double() * nrm).sum(0)
         rel_tree8 = float(torch.linalg.norm(g8 - g64) / torch.linalg.norm(g64))
         out["ms"].update({"tree_forward_beta8": ms_fwd8, "tree_adjoint_beta8": ms_vjp8})
+        # the same with the leaf radiation sum on FP32 MFMA instead of split BF16 x3
+        with wf.POSolver(device=0, precision="fp32") as sv32:
+            sv32.set_geometry(s32, depth_used)
+            ms_fwd32, _ = sync_time(lambda: sv32.solve(incs, grid, out="torch"))
+            ms_vjp32, _ = sync_time(lambda: sv32.vjp(incs, grid, gI))
+            x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(s32)]
+            L = log_intensity_loss(po_intensity(x, incs, grid, depth_used, sv32), target)
+            L.backward()
+            g32 = (torch.stack([x[c].grad for c in range(3)]).double() * nrm).sum(0)
+        rel_tree32 = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+        out["ms"].update({"tree_forward_fp32": ms_fwd32, "tree_adjoint_fp32": ms_vjp32})
 
         def loss64(surf):
             Ix = sv.solve_fp64_direct(surf, incs, grid)
@@ -824,6 +835,7 @@ def bench_adjoint(args, W):
         fd_tau = tilt_ratio * g_tau_t
     out["gradient"] = {
         "tree_vs_fp64_rel_l2": rel_tree, "tree_beta8_vs_fp64_rel_l2": rel_tree8,
+        "tree_fp32_mfma_vs_fp64_rel_l2": rel_tree32,
         "fd_64pts_h1e-4um": fd, "fp64_adjoint_pred": pred64, "tree_adjoint_pred": pred_tree,
         "fp64_vs_fd_rel_h": abs(pred64 - fd) / abs(fd),
         "tree_vs_fd_rel_h": abs(pred_tree - fd) / abs(fd),
@@ -834,7 +846,7 @@ def bench_adjoint(args, W):
         "tilt_rel": abs(g_tau - fd_tau) / abs(fd_tau),
         "gate": 1e-4,
     }
-    print(f"[bench] gradients: tree vs fp64 {rel_tree:.2e} (beta 8: {rel_tree8:.2e}); FD 64 pts: fp64 "
+    print(f"[bench] gradients: tree vs fp64 {rel_tree:.2e} (beta 8: {rel_tree8:.2e}, FP32 MFMA: {rel_tree32:.2e}); FD 64 pts: fp64 "
           f"{out['gradient']['fp64_vs_fd_rel']:.2e}, tree {out['gradient']['tree_vs_fd_rel']:.2e}; "
           f"tilt {out['gradient']['tilt_rel']:.2e} (dL/dtau {g_tau:.6g})", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
