@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 4
+#define WF_ABI_VERSION 5
 
 /* depth argument of wf_set_geometry: build the octree to the deepest useful
  * depth (about log8 n) and let each wavelength group of a solve use the
@@ -115,6 +115,17 @@ typedef struct wf_opts {
                             0 = default (4); graph capture and wf_autotune run sequentially.
                             Each extra stream keeps a workspace of its own (as large as the
                             handle's; workspace_cap_bytes applies to each) */
+  double ref_dtheta_m;   /* > 0 together with ref_dphi_m: the reference's level sampling
+                            rule instead of this library's (treePO.cu:342-351): level i
+                            takes N = ceil(k d + beta (k d)^(1/3)), k = 2 pi / lambda_s,
+                            with d = 0.5^i ref_dtheta_m for theta and 0.5^i ref_dphi_m
+                            for phi (the reference's extent x rho: fiber period x 1.2 and
+                            2 max(r1, r2) x 1.2, sphere diameter x 1.3, ellipsoid box
+                            extents x 3), rounded up to 5-smooth lengths (phi even).  Use
+                            with beta = 6 and lambda_sample_m = 0.4e-6 for the reference's
+                            grids; its theta count covers the z extent only, so nodes wider
+                            than tall are under-sampled as in the reference */
+  double ref_dphi_m;
 } wf_opts;
 
 typedef struct wf_handle wf_handle;
@@ -143,6 +154,13 @@ typedef struct wf_stats {
   int32_t depth, n_levels;
   int32_t level_nt[16], level_np[16]; /* level grids of the last lambda group */
   int32_t level_nodes[16];            /* active nodes per level */
+  /* Level transitions (aggregation) of the last solve: their number and their
+   * compulsory HBM bytes -- every child pattern read once and every parent
+   * pattern written once (complex64; 16 B per output direction and field
+   * for the FP64 root accumulators) -- whatever intermediate buffers and
+   * tables the kernels use. */
+  int32_t agg_transitions;
+  double agg_bytes;
 } wf_stats;
 
 int32_t wf_abi_version(void);

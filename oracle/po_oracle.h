@@ -129,7 +129,7 @@ void orc_sphere_interp(int ntc, int npc, double off_tc, const double* in,
 /* compareBEM2d/rough2d.cu:237-585 for one incidence: closed polyline x, y [n]
  * (sample i at vertex i, segment i -> i+1 mod n), scattering [nphi] =
  * |E''|^2 Dis/(2 radius) at phi_p = 2 pi p/nphi; C (nullable) receives the
- * unscaled currents [n][J, M][2] (rough2d.cu:348-361). */
+ * unscaled currents [n][6][2], components Jx, Jy, Jz, Mx, My, Mz (rough2d.cu:348-361). */
 void orc_rough2d(int n, const double* x, const double* y, double theta, double phii,
                  double lambda_m, char mode, double eta_re, double eta_im, int nphi, double Dis,
                  double radius, double* scattering, double* C);

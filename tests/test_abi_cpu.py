@@ -31,7 +31,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_default_opts():
     L = wf.lib()
-    assert L.wf_abi_version() == 4
+    assert L.wf_abi_version() == 5
     o = _lib.Opts()
     L.wf_default_opts(ct.byref(o))
     assert o.struct_size == ct.sizeof(_lib.Opts)

@@ -223,3 +223,27 @@ def test_solve_streams_cancel_then_resume():
         I1, E1 = I1.cpu().numpy(), E1.cpu().numpy()
     assert np.array_equal(I, I1)
     assert np.array_equal(E, E1)
+
+
+def test_batch_cancel_then_resume():
+    """A cancelled batch solve returns WF_ERR_CANCELLED after joining every
+    worker stream back into the caller's stream (device output); cleared, the
+    next batch solve on the same batch matches a fresh batch bit for bit."""
+    import ctypes as ct
+
+    segs = [wf.fibonacci_sphere(n, r) for n, r in ((9000, 1.5e-6), (5000, 1.2e-6), (3000, 1e-6))]
+    flag = ct.c_int32(1)
+    print("case: cancelled batch solve, 3 segments, 3 streams", flush=True)
+    with wf.Batch(n_streams=3, device=0, cancel_flag=flag) as b:
+        for s in segs:
+            b.add(s, 2)
+        with pytest.raises(wf.WfError, match="CANCELLED"):
+            b.solve(INCS, GRID, out="torch")
+        torch.cuda.synchronize()
+        flag.value = 0
+        I = b.solve(INCS, GRID, out="torch").cpu().numpy()
+    with wf.Batch(n_streams=3, device=0) as b1:
+        for s in segs:
+            b1.add(s, 2)
+        I1 = b1.solve(INCS, GRID)
+    assert np.array_equal(I, I1)

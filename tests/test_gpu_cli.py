@@ -95,3 +95,110 @@ def test_rough2d_command_writes_reference_files(tmp_path):
         assert got.shape == (3600,)
         assert np.abs(got - ref).max() < 1e-11 * ref.max()
     assert os.path.exists(os.path.join(out, "ellipse_TE_0.binary"))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _oracle_dirs(s, incs, I_full, n_dirs=256, seed=0):
+    """Tree output vs the C FP64 oracle on n_dirs seeded directions of 900 x 1440."""
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(900 * 1440, n_dirs, replace=False))
+    dirs = oracle.output_dirs(900, 1440)[idx]
+    Io, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs], 0, 0, dirs=dirs)
+    return max(_rel(I_full[k].ravel()[idx], Io[k]) for k in range(len(incs)))
+
+
+def test_ellipsoid_driver_on_reference_mesh(tmp_path):
+    """compareBEM3d/ellipsoid.cu's driver on the reference's own 30,322-sample
+    mesh: file names and sizes, TM and TE within 1e-4 of the FP64 direct sum
+    on the full 900 x 1440 grid and of the C oracle on 256 directions."""
+    import shutil
+
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    po = os.path.join(tmp_path, "mesh", "PO")
+    os.makedirs(po)
+    for f in ("ellipsoid_3_3_2_float.binary", "ellipsoid_3_3_2_normal_float.binary",
+              "ellipsoid_3_3_2_area_float.binary"):
+        shutil.copy(os.path.join(g, f), po)
+    out = os.path.join(tmp_path, "output")
+    print("case: cli ellipsoid depth 3", flush=True)
+    assert cli.main(["ellipsoid", "3", "--mesh-dir", os.path.join(tmp_path, "mesh"),
+                     "--output-dir", out]) == 0
+    s = cli._read_soa_mesh(po, "ellipsoid_3_3_2")
+    assert s.n == 30322
+    incs = [wf.Incidence(0.0, 0.0, cli.f32(500e-9), m, 1.55 + 0.3j, cli.f32(10000e-6)) for m in "ME"]
+    got = [np.fromfile(os.path.join(out, f"ellipsoid_lambda0_T{m}_depth3_kappa0.3.binary"),
+                       np.float32).reshape(900, 1440) for m in "ME"]
+    with wf.POSolver(device=0) as sv:
+        I64 = sv.solve_fp64_direct(s, incs, (900, 1440))
+    for k in range(2):
+        assert _rel(got[k], I64[k]) < 1e-4
+    assert _oracle_dirs(s, incs, got) < 1e-4
+
+
+def test_sphere_driver_file_names_and_parity(tmp_path):
+    """compareMie/sphere.cu's driver: a 401,182-sample Fibonacci sphere of
+    radius 10 um written in the reference's mesh layout (the reference's own
+    point and normal files are not shipped), two wavelengths x TM/TE at depth 3;
+    file names and sizes, and each grid within 1e-4 of the FP64 direct sum
+    and of the C oracle on 256 directions."""
+    s = wf.fibonacci_sphere(401_182, 10e-6)
+    mesh = os.path.join(tmp_path, "mesh")
+    os.makedirs(mesh)
+    x, y, z, nx, ny, nz, a = s.arrays()
+    np.concatenate([x, y, z]).astype(np.float32).tofile(os.path.join(mesh, "sphere10um_float.binary"))
+    np.concatenate([nx, ny, nz]).astype(np.float32).tofile(
+        os.path.join(mesh, "sphere10um_normal_float.binary"))
+    a.astype(np.float32).tofile(os.path.join(mesh, "sphere10um_area_float.binary"))
+    out = os.path.join(tmp_path, "output")
+    print("case: cli sphere 0 2 3", flush=True)
+    assert cli.main(["sphere", "0", "2", "3", "--mesh-dir", mesh, "--output-dir", out]) == 0
+    with wf.POSolver(device=0) as sv:
+        for lam in (0, 1):
+            incs = [wf.Incidence(0.0, 0.0, cli.wavelength(lam), m, 1.55 + 0j, cli.f32(10000e-6))
+                    for m in "ME"]
+            got = [np.fromfile(os.path.join(out, f"sphere_10um_lambda{lam}_T{m}_depth3.binary"),
+                               np.float32).reshape(900, 1440) for m in "ME"]
+            I64 = sv.solve_fp64_direct(s, incs, (900, 1440))
+            for k in range(2):
+                assert _rel(got[k], I64[k]) < 1e-4
+            if lam == 0:
+                assert _oracle_dirs(s, incs, got) < 1e-4
+
+
+def test_reference_sampling_rule_level_grids():
+    """--reference-sampling: the level grids are the reference's counts
+    (treePO.cu:342-351, beta 6, rho 1.2 at 0.4 um) rounded up to 5-smooth
+    lengths (phi even).  The reference's theta count covers the z extent
+    only, so its grids under-sample wide boxes near the poles; the error
+    against the FP64 sum is reported, not gated."""
+    s = wf.fiber_segment(8e-6, 6e-6, 4e-6, 0.1e-6, seed=1)
+    period, r = np.float32(4e-6), np.float32(8e-6)
+    dt = float(period * np.float32(1.2))
+    dp = float(np.float32(2.0) * r * np.float32(1.2))
+    inc = [wf.Incidence(0.2, 0.4, 0.5e-6, "M", 1.55 + 0.02j)]
+    print("case: reference sampling rule, fiber 8x6x4 um depth 3", flush=True)
+    with wf.POSolver(device=0, beta=6.0, lambda_sample_m=0.4e-6, ref_sampling=(dt, dp)) as sv:
+        sv.set_geometry(s, 3)
+        I = sv.solve(inc, (90, 180))
+        lv = sv.stats()["levels"]
+
+    def smooth(n, even=False):
+        def ok(m):
+            for q in (2, 3, 5):
+                while m % q == 0:
+                    m //= q
+            return m == 1
+        while not ok(n) or (even and n % 2):
+            n += 1
+        return n
+
+    for l in range(1, 4):
+        nt = smooth(max(4, oracle.ref_level_count(l, dt, 6.0)))
+        np_ = smooth(max(8, oracle.ref_level_count(l, dp, 6.0)), even=True)
+        assert tuple(lv[l][:2]) == (nt, np_), (l, lv[l], nt, np_)
+    Iref, _ = oracle.solve_direct(s.arrays(), [inc[0].as_tuple()], 90, 180)
+    assert np.isfinite(I).all()
+    print(f"  levels {lv}; rel-L2 vs FP64 oracle {_rel(I[0], Iref[0]):.2e}", flush=True)

@@ -89,6 +89,8 @@ class Opts(ct.Structure):
         ("field_chunk", ct.c_int32),
         ("use_graph", ct.c_int32),
         ("solve_streams", ct.c_int32),
+        ("ref_dtheta_m", ct.c_double),
+        ("ref_dphi_m", ct.c_double),
     ]
 
 
@@ -107,6 +109,7 @@ class Stats(ct.Structure):
         ("depth", ct.c_int32), ("n_levels", ct.c_int32),
         ("level_nt", ct.c_int32 * 16), ("level_np", ct.c_int32 * 16),
         ("level_nodes", ct.c_int32 * 16),
+        ("agg_transitions", ct.c_int32), ("agg_bytes", ct.c_double),
     ]
 
 

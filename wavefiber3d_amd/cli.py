@@ -15,6 +15,13 @@ fiber's sigma; D = 1 m) and the octree box is +-1.1 (r1, r2, period/2)
 (treePO.cu:94-99).  The tree's own sampling rule and kernels are this
 library's (the output grid and file layout are the reference's).
 
+``python -m wavefiber3d_amd.cli sphere LSTART LEND DEPTH`` and
+``python -m wavefiber3d_amd.cli ellipsoid DEPTH`` are the drivers of
+compareMie/sphere.cu and compareBEM3d/ellipsoid.cu (arguments, mesh files,
+incidence, grids and output file names; see ``sphere`` and ``ellipsoid``).
+Every tree driver takes ``--reference-sampling``: the reference's level grids
+(its beta = 6 and extent x rho rule at 0.4 um) instead of this library's rule.
+
 ``python -m wavefiber3d_amd.cli mie`` writes the Mie curves of
 compareMie/mie.ipynb cells 2-3 (see ``mie_curves``);
 ``python -m wavefiber3d_amd.cli rough2d`` is compareBEM2d/rough2d.cu's driver
@@ -83,6 +90,9 @@ def treepo(argv) -> int:
     ap.add_argument("--final", action="store_true")
     ap.add_argument("--fiber-table", default=None)
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--reference-sampling", action="store_true",
+                    help="the reference's level grids: beta 6, rho 1.2 at 0.4 um on the period "
+                         "(theta) and 2 max(r1, r2) (phi), treePO.cu:92-93, 342-351")
     a = ap.parse_args(argv)
     table = FIBER_TABLE
     if a.fiber_table:
@@ -114,7 +124,13 @@ def treepo(argv) -> int:
     os.makedirs(a.output_dir, exist_ok=True)
     thetai, phii = incidence_angles(a.thetaindexstart, a.phiiindexstart)
     grid = (int(p["thetao"]), int(p["phio"]))
-    with wf.POSolver(device=a.device, box=(-xr, -yr, -zr, xr, yr, zr)) as solver:
+    kw = {}
+    if a.reference_sampling:
+        rho = np.float32(1.2)
+        kw = dict(beta=6.0, lambda_sample_m=0.4e-6,
+                  ref_sampling=(float(period * rho),
+                                float(np.float32(2.0) * np.float32(max(p["r1"], p["r2"])) * rho)))
+    with wf.POSolver(device=a.device, box=(-xr, -yr, -zr, xr, yr, zr), **kw) as solver:
         solver.set_geometry(surf, a.max_depth)
         for lam in range(a.lambdaindexstart, a.lambdaindexend):
             inc = wf.Incidence(thetai, phii, wavelength(lam), "M", 1.55 + 0j, f32(p["sigma"]))
@@ -125,6 +141,119 @@ def treepo(argv) -> int:
                 wf.average_grid(I, 180, 360).cpu().numpy().tofile(
                     output_name(a.output_dir, a.fiberindex, lam, "M", a.max_depth, final=True))
             print(f"lambdaindex {lam} done", flush=True)
+    return 0
+
+
+def _read_soa_mesh(mesh_dir: str, stem: str):
+    """``<stem>_float.binary`` (positions [x...|y...|z...]), ``<stem>_normal_float.binary``
+    and ``<stem>_area_float.binary`` in float32 (compareMie/sphere.cu:92-125,
+    compareBEM3d/ellipsoid.cu:84-114); the sample count is taken from the area file."""
+    import wavefiber3d_amd as wf
+
+    pts = os.path.join(mesh_dir, f"{stem}_float.binary")
+    nrm = os.path.join(mesh_dir, f"{stem}_normal_float.binary")
+    area = os.path.join(mesh_dir, f"{stem}_area_float.binary")
+    for f in (pts, nrm, area):
+        if not os.path.exists(f):
+            raise FileNotFoundError(f)
+    return wf.read_soa_mesh(pts, nrm, area)
+
+
+def _solve_both_pols(surf, box, depth, grid, lam, eta, sigma, outname, device, ref_sampling):
+    """The reference's polarization loop (TM then TE) at incidence theta = phi = 0
+    (along +x), one output file per polarization."""
+    import wavefiber3d_amd as wf
+
+    kw = {}
+    if ref_sampling is not None:
+        kw = dict(beta=6.0, lambda_sample_m=0.4e-6, ref_sampling=ref_sampling)
+    with wf.POSolver(device=device, box=box, **kw) as solver:
+        solver.set_geometry(surf, depth)
+        incs = [wf.Incidence(0.0, 0.0, lam, m, eta, sigma) for m in "ME"]
+        I = solver.solve(incs, grid)
+    for k, m in enumerate("ME"):
+        I[k].astype(np.float32).tofile(outname(m))
+    return I
+
+
+def sphere(argv) -> int:
+    """compareMie/sphere.cu:21-966: ``sphere LSTART LEND DEPTH`` reads the 10 um
+    sphere mesh (``<mesh-dir>/sphere10um_float.binary``, ``..._normal_float``,
+    ``..._area_float``; the reference reads 401,182 samples) and writes, per
+    wavelength index l in [LSTART, LEND) (lambda = 400 + 300/25 l nm in float)
+    and polarization, ``<output-dir>/sphere_10um_lambda<l>_T<M|E>_depth<DEPTH>.binary``:
+    the float32 intensity [900][1440] for incidence along +x (theta = phi = 0),
+    IOR 1.55, Gaussian beam sigma 1 cm, octree box +-1.1 r (sphere.cu:36-66, 544-590,
+    929).  ``--reference-sampling`` takes the reference's level grids (beta 6,
+    rho 1.3 on the diameter at 0.4 um, sphere.cu:345-360) instead of the library's rule."""
+    ap = argparse.ArgumentParser(prog="python -m wavefiber3d_amd.cli sphere")
+    for a_ in ("lambdaindexstart", "lambdaindexend", "max_depth"):
+        ap.add_argument(a_, type=int)
+    ap.add_argument("--mesh-dir", default="mesh")
+    ap.add_argument("--output-dir", default="output")
+    ap.add_argument("--radius-um", type=int, default=10)
+    ap.add_argument("--reference-sampling", action="store_true")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    try:
+        surf = _read_soa_mesh(a.mesh_dir, f"sphere{a.radius_um}um")
+    except FileNotFoundError as e:
+        print(f"missing mesh file {e}", file=sys.stderr)
+        return 2
+    radius = f32(a.radius_um * 1e-6)
+    r = float(np.float32(radius) * np.float32(1.1))
+    box = (-r, -r, -r, r, r, r)
+    ref = None
+    if a.reference_sampling:
+        d = float(np.float32(radius * 2) * np.float32(1.3))
+        ref = (d, d)
+    os.makedirs(a.output_dir, exist_ok=True)
+    for lam in range(a.lambdaindexstart, a.lambdaindexend):
+        _solve_both_pols(
+            surf, box, a.max_depth, (900, 1440), wavelength(lam), 1.55 + 0j, f32(10000e-6),
+            lambda m: os.path.join(a.output_dir,
+                                   f"sphere_{a.radius_um}um_lambda{lam}_T{m}_depth{a.max_depth}.binary"),
+            a.device, ref)
+        print(f"lambdaindex {lam} done", flush=True)
+    return 0
+
+
+def ellipsoid(argv) -> int:
+    """compareBEM3d/ellipsoid.cu:20-990: ``ellipsoid DEPTH`` reads the
+    3 x 3 x 2 um ellipsoid mesh (``<mesh-dir>/PO/ellipsoid_3_3_2_float.binary``,
+    ``..._normal_float``, ``..._area_float``: 30,322 samples) and writes
+    ``<output-dir>/ellipsoid_lambda0_T<M|E>_depth<DEPTH>_kappa0.3.binary``: the
+    float32 intensity [900][1440] at 500 nm for incidence along +x, IOR
+    1.55 + 0.3i, Gaussian beam sigma 1 cm, octree box +-(10, 10, 5) um
+    (ellipsoid.cu:38-56, 486-590, 956).  ``--reference-sampling``: the
+    reference's level grids (beta 6, rho 3 at 0.4 um, ellipsoid.cu:342-352)."""
+    ap = argparse.ArgumentParser(prog="python -m wavefiber3d_amd.cli ellipsoid")
+    ap.add_argument("max_depth", type=int)
+    ap.add_argument("--mesh-dir", default="mesh")
+    ap.add_argument("--output-dir", default="output")
+    ap.add_argument("--reference-sampling", action="store_true")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    try:
+        surf = _read_soa_mesh(os.path.join(a.mesh_dir, "PO"), "ellipsoid_3_3_2")
+    except FileNotFoundError as e:
+        print(f"missing mesh file {e}", file=sys.stderr)
+        return 2
+    r1 = r2 = f32(10e-6)
+    period = f32(10e-6)
+    zr = f32(period * 0.5)
+    box = (-r1, -r2, -zr, r1, r2, zr)
+    ref = None
+    if a.reference_sampling:
+        ref = (float(np.float32(period) * np.float32(3.0)),
+               float(np.float32(2.0) * np.float32(max(r1, r2)) * np.float32(3.0)))
+    os.makedirs(a.output_dir, exist_ok=True)
+    _solve_both_pols(
+        surf, box, a.max_depth, (900, 1440), f32(500e-9), 1.55 + 0.3j, f32(10000e-6),
+        lambda m: os.path.join(a.output_dir,
+                               f"ellipsoid_lambda0_T{m}_depth{a.max_depth}_kappa0.3.binary"),
+        a.device, ref)
+    print("lambdaindex 0 done", flush=True)
     return 0
 
 
@@ -201,7 +330,8 @@ def rough2d(argv) -> int:
     return 0
 
 
-COMMANDS = {"treepo": treepo, "mie": mie_curves, "rough2d": rough2d}
+COMMANDS = {"treepo": treepo, "sphere": sphere, "ellipsoid": ellipsoid, "mie": mie_curves,
+            "rough2d": rough2d}
 
 
 def main(argv=None) -> int:
@@ -209,6 +339,10 @@ def main(argv=None) -> int:
     if not argv or argv[0] not in COMMANDS:
         print("usage: python -m wavefiber3d_amd.cli treepo LSTART LEND THETAI PHII FIBER DEPTH "
               "START END [--mesh-dir D] [--output-dir D] [--final] [--fiber-table F] [--device N]\n"
+              "       python -m wavefiber3d_amd.cli sphere LSTART LEND DEPTH [--mesh-dir D] "
+              "[--output-dir D] [--reference-sampling] [--device N]\n"
+              "       python -m wavefiber3d_amd.cli ellipsoid DEPTH [--mesh-dir D] [--output-dir D] "
+              "[--reference-sampling] [--device N]\n"
               "       python -m wavefiber3d_amd.cli mie [--wavelength NM] [--radius NM] [--ior C] "
               "[--resolution DEG] [--output-dir D]\n"
               "       python -m wavefiber3d_amd.cli rough2d TESTCASE MODE LSTART LEND "

@@ -450,8 +450,19 @@ Grid level_grid(const wf_handle* h, int level, double lambda_s) {
   const double R = std::sqrt(hx * hx + hy * hy + hz * hz);
   const double Rxy = std::sqrt(hx * hx + hy * hy);
   const double beta = h->opts.beta;
-  auto L = [&](double r) { return ks * r + beta * std::cbrt(ks * r); };
   Grid g;
+  if (h->opts.ref_dtheta_m > 0 && h->opts.ref_dphi_m > 0) {
+    // the reference's rule (treePO.cu:346-351): d in float, the count in double
+    auto N = [&](double d0) {
+      const float d = (float)(std::ldexp(1.0, -level) * d0);
+      const double kd = ks * d;
+      return (int)std::ceil(kd + beta * std::pow(kd, 1.0 / 3.0));
+    };
+    g.nt = smooth_up(std::max(4, N(h->opts.ref_dtheta_m)));
+    g.np = smooth_even_up(std::max(8, N(h->opts.ref_dphi_m)));
+    return g;
+  }
+  auto L = [&](double r) { return ks * r + beta * std::cbrt(ks * r); };
   g.nt = smooth_up(std::max(4, (int)std::ceil(L(R))));
   g.np = smooth_even_up(std::max(8, (int)std::ceil(2.0 * L(Rxy))));
   return g;
@@ -556,6 +567,8 @@ wf_status prof_finalize(wf_handle* h) {
       h->stats.kern_work[k] += w->stats.kern_work[k];
     }
     h->stats.leaf_pairs += w->stats.leaf_pairs;
+    h->stats.agg_transitions += w->stats.agg_transitions;
+    h->stats.agg_bytes += w->stats.agg_bytes;
   }
   h->stats_final = true;
   return WF_OK;
@@ -1327,6 +1340,11 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     const float2* Tl = gc.T ? gc.T + gc.toff[l] : nullptr;
     const LevelPath lpath = level_path(h, gcg, gp);
     const bool mfma_ok = lpath.mfma, fft2_ok = lpath.fft2;
+    // compulsory bytes of this transition: children read once, parents
+    // written once (the FP64 root accumulators at 16 B per sample)
+    h->stats.agg_transitions += 1;
+    h->stats.agg_bytes += 8.0 * planes * gcg.nt * gcg.np +
+                          ((l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np);
     if (fft2_ok && (2 * gp.nt > h->tune.agg_fft_rows || !mfma_ok)) {
       // large grids: LDS FFT (each sample read and written once)
       Fft2Plan f_pc, f_pp, f_tc, f_tp;
@@ -2259,40 +2277,12 @@ wf_status wf_batch_add(wf_batch* b, const wf_surface* s, int32_t depth, int32_t*
   return WF_OK;
 }
 
-wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
-                         float* intensity, int32_t out_memspace, void* stream) {
-  if (!b) return set_err(WF_ERR_INVALID_ARG, "batch is null");
-  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
-  if (b->segs.empty()) return set_err(WF_ERR_NO_GEOMETRY, "wf_batch_solve: no segments");
-  if (out_memspace != WF_MEM_HOST && out_memspace != WF_MEM_DEVICE)
-    return set_err(WF_ERR_INVALID_ARG, "out_memspace unknown");
-  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
-  WF_TRY(check_inc(inc, n_inc));
-  WF_HIP(hipSetDevice(b->device));
+// Queues every segment's solve on the worker streams (the body of
+// wf_batch_solve; the caller joins the workers whatever this returns).
+static wf_status batch_queue(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf_grid grid, float* out,
+                      int64_t per_seg) {
   const int nseg = (int)b->segs.size();
   const int nw = (int)b->workers.size();
-  const int64_t per_seg = (int64_t)n_inc * grid.n_theta * grid.n_phi;
-  for (const SegGeom& g : b->segs)
-    if (g.depth_max > 0) WF_TRY(check_tree_grid(grid));
-  float* out = intensity;
-  if (out_memspace == WF_MEM_HOST) {
-    const size_t need = sizeof(float) * (size_t)per_seg * nseg;
-    if (b->out_cap < need) {
-      if (b->out) {
-        for (auto s : b->streams) WF_HIP(hipStreamSynchronize(s));
-        hipFree(b->out);
-      }
-      b->out = nullptr;
-      b->out_cap = 0;
-      WF_HIP(hipMalloc(&b->out, need));
-      b->out_cap = need;
-    }
-    out = static_cast<float*>(b->out);
-  }
-  hipStream_t caller = as_stream(stream);
-  // the workers start after the caller's prior work (device outputs)
-  WF_HIP(hipEventRecord(b->done[0], caller));
-  for (int w = 0; w < nw; ++w) WF_HIP(hipStreamWaitEvent(b->streams[w], b->done[0], 0));
   // largest segments first; each goes to the worker with the least queued
   // points (deterministic, so a segment keeps its worker and its graph)
   std::vector<int> order(nseg);
@@ -2338,10 +2328,56 @@ wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf
     WF_HIP(hipEventRecord(g.ev1, st));
     g.timed = true;
   }
-  for (int w = 0; w < nw; ++w) {
-    WF_HIP(hipEventRecord(b->done[w], b->streams[w]));
-    WF_HIP(hipStreamWaitEvent(caller, b->done[w], 0));
+  return WF_OK;
+}
+
+wf_status wf_batch_solve(wf_batch* b, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
+                         float* intensity, int32_t out_memspace, void* stream) {
+  if (!b) return set_err(WF_ERR_INVALID_ARG, "batch is null");
+  if (!intensity) return set_err(WF_ERR_INVALID_ARG, "intensity is null");
+  if (b->segs.empty()) return set_err(WF_ERR_NO_GEOMETRY, "wf_batch_solve: no segments");
+  if (out_memspace != WF_MEM_HOST && out_memspace != WF_MEM_DEVICE)
+    return set_err(WF_ERR_INVALID_ARG, "out_memspace unknown");
+  if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
+  WF_TRY(check_inc(inc, n_inc));
+  WF_HIP(hipSetDevice(b->device));
+  const int nseg = (int)b->segs.size();
+  const int nw = (int)b->workers.size();
+  const int64_t per_seg = (int64_t)n_inc * grid.n_theta * grid.n_phi;
+  for (const SegGeom& g : b->segs)
+    if (g.depth_max > 0) WF_TRY(check_tree_grid(grid));
+  float* out = intensity;
+  if (out_memspace == WF_MEM_HOST) {
+    const size_t need = sizeof(float) * (size_t)per_seg * nseg;
+    if (b->out_cap < need) {
+      if (b->out) {
+        for (auto s : b->streams) WF_HIP(hipStreamSynchronize(s));
+        hipFree(b->out);
+      }
+      b->out = nullptr;
+      b->out_cap = 0;
+      WF_HIP(hipMalloc(&b->out, need));
+      b->out_cap = need;
+    }
+    out = static_cast<float*>(b->out);
   }
+  hipStream_t caller = as_stream(stream);
+  // the workers start after the caller's prior work (device outputs)
+  WF_HIP(hipEventRecord(b->done[0], caller));
+  for (int w = 0; w < nw; ++w) WF_HIP(hipStreamWaitEvent(b->streams[w], b->done[0], 0));
+  // from here on every exit joins all worker streams back into the caller's
+  // stream: segments queued before an error or a cancellation keep writing
+  // into `out`, so the caller's stream must not run ahead of them
+  wf_status err = batch_queue(b, inc, n_inc, grid, out, per_seg);
+  for (int w = 0; w < nw; ++w) {
+    const hipError_t e1 = hipEventRecord(b->done[w], b->streams[w]);
+    const hipError_t e2 = (e1 == hipSuccess) ? hipStreamWaitEvent(caller, b->done[w], 0) : e1;
+    if (e2 != hipSuccess && err == WF_OK)
+      err = set_err(WF_ERR_HIP, "batch join failed: %s", hipGetErrorString(e2));
+  }
+  if (err != WF_OK && out_memspace == WF_MEM_HOST) hipStreamSynchronize(caller);
+  WF_TRY(err);
+
   if (out_memspace == WF_MEM_HOST) {
     WF_HIP(hipMemcpyAsync(intensity, out, sizeof(float) * (size_t)per_seg * nseg,
                           hipMemcpyDeviceToHost, caller));
@@ -2402,10 +2438,12 @@ wf_status wf_heightfield_points(const float* height, int32_t numel, int32_t nume
   return WF_OK;
 }
 
-wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
-                           const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
-                           double distance_m, double radius_m, double* intensity,
-                           int32_t memspace, void* stream) {
+// Argument checks of the 2D cross-section solve and the per-incidence device
+// parameters (incidence basis, k, far-field factors) in hinc[n_inc].
+static wf_status rough2d_params(const double* x, const double* y, int32_t numel,
+                                const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
+                                double distance_m, double radius_m, const double* intensity,
+                                int32_t memspace, wf::Rough2dIncDev* hinc) {
   if (!x || !y || !inc || !intensity) return set_err(WF_ERR_INVALID_ARG, "null argument");
   if (numel < 2 || numel > (1 << 26)) return set_err(WF_ERR_INVALID_ARG, "numel must be in [2, 2^26]");
   if (n_inc < 1 || n_inc > 65535) return set_err(WF_ERR_INVALID_ARG, "n_inc must be in [1, 65535]");
@@ -2417,7 +2455,6 @@ wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
   if (memspace != WF_MEM_HOST && memspace != WF_MEM_DEVICE)
     return set_err(WF_ERR_INVALID_ARG, "memspace unknown");
   const double mu0 = 4e-7 * WF_PI, eps0 = 8.8541878e-12, c0 = 299792458.0;  // rough2d.cu:25-28
-  std::vector<wf::Rough2dIncDev> hinc((size_t)n_inc);
   for (int i = 0; i < n_inc; ++i) {
     const wf_rough2d_incidence& a = inc[i];
     if (!(a.lambda_m > 0 && std::isfinite(a.lambda_m)))
@@ -2458,6 +2495,17 @@ wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
     d.fe = -omega * mu0 / 4.0;  // :82-90
     d.fh = -omega * eps0 / 4.0;
   }
+  return WF_OK;
+}
+
+wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
+                           const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
+                           double distance_m, double radius_m, double* intensity,
+                           int32_t memspace, void* stream) {
+  if (n_inc < 1 || n_inc > 65535) return set_err(WF_ERR_INVALID_ARG, "n_inc must be in [1, 65535]");
+  std::vector<wf::Rough2dIncDev> hinc((size_t)n_inc);
+  WF_TRY(rough2d_params(x, y, numel, inc, n_inc, n_phi, distance_m, radius_m, intensity, memspace,
+                        hinc.data()));
   hipStream_t st = as_stream(stream);
   const int64_t nI = (int64_t)n_inc * n_phi;
   double* dx = const_cast<double*>(x);

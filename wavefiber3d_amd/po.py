@@ -127,7 +127,7 @@ def _make_opts(device: int = -1, precision: str = "bf16x3", beta: float = 4.0, b
                box_enlarge: float = 1.1, lambda_sample_m: float = 0.0, strict_grid: bool = False,
                workspace_cap_bytes: int = 0, field_chunk: int = 0,
                cancel_flag: ct.c_int32 | None = None, use_graph: bool = False,
-               solve_streams: int = 0) -> _lib.Opts:
+               solve_streams: int = 0, ref_sampling=None) -> _lib.Opts:
     o = _lib.Opts()
     lib().wf_default_opts(ct.byref(o))
     o.device = device
@@ -141,6 +141,8 @@ def _make_opts(device: int = -1, precision: str = "bf16x3", beta: float = 4.0, b
     o.field_chunk = field_chunk
     o.use_graph = int(use_graph)
     o.solve_streams = int(solve_streams)
+    if ref_sampling is not None:  # (d_theta, d_phi) of the reference's sampling rule
+        o.ref_dtheta_m, o.ref_dphi_m = float(ref_sampling[0]), float(ref_sampling[1])
     if box is not None:
         o.has_box = 1
         for i in range(6):
@@ -183,9 +185,10 @@ class POSolver:
                  box=None, box_enlarge: float = 1.1, lambda_sample_m: float = 0.0,
                  strict_grid: bool = False, workspace_cap_bytes: int = 0,
                  field_chunk: int = 0, cancel_flag: ct.c_int32 | None = None,
-                 use_graph: bool = False, solve_streams: int = 0):
+                 use_graph: bool = False, solve_streams: int = 0, ref_sampling=None):
         o = _make_opts(device, precision, beta, box, box_enlarge, lambda_sample_m, strict_grid,
-                       workspace_cap_bytes, field_chunk, cancel_flag, use_graph, solve_streams)
+                       workspace_cap_bytes, field_chunk, cancel_flag, use_graph, solve_streams,
+                       ref_sampling)
         self._cancel = cancel_flag
         self._h = ct.c_void_p()
         check(lib().wf_create(ct.byref(self._h), ct.byref(o)))
@@ -414,6 +417,7 @@ class POSolver:
         return {
             "ms_total": s.ms_total, "kernels": kern, "leaf_pairs": s.leaf_pairs, "depth": s.depth,
             "levels": [(s.level_nt[i], s.level_np[i], s.level_nodes[i]) for i in range(nl)],
+            "agg_transitions": s.agg_transitions, "agg_bytes": s.agg_bytes,
         }
 
 
