@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="wf_autotune before the warm-up")
     ap.add_argument("--agg-fft-rows", type=int, default=None,
                     help="tune.agg_fft_rows: levels with 2 nt above it use the FFT transition")
+    ap.add_argument("--agg-fused", type=int, default=None, choices=[0, 1],
+                    help="tune.agg_fused: one-kernel internal level transitions (1) or two passes")
     ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
     ap.add_argument("--streams", type=int, default=4, help="fibers256: worker streams")
     ap.add_argument("--graph", action="store_true", help="capture solves into hipGraphs")
@@ -217,6 +219,8 @@ def main():
                          solve_streams=args.solve_streams)
     if args.agg_fft_rows is not None:
         solver.tune = {"agg_fft_rows": args.agg_fft_rows}
+    if args.agg_fused is not None:
+        solver.tune = {"agg_fused": args.agg_fused}
     t0 = time.perf_counter()
     solver.set_geometry(dev, W["depth"])
     torch.cuda.synchronize()

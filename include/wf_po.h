@@ -276,6 +276,8 @@ typedef struct wf_tune {
                              rows and the LDS FFT above; 1 = the original LDS FFT kernels */
   int32_t agg_fft_rows;   /* see agg_kernel (default 600) */
   int32_t calibrated;     /* 1 after wf_autotune (or a wf_set_tune of a calibrated tune) */
+  int32_t agg_fused;      /* 1: internal MFMA level transitions whose grids fit run as one
+                             kernel (phi output kept in LDS); 0 (default): two passes */
 } wf_tune;
 
 wf_status wf_get_tune(wf_handle* h, wf_tune* out);

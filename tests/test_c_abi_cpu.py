@@ -49,3 +49,21 @@ def test_c_caller_error_paths(tmp_path):
     r = subprocess.run([exe, "errors"], capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "errors: ok" in r.stdout
+
+
+def test_analytic_normal_derivative_of_currents(tmp_path):
+    """The reverse pass's dL/dn uses the analytic derivative of the Fresnel
+    currents (wavefiber3d_amd/csrc/wf_currents.h, current_point_dn); on the
+    host it matches central differences of the currents to 1e-6 over random
+    lit points, normals and incidences (tests/c_abi/currents_dn_check.cpp)."""
+    hipcc = "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else shutil.which("hipcc")
+    if not hipcc:
+        pytest.skip("hipcc not found")
+    exe = os.path.join(tmp_path, "cdn")
+    r = subprocess.run([hipcc, "-std=c++17", "-O2",
+                        "-I" + os.path.join(ROOT, "wavefiber3d_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "c_abi", "currents_dn_check.cpp"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -1,8 +1,9 @@
 """GPU parity tests: each HIP stage through the C-ABI vs the FP64 oracle.
 
-Bars: FP64 paths <= 1e-12 relative; FP32 stages <= 1e-5; tree solves <= 1e-3
-relative L2 of intensity against the FP64 depth-0 direct sum (the gate of the
-tree path), plus exact integer parity for the octree.
+Bars: FP64 paths <= 1e-12 relative; FP32 stages <= 1e-5; tree solves <= 1e-4
+relative L2 of intensity against the FP64 oracle (the tree path's gate is
+1e-3; its observed error is 2e-6 to 2e-5), plus exact integer parity for the
+octree.
 """
 import numpy as np
 import pytest
@@ -89,15 +90,18 @@ def test_octree_matches_reference_partition(solver, depth):
     np.testing.assert_array_equal(ne, re_)
 
 
-@pytest.fixture(params=[(0, 100000), (0, 0), (1, 288)], ids=["agg_mfma", "agg_fft2", "agg_fft"])
+@pytest.fixture(params=[(0, 100000, 1), (0, 100000, 0), (0, 0, 1), (1, 288, 1)],
+                ids=["agg_mfma_fused", "agg_mfma_2pass", "agg_fft2", "agg_fft"])
 def agg_solver(request):
     """A solver with the level-transition kernels selected: dense split-BF16
-    MFMA interpolation on every level, the large-grid LDS FFT kernels on every
-    level (5-smooth grids), or the original LDS FFT kernels."""
+    MFMA interpolation on every level (internal levels in the one-kernel fused
+    form, or as the phi and theta passes), the large-grid LDS FFT kernels on
+    every level (5-smooth grids), or the original LDS FFT kernels."""
     import wavefiber3d_amd as wfm
 
     s = wfm.POSolver(device=0)
-    s.tune = {"agg_kernel": request.param[0], "agg_fft_rows": request.param[1]}
+    s.tune = {"agg_kernel": request.param[0], "agg_fft_rows": request.param[1],
+              "agg_fused": request.param[2]}
     yield s
     s.close()
 
@@ -139,7 +143,7 @@ def test_tree_sphere_vs_fp64(agg_solver, depth):
     I = solver.solve(INCS, (nt, np_))
     Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS], nt, np_)
     for k in range(len(INCS)):
-        assert rel(I[k], Iref[k]) < 1e-3, (k, rel(I[k], Iref[k]))
+        assert rel(I[k], Iref[k]) < 1e-4, (k, rel(I[k], Iref[k]))
 
 
 @pytest.mark.parametrize("depth", [2, 3])
@@ -159,7 +163,7 @@ def test_tree_root_reads_deeper_level(agg_solver, depth):
     assert lv[1][0] > nt or lv[1][1] > np_  # level 1 finer than the output grid
     Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS], nt, np_)
     for k in range(len(INCS)):
-        assert rel(I[k], Iref[k]) < 1e-3, (k, rel(I[k], Iref[k]))
+        assert rel(I[k], Iref[k]) < 1e-4, (k, rel(I[k], Iref[k]))
 
 
 def test_tree_complex_E_vs_fp64(solver):
@@ -169,8 +173,8 @@ def test_tree_complex_E_vs_fp64(solver):
     solver.set_geometry(s, 2)
     I, E = solver.solve(INCS[:2], (nt, np_), return_E=True)
     Iref, Eref = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in INCS[:2]], nt, np_)
-    for k in range(2):
-        assert rel(E[k].reshape(3, -1), Eref[k]) < 1e-3
+    for k in range(2):  # complex E carries the phase error as well: 2.4e-4 observed
+        assert rel(E[k].reshape(3, -1), Eref[k]) < 5e-4
 
 
 def test_tree_ellipsoid_fixture(agg_solver):
@@ -187,15 +191,15 @@ def test_tree_ellipsoid_fixture(agg_solver):
     print("case: ellipsoid fixture depth=3", flush=True)
     solver.set_geometry(s, 3)
     I = solver.solve(incs, (nt, np_))
-    Id = solver.solve_fp64_direct(s, incs, (nt, np_))
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs], nt, np_)
     for k in range(2):
-        assert rel(I[k], Id[k]) < 1e-3
+        assert rel(I[k], Iref[k]) < 1e-4, (k, rel(I[k], Iref[k]))
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp32_valu"])
 def test_tree_precision_modes(precision):
     """Leaf GEMM on FP32 MFMA, split-BF16x3 MFMA and the vector ALU: each tree
-    solve within 1e-3 of the FP64 oracle, and MFMA FP32 vs vector FP32 close."""
+    solve within 1e-4 of the FP64 oracle, and MFMA FP32 vs vector FP32 close."""
     s = wf.fibonacci_sphere(8000, 1.5e-6)
     # 7 identical solves share one wavelength group (F = 42 fields, 6 MFMA tiles)
     incs = INCS[:2] + [wf.Incidence(0.2, 0.4, 0.45e-6, "M", 1.55 + 0.05j)] * 7
@@ -206,6 +210,31 @@ def test_tree_precision_modes(precision):
         I = sv.solve(incs, (nt, np_))
     Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs[:3]], nt, np_)
     for k in range(3):
-        assert rel(I[k], Iref[k]) < 1e-3, (precision, k, rel(I[k], Iref[k]))
+        assert rel(I[k], Iref[k]) < 1e-4, (precision, k, rel(I[k], Iref[k]))
     for k in range(3, len(incs)):
         assert rel(I[k], I[2]) < 1e-6  # identical incidences in other field slots
+
+
+@pytest.mark.parametrize("depth", [3, 4])
+def test_fused_transition_matches_two_pass(depth):
+    """The one-kernel internal level transition (phi output in LDS) and the
+    two-pass MFMA transition use the same split-BF16 matrices: their trees
+    agree to FP32 rounding, both meet the oracle, and the fused form runs no
+    separate phi pass below the root."""
+    s = wf.fibonacci_sphere(60000, 4e-6)
+    nt, np_ = 60, 120
+    incs = INCS + [wf.Incidence(0.1 * k, 0.3 * k, 0.5e-6, "ME"[k % 2], 1.5 + 0.05j) for k in range(4)]
+    print(f"case: fused vs two-pass transitions, sphere n=60000 depth={depth}", flush=True)
+    out = {}
+    for fused in (1, 0):
+        with wf.POSolver(device=0, solve_streams=1) as sv:
+            sv.tune = {"agg_fused": fused, "agg_fft_rows": 100000}
+            sv.set_geometry(s, depth)
+            out[fused] = (sv.solve(incs, (nt, np_)), sv.stats())
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs], nt, np_)
+    for k in range(len(incs)):
+        assert rel(out[1][0][k], out[0][0][k]) < 2e-6, (k, rel(out[1][0][k], out[0][0][k]))
+        assert rel(out[1][0][k], Iref[k]) < 1e-4
+    phi_fused = out[1][1]["kernels"].get("phi_pass", {"launches": 0})["launches"]
+    phi_2pass = out[0][1]["kernels"]["phi_pass"]["launches"]
+    assert phi_fused < phi_2pass

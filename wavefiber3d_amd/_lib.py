@@ -123,6 +123,7 @@ class Tune(ct.Structure):
         ("agg_kernel", ct.c_int32),
         ("agg_fft_rows", ct.c_int32),
         ("calibrated", ct.c_int32),
+        ("agg_fused", ct.c_int32),
     ]
 
 

@@ -18,8 +18,8 @@
 // the gradient is then assembled in FP64:
 //   dL/dx = 2 Re sum_f [ i k C_f conj(Q_f) ]                (radiation phase)
 //         + 2 Re sum_f [ conj(Q0_f) dC_f/dx ]               (incident phase, taper)
-//   dL/dn = 2 Re sum_f [ conj(Q0_f) dC_f/dn ]               (Fresnel currents; FP64 central
-//                                                            differences, h = 1e-6)
+//   dL/dn = 2 Re sum_f [ conj(Q0_f) dC_f/dn ]               (Fresnel currents; analytic
+//                                                            derivative, current_point_dn)
 //   dL/da = 2 Re sum_f [ conj(Q0_f) C_f / a ]
 #include "wf_adjoint.h"
 #include "wf_common.h"
@@ -373,23 +373,16 @@ __global__ __launch_bounds__(256) void grad_assemble_kernel(const float* __restr
 #pragma unroll
   for (int j = 0; j < 3; ++j) gx[j] += 2.0 * k * inc.w[j] * s0im;
   if (inc.sigma > 0.0) gx[2] += 2.0 * s0re * (-p[2] / (inc.sigma * inc.sigma));
-  // normals: FP64 central differences of the currents
+  // normals: the analytic derivative of the Fresnel currents along each axis
   double gn[3] = {0, 0, 0};
-  const double h = 1e-6;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    double np_[3] = {nr[0], nr[1], nr[2]}, nm[3] = {nr[0], nr[1], nr[2]};
-    np_[j] += h;
-    nm[j] -= h;
-    double2 Cp[6], Cm[6];
-    wfdev::current_point(p, np_, area, inc, Cp);
-    wfdev::current_point(p, nm, area, inc, Cm);
+    const double v[3] = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0};
+    double2 dC[6];
+    wfdev::current_point_dn(p, nr, area, inc, v, dC);
     double sacc = 0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const double2 dC = zscale(zsub(Cp[c], Cm[c]), 0.5 / h);
-      sacc += q[c][0].x * dC.x + q[c][0].y * dC.y;
-    }
+    for (int c = 0; c < 6; ++c) sacc += q[c][0].x * dC[c].x + q[c][0].y * dC[c].y;
     gn[j] = 2.0 * sacc;
   }
   acc[i] += gx[0];

@@ -1,0 +1,374 @@
+// wf_agg_fused.hip -- one level transition of the tree aggregation in a
+// single kernel: phi interpolation, theta interpolation over the great
+// circle, translation and child sum, with the phi-pass output kept in LDS.
+//
+// The reference resamples every (child, component) pattern with cuFFT plans
+// (interpolation.h:11-92), translates it (fields.h:92-103) and adds it into
+// the parent (fields.h:153-170), one thrust call or kernel per step.  Here
+// resampling is the dense Dirichlet-kernel map of the MFMA transition
+// (wf_agg_mfma.hip): for a child pattern X [ntc][npc],
+//
+//   G[theta][q]  = sum_j X[theta][j] Mphi[q][j]                   (q < npp)
+//   U[e][k]      = sum_j' Mth[e][j'] G'[j'][k]                   (e < 2 ntp)
+//   parent(e, k) += T_octant(e, k) U[e][k]
+//
+// with G' the column pair (k, k + npp/2) stored as one great-circle row
+// (column k theta-ascending, then column k + npp/2 theta-ascending, each
+// padded to gc_half_len(ntc); the theta matrix's columns are permuted to
+// that order).  Both GEMMs run on v_mfma_f32_16x16x32_bf16 with split BF16
+// x3 operands (hi*hi + hi*lo + lo*hi) and FP32 accumulation.
+//
+// Work item = (parent, field, pair tile of up to 64 column pairs).  A block
+// of NW waves walks items with a grid stride; for every child of the item:
+//   1. phi GEMM: units (16 child theta rows) x (16 parent phi columns of the
+//      tile) spread over the waves; the A operand (child rows) is read
+//      straight from HBM into registers, the B operand (phi matrix rows) from
+//      L2; the result is split into BF16 hi/lo and written to LDS in the B
+//      layout of the theta GEMM;
+//   2. barrier;
+//   3. theta GEMM: wave w owns e-tiles w, w + NW (the theta matrix
+//      fragments stay in registers for the block's life); per 16-pair tile
+//      of columns, the MFMA result is multiplied by the translation factor
+//      of the child's octant and added into the parent accumulators;
+//   4. barrier.
+// Each parent sample is written once, after its last child.  The compulsory
+// HBM traffic is what remains: every child pattern read once per pair tile
+// (once when npp/2 <= 64) and every parent written once.
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "wf_common.h"
+#include "wf_interp.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+constexpr int kMaxK = 4;              // 16-pair column tiles per item
+constexpr int kNcol = 16 * kMaxK;     // column pairs per item
+
+__device__ inline void split8(const float v[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 h = (__bf16)v[j];
+    hi[j] = h;
+    lo[j] = (__bf16)(v[j] - (float)h);
+  }
+}
+
+__device__ inline floatx4 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh,
+                                const bf16x8& bl, floatx4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+struct FusedK {
+  const float2* X;  // child patterns [child][F][ntc][npc]
+  float2* out;      // parent patterns [parent][F][ntp][npp]
+  int F, ntc, npc, ntp, npp;
+  int half, h8, nth, net, npt;  // npp/2, gc_half_len(ntc), child theta tiles, e-tiles, pair tiles
+  int nkp, nkt;                 // K steps of the phi and theta GEMMs in use
+  int mirror;
+  int64_t nitems, used;
+  const int32_t* cb;
+  const int32_t* ce;
+  const uint32_t* child_key;
+  const float2* Text;  // [8][2 ntp + 1][npp/2]
+  const bf16x8* Mphi_hi;
+  const bf16x8* Mphi_lo;
+  int kpad_phi, mrows_phi;
+  const bf16x8* Mth_hi;
+  const bf16x8* Mth_lo;
+  int kpad_th, mrows_th;
+};
+
+// NKP / NKT: maximum K steps (32 samples) of the phi / theta GEMMs; MAXE:
+// e-tiles per wave.  LDS: the theta GEMM's B operand for one child,
+// [NKT][4 planes][kNcol columns][4 k groups] of 8 BF16 (planes: re hi,
+// re lo, im hi, im lo), so an MFMA lane (column c16, k group g) reads 16
+// consecutive bytes and a wave reads 1 KB contiguously.
+template <int NKP, int NKT, int MAXE>
+__global__ __launch_bounds__(MAXE == 1 ? 768 : 512) void level_fused_mfma_kernel(const FusedK P) {
+  __shared__ bf16x8 Gs[NKT * 4 * kNcol * 4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int NW = blockDim.x >> 6;
+  const int c16 = lane & 15, g = lane >> 4;
+  // XCD-aware: blocks b = x (mod 8) run on XCD x and take a contiguous range
+  const int64_t nb = gridDim.x, per = (nb + 7) / 8;
+  const int64_t w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (w >= P.used) return;  // whole block
+  const int F = P.F, ntc = P.ntc, npc = P.npc, ntp = P.ntp, npp = P.npp;
+  const int half = P.half, h8 = P.h8, n2p = 2 * ntp;
+  // the padding of the LDS operand (columns beyond the pair tile, K beyond
+  // the great-circle row) stays zero for the block's life
+  {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
+    for (int i = threadIdx.x; i < NKT * 4 * kNcol * 4; i += blockDim.x) Gs[i] = z;
+  }
+  // resident theta-matrix fragments: A lane (row e = 16 et + c16, k group g)
+  bf16x8 ah[MAXE][NKT], al[MAXE][NKT];
+#pragma unroll
+  for (int a = 0; a < MAXE; ++a) {
+    const int et = wave + a * NW;
+    const int row = 16 * et + c16;
+    const int rr = row < P.mrows_th ? row : 0;
+    const bf16x8* ph = P.Mth_hi + (((int64_t)rr * P.kpad_th) >> 3);
+    const bf16x8* pl = P.Mth_lo + (((int64_t)rr * P.kpad_th) >> 3);
+#pragma unroll
+    for (int ks = 0; ks < NKT; ++ks) {
+      const int kq = ks < P.nkt ? ks * 4 + g : 0;
+      ah[a][ks] = ph[kq];
+      al[a][ks] = pl[kq];
+    }
+  }
+  __syncthreads();
+  const int nunits_max = P.nth * 2 * kMaxK;
+  for (int64_t item = w; item < P.nitems; item += P.used) {
+    const int pt = (int)(item % P.npt);
+    const int64_t pf = item / P.npt;
+    const int p = (int)(pf / F), f = (int)(pf - (int64_t)p * F);
+    const int kbase = pt * kNcol;
+    const int nk_here = min(kMaxK, (half - kbase + 15) >> 4);  // 16-pair tiles in use
+    float2 acc[MAXE][kMaxK][4];
+#pragma unroll
+    for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+      for (int b = 0; b < kMaxK; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[a][b][j] = make_float2(0.f, 0.f);
+    const int c_begin = P.cb[p], c_end = P.ce[p];
+    for (int c = c_begin; c < c_end; ++c) {
+      const float2* Xc = P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc);
+      // ---- phi GEMM into LDS
+      for (int u = wave; u < nunits_max; u += NW) {
+        const int t = u / (2 * kMaxK), n = u - t * (2 * kMaxK);
+        const int nt16 = n % kMaxK;
+        if (nt16 >= nk_here) continue;  // whole wave: no column of this tile in use
+        const bool second = n >= kMaxK;
+        const int kk = kbase + 16 * nt16 + c16;  // this lane's column pair
+        const bool kok = kk < half;
+        const int q = second ? kk + half : kk;  // its phi column
+        const int qq = (kok && q < P.mrows_phi) ? q : 0;
+        const int th = 16 * t + c16;  // A row of this lane
+        const bool rok = th < ntc;
+        const float2* xr = Xc + (int64_t)(rok ? th : 0) * npc;
+        floatx4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < NKP; ++ks) {
+          if (ks < P.nkp) {
+            const int j0 = 32 * ks + 8 * g;
+            float vr[8], vi[8];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+              const int jj = j0 + 2 * h;  // npc even: a pair never straddles the row end
+              float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+              if (rok && jj < npc) v = *reinterpret_cast<const float4*>(xr + jj);
+              vr[2 * h] = v.x;
+              vi[2 * h] = v.y;
+              vr[2 * h + 1] = v.z;
+              vi[2 * h + 1] = v.w;
+            }
+            bf16x8 arh, arl, aih, ail;
+            split8(vr, arh, arl);
+            split8(vi, aih, ail);
+            const int64_t mo = (((int64_t)qq * P.kpad_phi) >> 3) + ks * 4 + g;
+            const bf16x8 bh = P.Mphi_hi[mo], bl = P.Mphi_lo[mo];
+            dr = mfma3(arh, arl, bh, bl, dr);
+            di = mfma3(aih, ail, bh, bl, di);
+          }
+        }
+        // D lane: child theta rows 16 t + 4 g + j, column q; stored at
+        // great-circle index (second ? h8 : 0) + theta of column pair kk
+        const int th0 = 16 * t + 4 * g;
+        if (kok && th0 < h8) {
+          const int jg = (second ? h8 : 0) + th0;
+          const int ks = jg >> 5, g8 = (jg >> 3) & 3, hoff = (jg >> 2) & 1;
+          const int col = 16 * nt16 + c16;
+          bf16x4 rh, rl, ih, il;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const __bf16 hr = (__bf16)dr[j], hi_ = (__bf16)di[j];
+            rh[j] = hr;
+            rl[j] = (__bf16)(dr[j] - (float)hr);
+            ih[j] = hi_;
+            il[j] = (__bf16)(di[j] - (float)hi_);
+          }
+          bf16x4* base = reinterpret_cast<bf16x4*>(Gs);
+          const int unit = ((ks * 4) * kNcol + col) * 4 + g8;  // plane 0
+          base[2 * unit + hoff] = rh;
+          base[2 * (unit + 1 * kNcol * 4) + hoff] = rl;
+          base[2 * (unit + 2 * kNcol * 4) + hoff] = ih;
+          base[2 * (unit + 3 * kNcol * 4) + hoff] = il;
+        }
+      }
+      __syncthreads();
+      // ---- theta GEMM, translation, child sum
+      const int oct = (int)(P.child_key[c] & 7u);
+      const float2* To = P.Text + (int64_t)oct * (n2p + 1) * half;
+#pragma unroll
+      for (int b = 0; b < kMaxK; ++b) {
+        if (b < nk_here) {
+          const int kk = kbase + 16 * b + c16;
+          const bool kok = kk < half;
+          float2 tf[MAXE][4];
+#pragma unroll
+          for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int e = 16 * (wave + a * NW) + 4 * g + j;
+              tf[a][j] = (kok && e < n2p) ? To[(int64_t)e * half + kk] : make_float2(0.f, 0.f);
+            }
+          floatx4 ur[MAXE], ui[MAXE];
+#pragma unroll
+          for (int a = 0; a < MAXE; ++a) {
+            ur[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+            ui[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+          }
+#pragma unroll
+          for (int ks = 0; ks < NKT; ++ks) {
+            if (ks < P.nkt) {
+              const int col = 16 * b + c16;
+              const bf16x8 brh = Gs[((ks * 4 + 0) * kNcol + col) * 4 + g];
+              const bf16x8 brl = Gs[((ks * 4 + 1) * kNcol + col) * 4 + g];
+              const bf16x8 bih = Gs[((ks * 4 + 2) * kNcol + col) * 4 + g];
+              const bf16x8 bil = Gs[((ks * 4 + 3) * kNcol + col) * 4 + g];
+#pragma unroll
+              for (int a = 0; a < MAXE; ++a) {
+                ur[a] = mfma3(ah[a][ks], al[a][ks], brh, brl, ur[a]);
+                ui[a] = mfma3(ah[a][ks], al[a][ks], bih, bil, ui[a]);
+              }
+            }
+          }
+#pragma unroll
+          for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float2 tu = cmul(tf[a][j], make_float2(ur[a][j], ui[a][j]));
+              acc[a][b][j].x += tu.x;
+              acc[a][b][j].y += tu.y;
+            }
+        }
+      }
+      __syncthreads();  // the next child's phi pass overwrites the LDS operand
+    }
+    // ---- the parent samples of this item, written once
+    float2* O = P.out + ((int64_t)p * F + f) * ((int64_t)ntp * npp);
+#pragma unroll
+    for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+      for (int b = 0; b < kMaxK; ++b) {
+        const int kk = kbase + 16 * b + c16;
+        if (b < nk_here && kk < half) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int e = 16 * (wave + a * NW) + 4 * g + j;
+            if (e < n2p) {
+              int row = e, col = kk;
+              if (e >= ntp) {
+                row = P.mirror - e;
+                col = kk + half;
+              }
+              if (row >= 0 && row < ntp) O[(int64_t)row * npp + col] = acc[a][b][j];
+            }
+          }
+        }
+      }
+  }
+}
+
+template <int NKP, int NKT, int MAXE>
+hipError_t launch_fused_t(const FusedK& P, int nw, hipStream_t st) {
+  // resident blocks: LDS NKT * 16 KB per block, waves per SIMD by register use
+  const int lds_kb = NKT * 16;
+  int bpc = std::max(1, std::min(4, 160 / std::max(lds_kb, 1)));
+  bpc = std::min(bpc, std::max(1, 16 / nw));
+  FusedK Q = P;
+  int64_t used = std::min<int64_t>(256LL * bpc, Q.nitems);
+  Q.used = used;
+  const int64_t nb = ((used + 7) / 8) * 8;
+  hipLaunchKernelGGL((level_fused_mfma_kernel<NKP, NKT, MAXE>), dim3((unsigned)nb), dim3(64 * nw), 0, st,
+                     Q);
+  return hipGetLastError();
+}
+
+template <int NKP>
+hipError_t launch_fused_k(const FusedK& P, int nkt, int maxe, int nw, hipStream_t st) {
+  if (maxe == 1) {
+    if (nkt <= 2) return launch_fused_t<NKP, 2, 1>(P, nw, st);
+    if (nkt == 3) return launch_fused_t<NKP, 3, 1>(P, nw, st);
+    if (nkt == 4) return launch_fused_t<NKP, 4, 1>(P, nw, st);
+    return launch_fused_t<NKP, 5, 1>(P, nw, st);
+  }
+  if (nkt <= 2) return launch_fused_t<NKP, 2, 2>(P, nw, st);
+  if (nkt == 3) return launch_fused_t<NKP, 3, 2>(P, nw, st);
+  if (nkt == 4) return launch_fused_t<NKP, 4, 2>(P, nw, st);
+  return launch_fused_t<NKP, 5, 2>(P, nw, st);
+}
+
+}  // namespace
+
+namespace wf {
+
+bool agg_fused_fits(int ntc, int npc, int ntp, int npp) {
+  const int nkp = (npc + 31) / 32, nkt = (gc_row_len(ntc) + 31) / 32;
+  const int net = (2 * ntp + 15) / 16;
+  return ntc >= 1 && npc >= 2 && (npc % 2) == 0 && npp >= 2 && (npp % 2) == 0 && nkp <= 4 &&
+         nkt <= 5 && net <= 16;
+}
+
+hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st) {
+  if (a.nparents <= 0) return hipSuccess;
+  if (!agg_fused_fits(a.ntc, a.npc, a.ntp, a.npp) || a.F < 1 || !a.cb || !a.ce || !a.child_key ||
+      !a.Text)
+    return hipErrorInvalidValue;
+  if (a.mirror != 2 * a.ntp - 1) return hipErrorInvalidValue;  // internal levels only
+  const int nkp = (a.npc + 31) / 32, nkt = (gc_row_len(a.ntc) + 31) / 32;
+  if (a.kpad_phi < 32 * nkp || (a.kpad_phi & 31) || a.kpad_th < 32 * nkt || (a.kpad_th & 31) ||
+      a.mrows_phi < a.npp || a.mrows_th < 2 * a.ntp)
+    return hipErrorInvalidValue;
+  FusedK P;
+  P.X = a.X;
+  P.out = a.out;
+  P.F = a.F;
+  P.ntc = a.ntc;
+  P.npc = a.npc;
+  P.ntp = a.ntp;
+  P.npp = a.npp;
+  P.half = a.npp / 2;
+  P.h8 = gc_half_len(a.ntc);
+  P.nth = (a.ntc + 15) / 16;
+  P.net = (2 * a.ntp + 15) / 16;
+  P.npt = (P.half + kNcol - 1) / kNcol;
+  P.nkp = nkp;
+  P.nkt = nkt;
+  P.mirror = a.mirror;
+  P.nitems = (int64_t)a.nparents * a.F * P.npt;
+  P.used = 0;
+  P.cb = a.cb;
+  P.ce = a.ce;
+  P.child_key = a.child_key;
+  P.Text = a.Text;
+  P.Mphi_hi = reinterpret_cast<const bf16x8*>(a.Mphi_hi);
+  P.Mphi_lo = reinterpret_cast<const bf16x8*>(a.Mphi_lo);
+  P.kpad_phi = a.kpad_phi;
+  P.mrows_phi = a.mrows_phi;
+  P.Mth_hi = reinterpret_cast<const bf16x8*>(a.Mth_hi);
+  P.Mth_lo = reinterpret_cast<const bf16x8*>(a.Mth_lo);
+  P.kpad_th = a.kpad_th;
+  P.mrows_th = a.mrows_th;
+  // one e-tile per wave up to 12 waves, else two
+  const int maxe = P.net <= 12 ? 1 : 2;
+  const int nw = (P.net + maxe - 1) / maxe;
+  if (nkp <= 2) return launch_fused_k<2>(P, nkt, maxe, nw, st);
+  if (nkp == 3) return launch_fused_k<3>(P, nkt, maxe, nw, st);
+  return launch_fused_k<4>(P, nkt, maxe, nw, st);
+}
+
+}  // namespace wf

@@ -481,6 +481,7 @@ void default_tune(wf_tune* t) {
   t->agg_kernel = 0;
   t->agg_fft_rows = 600;
   t->calibrated = 0;
+  t->agg_fused = 0;
 }
 
 // Deepest useful level of a WF_DEPTH_AUTO build: about one octant digit per
@@ -1384,6 +1385,42 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       if (l > 0) cur = nxt;
       continue;
     }
+    if (mfma_ok && l > 0 && h->tune.agg_fused && Tx_l &&
+        wf::agg_fused_fits(gcg.nt, gcg.np, gp.nt, gp.np) && mirror == 2 * gp.nt - 1) {
+      // one kernel: phi GEMM into LDS, theta GEMM, translation, child sum
+      MatEntry mp, mt;
+      WF_TRY(get_mat(h, gcg.np, gp.np, 0.0, &mp));
+      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true));
+      float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
+      wf::AggFusedArgs fa;
+      fa.X = cur;
+      fa.out = nxt;
+      fa.F = F;
+      fa.ntc = gcg.nt;
+      fa.npc = gcg.np;
+      fa.ntp = gp.nt;
+      fa.npp = gp.np;
+      fa.nparents = npar;
+      fa.cb = pcb;
+      fa.ce = pce;
+      fa.child_key = h->levels[lc].key;
+      fa.Text = Tx_l;
+      fa.Mphi_hi = mp.hi;
+      fa.Mphi_lo = mp.lo;
+      fa.kpad_phi = mp.kpad;
+      fa.mrows_phi = mp.rows_pad;
+      fa.Mth_hi = mt.hi;
+      fa.Mth_lo = mt.lo;
+      fa.kpad_th = mt.kpad;
+      fa.mrows_th = mt.rows_pad;
+      fa.mirror = mirror;
+      const double cbytes = 8.0 * planes * gcg.nt * gcg.np;
+      const double pbytes = 8.0 * F * (double)npar * gp.nt * gp.np;
+      WF_TRY(timed(h, st, WF_K_THETA, cbytes + pbytes,
+                   [&] { return wf::launch_agg_fused_mfma(fa, st); }));
+      cur = nxt;
+      continue;
+    }
     if (mfma_ok) {
       // dense split-BF16 MFMA interpolation: phi GEMM -> GT, theta GEMM +
       // translation + child sum -> parent (FP64 at the root)
@@ -1836,6 +1873,24 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
                    [&](int a, int b) { return inc[a].lambda_m < inc[b].lambda_m; });
   const double centre[3] = {0.5 * (h->box[0] + h->box[3]), 0.5 * (h->box[1] + h->box[4]),
                             0.5 * (h->box[2] + h->box[5])};
+  // The reverse kernels (theta_adj, phi_adj) are the exact adjoints of the
+  // FP32 LDS FFT level transitions, so the recomputed forward of a tree
+  // gradient runs those transitions (tune.agg_kernel = 1) whatever the
+  // handle's setting, and its leaf sum on FP32 MFMA: the gradient is then the
+  // adjoint of the forward it pairs with, up to FP32 rounding.
+  struct TuneGuard {
+    wf_handle* h;
+    wf_tune t;
+    int32_t prec;
+    ~TuneGuard() {
+      h->tune = t;
+      h->opts.precision = prec;
+    }
+  } guard{h, h->tune, h->opts.precision};
+  if (!direct) {
+    h->tune.agg_kernel = 1;
+    if (h->opts.precision == WF_PREC_BF16X3) h->opts.precision = WF_PREC_FP32;
+  }
   prof_reset(h);
   WF_TRY(prof_event(h, st, &h->ev_t0));
   const float* px = h->pts;
@@ -1952,8 +2007,10 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: coefficients must be finite and >= 0");
   if (t->chunk_solves < 0 || t->chunk_solves > 8)
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: chunk_solves must be in [0, 8]");
-  if (t->agg_kernel < 0 || t->agg_kernel > 1 || t->agg_fft_rows < 0)
-    return set_err(WF_ERR_INVALID_ARG, "wf_tune: agg_kernel must be 0 or 1, agg_fft_rows >= 0");
+  if (t->agg_kernel < 0 || t->agg_kernel > 1 || t->agg_fft_rows < 0 || t->agg_fused < 0 ||
+      t->agg_fused > 1)
+    return set_err(WF_ERR_INVALID_ARG,
+                   "wf_tune: agg_kernel and agg_fused must be 0 or 1, agg_fft_rows >= 0");
   h->tune = *t;
   h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
   return WF_OK;

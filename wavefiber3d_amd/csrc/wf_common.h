@@ -45,7 +45,7 @@ __device__ inline float2 cis_rev(double rev) {
 // (truncation below 1e-17 there), evaluated by Horner in t^2; the quarter
 // turn then rotates (cos t, sin t) by i^q.  No branches and no further range
 // reduction, unlike a general-purpose sincos.
-__device__ inline double2 zcis_rev(double rev) {
+__host__ __device__ inline double2 zcis_rev(double rev) {
   const double f = rev - rint(rev);
   const double q = rint(4.0 * f);
   const double t = (f - 0.25 * q) * WF_TWO_PI;

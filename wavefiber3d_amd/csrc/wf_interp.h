@@ -85,6 +85,30 @@ struct ThetaMfmaArgs {
 };
 hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
 
+// One internal level transition in a single kernel (wf_agg_fused.hip): phi
+// and theta interpolation with the split-BF16 matrices of the two-pass MFMA
+// transition, translation and child sum, the phi output kept in LDS.
+// Shapes: npc, npp even, ceil(npc/32) <= 4, ceil(gc_row_len(ntc)/32) <= 5,
+// 2 ntp <= 256; internal levels only (mirror = 2 ntp - 1).
+struct AggFusedArgs {
+  const float2* X;  // child patterns [child][F][ntc][npc]
+  float2* out;      // parent patterns [parent][F][ntp][npp]
+  int F, ntc, npc, ntp, npp, nparents;
+  const int32_t* cb;  // per-parent child range
+  const int32_t* ce;
+  const uint32_t* child_key;  // octant = key & 7
+  const float2* Text;         // [8][2 ntp + 1][npp/2] (launch_trans_ext)
+  const uint16_t* Mphi_hi;    // build_interp_matrix(npc, npp, 0)
+  const uint16_t* Mphi_lo;
+  int kpad_phi, mrows_phi;
+  const uint16_t* Mth_hi;  // build_theta_matrix_gc(ntc, 2 ntp, shift)
+  const uint16_t* Mth_lo;
+  int kpad_th, mrows_th;
+  int mirror;
+};
+bool agg_fused_fits(int ntc, int npc, int ntp, int npp);
+hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st);
+
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
                             const double h[3], double k0, hipStream_t st);
 // heightfield -> 7 SoA arrays (wf_geometry.hip); *first_bad receives the
