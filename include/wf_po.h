@@ -386,6 +386,28 @@ wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
                            double distance_m, double radius_m, double* intensity,
                            int32_t memspace, void* stream);
 
+/* Allocation-free variant of wf_rough2d_solve.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid allocation).
+ * Same arguments and results as wf_rough2d_solve with memspace
+ * WF_MEM_DEVICE (x, y and intensity are device pointers; inc is host memory),
+ * and the call likewise returns after the stream has finished.  Its scratch
+ * memory -- the incidence parameters, the currents (96 B per sample and
+ * incidence) and the partial sums of the azimuth kernel (96 B per split,
+ * incidence and azimuth) -- is one static device array of
+ * WF_ROUGH2D_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+ * larger problems, or more than WF_ROUGH2D_NOALLOC_MAX_INC incidences, are
+ * rejected with WF_ERR_INVALID_ARG.  Every call on a GPU shares that array
+ * and it keeps the last call's contents, so calls that overlapped on one GPU
+ * would race on it: the library serialises the calls of a process (each
+ * holds a lock until its stream has finished), so concurrent callers wait. */
+#define WF_ROUGH2D_NOALLOC_SCRATCH_BYTES (64 << 20)
+#define WF_ROUGH2D_NOALLOC_MAX_INC 1024
+wf_status wf_rough2d_solve_noalloc(const double* x, const double* y, int32_t numel,
+                                   const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
+                                   double distance_m, double radius_m, double* intensity,
+                                   void* stream);
+
 /* Reads a fiber heightfield file (treePO.cu:79-88: raw float32, 2 * numel *
  * numelvert values in host byte order) into host memory; a missing or short
  * file is an error. */

@@ -32,6 +32,7 @@ EXPORTS = (
     "wf_get_tune", "wf_set_tune", "wf_autotune", "wf_batch_create", "wf_batch_destroy",
     "wf_batch_add", "wf_batch_solve", "wf_batch_info", "wf_heightfield_points",
     "wf_read_heightfield", "wf_average_grid", "wf_po_solve_fp64_finite", "wf_rough2d_solve",
+    "wf_rough2d_solve_noalloc",
 )
 WF_DEPTH_AUTO = -1
 
@@ -182,6 +183,9 @@ def lib(path: str | None = None):
     L.wf_read_heightfield.argtypes = [ct.c_char_p, ct.c_int32, ct.c_int32, vp]
     L.wf_rough2d_solve.argtypes = [vp, vp, ct.c_int32, ct.POINTER(Rough2dIncidence), ct.c_int32,
                                    ct.c_int32, ct.c_double, ct.c_double, vp, ct.c_int32, vp]
+    L.wf_rough2d_solve_noalloc.argtypes = [vp, vp, ct.c_int32, ct.POINTER(Rough2dIncidence),
+                                           ct.c_int32, ct.c_int32, ct.c_double, ct.c_double, vp,
+                                           vp]
     L.wf_po_solve_fp64_finite.argtypes = [vp, ct.POINTER(Surface), ct.POINTER(Incidence),
                                           ct.c_int32, Grid, ct.c_double, vp, vp, vp]
     L.wf_average_grid.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32,

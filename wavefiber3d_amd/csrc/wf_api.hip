@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2612,6 +2613,75 @@ wf_status wf_rough2d_solve(const double* x, const double* y, int32_t numel,
   const hipError_t es = hipStreamSynchronize(st);
   if (e == hipSuccess) e = es;
   if (e != hipSuccess) return set_err(WF_ERR_HIP, "wf_rough2d_solve: %s", hipGetErrorString(e));
+  if (hbad != 0x7fffffff)
+    return set_err(WF_ERR_INVALID_ARG, "cross-section segment %d has zero length or a non-finite vertex",
+                   hbad);
+  return WF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free 2D cross-section solve.  Its scratch memory (incidence
+// parameters, currents, partial sums, the bad-segment flag) lives in one
+// file-scope device array; every call on a device shares that device's copy,
+// so calls that overlapped would race: a lock held until the call's stream has
+// finished serialises them.  In host code the array's name
+// stands for a host-side symbol, so kernels receive the device address that
+// hipGetSymbolAddress returns, looked up once per device.
+static __device__ __align__(256) unsigned char wf_r2_scratch[WF_ROUGH2D_NOALLOC_SCRATCH_BYTES];
+static void* g_r2_scratch_ptr[64] = {nullptr};
+static std::mutex g_r2_mutex;  // the host staging buffer and the per-device lookup
+static wf::Rough2dIncDev g_r2_hinc[WF_ROUGH2D_NOALLOC_MAX_INC];
+
+wf_status wf_rough2d_solve_noalloc(const double* x, const double* y, int32_t numel,
+                                   const wf_rough2d_incidence* inc, int32_t n_inc, int32_t n_phi,
+                                   double distance_m, double radius_m, double* intensity,
+                                   void* stream) {
+  if (n_inc < 1 || n_inc > WF_ROUGH2D_NOALLOC_MAX_INC)
+    return set_err(WF_ERR_INVALID_ARG, "wf_rough2d_solve_noalloc: n_inc must be in [1, %d]",
+                   WF_ROUGH2D_NOALLOC_MAX_INC);
+  std::lock_guard<std::mutex> lock(g_r2_mutex);
+  WF_TRY(rough2d_params(x, y, numel, inc, n_inc, n_phi, distance_m, radius_m, intensity,
+                        WF_MEM_DEVICE, g_r2_hinc));
+  // scratch layout: [incidences | currents | partial sums | flag], 256 B aligned
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const int P = wf::rough2d_split(numel, n_inc, n_phi);
+  const size_t b_inc = up(sizeof(wf::Rough2dIncDev) * (size_t)n_inc);
+  const size_t b_cur = up(sizeof(double2) * 6 * (size_t)numel * n_inc);
+  const size_t b_part = up(sizeof(double2) * 6 * (size_t)P * n_inc * n_phi);
+  const size_t need = b_inc + b_cur + b_part + 256;
+  if (need > (size_t)WF_ROUGH2D_NOALLOC_SCRATCH_BYTES)
+    return set_err(WF_ERR_INVALID_ARG,
+                   "wf_rough2d_solve_noalloc: needs %zu bytes of scratch (%d samples x %d "
+                   "incidences x %d azimuths), the maximum is %d",
+                   need, numel, n_inc, n_phi, WF_ROUGH2D_NOALLOC_SCRATCH_BYTES);
+  int dev = 0;
+  WF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(WF_ERR_INVALID_ARG, "device ordinal %d unsupported", dev);
+  if (!g_r2_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(wf_r2_scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(WF_ERR_HIP, "wf_rough2d_solve_noalloc: scratch lookup failed: %s",
+                     hipGetErrorString(e));
+    g_r2_scratch_ptr[dev] = ptr;
+  }
+  unsigned char* base = static_cast<unsigned char*>(g_r2_scratch_ptr[dev]);
+  wf::Rough2dIncDev* dinc = reinterpret_cast<wf::Rough2dIncDev*>(base);
+  double2* C = reinterpret_cast<double2*>(base + b_inc);
+  double2* part = reinterpret_cast<double2*>(base + b_inc + b_cur);
+  int* bad = reinterpret_cast<int*>(base + b_inc + b_cur + b_part);
+  hipStream_t st = as_stream(stream);
+  int hbad = 0x7fffffff;
+  hipError_t e = hipMemcpyAsync(dinc, g_r2_hinc, sizeof(wf::Rough2dIncDev) * n_inc,
+                                hipMemcpyHostToDevice, st);
+  // launch_rough2d clears the flag with a kernel before use
+  if (e == hipSuccess)
+    e = wf::launch_rough2d(x, y, numel, dinc, n_inc, n_phi, distance_m, radius_m, C, part, P,
+                           intensity, bad, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+  const hipError_t es = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = es;
+  if (e != hipSuccess) return set_err(WF_ERR_HIP, "wf_rough2d_solve_noalloc: %s", hipGetErrorString(e));
   if (hbad != 0x7fffffff)
     return set_err(WF_ERR_INVALID_ARG, "cross-section segment %d has zero length or a non-finite vertex",
                    hbad);

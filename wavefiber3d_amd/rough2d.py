@@ -83,3 +83,31 @@ def rough2d_solve(x, y, incidences, n_phi: int = 3600, distance: float = 1.0,
                                  xh.shape[0], arr, k, n_phi, distance, radius,
                                  out.ctypes.data_as(ct.c_void_p), WF_MEM_HOST, None))
     return out
+
+
+def rough2d_solve_noalloc(x, y, incidences, n_phi: int = 3600, distance: float = 1.0,
+                          radius: float = 10e-6, out=None, stream=None):
+    """wf_rough2d_solve_noalloc: the device path of ``rough2d_solve`` with no
+    device allocation inside the library (its scratch is a fixed static device
+    array; see the C header for the size limit).  x, y: float64 CUDA tensors;
+    ``out`` (optional) a float64 CUDA tensor [n_inc, n_phi] to write into."""
+    import torch
+
+    from ._lib import check, lib
+
+    incidences = list(incidences)
+    arr = _incs(incidences)
+    k = len(incidences)
+    if x.dtype != torch.float64 or y.dtype != torch.float64 or not (x.is_cuda and y.is_cuda):
+        raise ValueError("x and y must be float64 CUDA tensors")
+    if x.shape != y.shape or x.dim() != 1 or not (x.is_contiguous() and y.is_contiguous()):
+        raise ValueError("x and y must be contiguous 1-D tensors of equal length")
+    if out is None:
+        out = torch.empty((k, n_phi), dtype=torch.float64, device=x.device)
+    elif out.shape != (k, n_phi) or out.dtype != torch.float64 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 tensor [n_inc, n_phi]")
+    st = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+    check(lib().wf_rough2d_solve_noalloc(ct.c_void_p(x.data_ptr()), ct.c_void_p(y.data_ptr()),
+                                         x.shape[0], arr, k, n_phi, distance, radius,
+                                         ct.c_void_p(out.data_ptr()), ct.c_void_p(st)))
+    return out
