@@ -163,7 +163,7 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="wf_autotune before the warm-up")
     ap.add_argument("--agg-fft-rows", type=int, default=None,
                     help="tune.agg_fft_rows: levels with 2 nt above it use the FFT transition")
-    ap.add_argument("--agg-fused", type=int, default=None, choices=[0, 1],
+    ap.add_argument("--agg-fused", type=int, default=None, choices=[0, 1, 2],
                     help="tune.agg_fused: one-kernel internal level transitions (1) or two passes")
     ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
     ap.add_argument("--streams", type=int, default=4, help="fibers256: worker streams")
@@ -722,31 +722,29 @@ def bench_rough2d(args, W):
 
 
 def bench_adjoint(args, W):
-    """Reverse mode on the 200k-point fiber: forward and adjoint times of the
-    tree (autotuned depth) and of the FP64 direct sum, the tree gradient
-    against the FP64 gradient, and both against central differences of the
-    FP64 forward (h = 1e-4 um and h/2 on 64 seeded points displaced along their
-    normals, the pair exactly centred in float32, Richardson-extrapolated; the
-    cuticle tilt along the FP64 tangent of the surface recipe, 0.002 and 0.001
-    degrees, exactly centred, Richardson-extrapolated)."""
+    """Reverse mode on the 200k-point fiber: forward and adjoint times and
+    gradient parity of the tree with FP64 patterns (the default gradient
+    mode), the tree with FP32 patterns, and the FP64 direct sum (the
+    reference gradient), at the autotuned depth.  Each gradient is checked
+    against the FP64 direct-sum gradient (relative L2 of dL/d delta_n for the
+    normal displacement x -> x + delta n) and against central differences of
+    the FP64 forward: 64 seeded points moved along their normals by 1e-4 um
+    and h/2, and the cuticle tilt along the FP64 tangent of the surface
+    recipe at 0.002 and 0.001 degrees, each step exactly centred in float32
+    and the ratio FD / prediction Richardson-extrapolated
+    (wavefiber3d_amd/gradcheck.py).  Gate: 1e-4 relative."""
     import torch
 
     import wavefiber3d_amd as wf
-    from wavefiber3d_amd.autograd import fiber_surface_torch, log_intensity_loss, po_intensity
+    from wavefiber3d_amd import gradcheck as gc
+    from wavefiber3d_amd.autograd import log_intensity_loss, po_intensity
 
     incs, grid = W["incs"], W["grid"]
-    kw = dict(r1=40e-6, r2=30e-6, length=10e-6, spacing=0.1e-6, seed=0)
-
-    def surface(tau):
-        s = fiber_surface_torch(tau, **kw)
-        numel = s[0].numel() // 100  # 100 rings, ring-major
-        keep = (torch.arange(s[0].numel(), device=s[0].device) % numel) < 2000
-        return [t[keep] for t in s]
-
     tau0 = torch.tensor(3.0, dtype=torch.float64, device="cuda")
-    s64 = [t.detach() for t in surface(tau0)]
+    s64 = [t.detach() for t in gc.c5_surface(tau0)]
     s32 = [t.float().contiguous() for t in s64]
     n = s32[0].numel()
+    nrm = torch.stack(s64[3:6])
     out = {"metric": "adjoint: forward + reverse time and gradient parity (auxiliary workload)",
            "unit": "ms", "n_gpus": 1, "data": "synthetic", "higher_is_better": False,
            "config": {"workload": args.workload, "description": W["desc"], "n_points": n,
@@ -768,154 +766,47 @@ def bench_adjoint(args, W):
         sv.set_geometry(s32, "auto")
         best = sv.autotune(incs, grid)
         ms_fwd, I = sync_time(lambda: sv.solve(incs, grid, out="torch"))
-        depth_used = sv.stats()["depth"]
+        depth = sv.stats()["depth"]
+        sv.set_geometry(s32, depth)
         gI = (2.0 * (torch.log(I) - torch.log(target)) / I).contiguous()
-        ms_vjp, _ = sync_time(lambda: sv.vjp(incs, grid, gI))
-        ms_fwd64, _ = sync_time(lambda: sv.solve_fp64_direct(s32, incs, grid))
+        ms = {"tree_forward_solve": ms_fwd}
+        for mode in ("tree64", "tree32"):
+            ms[f"{mode}_forward"], _ = sync_time(lambda: sv.grad_forward(incs, grid, mode=mode))
+            ms[f"{mode}_vjp"], _ = sync_time(lambda: sv.vjp(incs, grid, gI, mode=mode))
+        ms["fp64_forward"], _ = sync_time(lambda: sv.solve_fp64_direct(s32, incs, grid))
         sv.set_geometry(s32, 0)
-        ms_vjp64, _ = sync_time(lambda: sv.vjp(incs, grid, gI, fp64=True))
-        out["ms"] = {"tree_forward": ms_fwd, "tree_adjoint": ms_vjp, "fp64_forward": ms_fwd64,
-                     "fp64_adjoint": ms_vjp64}
-        out["depth"] = {"autotuned_best": best, "used": depth_used}
-        print(f"[bench] adjoint: tree fwd {ms_fwd:.1f} ms, vjp {ms_vjp:.1f} ms; fp64 fwd "
-              f"{ms_fwd64:.1f} ms, vjp {ms_vjp64:.1f} ms", file=sys.stderr, flush=True)
-
-        # dL/dx for the tree (depth used) and FP64; dL/d(delta_n) = n . dL/dx for
-        # x -> x + delta n with n and a fixed
-        nrm = torch.stack(s64[3:6])
-        gxs = {}
-        for name, fp64 in (("tree", False), ("fp64", True)):
-            x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(s32)]
-            L = log_intensity_loss(po_intensity(x, incs, grid, 0 if fp64 else depth_used, sv,
-                                                fp64=fp64), target)
-            L.backward()
-            gxs[name] = torch.stack([x[c].grad for c in range(3)]).double()
-        g64 = (gxs["fp64"] * nrm).sum(0)
-        gtr = (gxs["tree"] * nrm).sum(0)
-        rel_tree = float(torch.linalg.norm(gtr - g64) / torch.linalg.norm(g64))
-        # the same with twice the excess bandwidth of the level sampling
-        with wf.POSolver(device=0, beta=8.0) as sv8:
-            sv8.set_geometry(s32, depth_used)
-            ms_fwd8, _ = sync_time(lambda: sv8.solve(incs, grid, out="torch"))
-            ms_vjp8, _ = sync_time(lambda: sv8.vjp(incs, grid, gI))
-            x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(s32)]
-            L = log_intensity_loss(po_intensity(x, incs, grid, depth_used, sv8), target)
-            L.backward()
-            g8 = (torch.stack([x[c].grad for c in range(3)]).double() * nrm).sum(0)
-        rel_tree8 = float(torch.linalg.norm(g8 - g64) / torch.linalg.norm(g64))
-        out["ms"].update({"tree_forward_beta8": ms_fwd8, "tree_adjoint_beta8": ms_vjp8})
-        # the same with the leaf radiation sum on FP32 MFMA instead of split BF16 x3
-        with wf.POSolver(device=0, precision="fp32") as sv32:
-            sv32.set_geometry(s32, depth_used)
-            ms_fwd32, _ = sync_time(lambda: sv32.solve(incs, grid, out="torch"))
-            ms_vjp32, _ = sync_time(lambda: sv32.vjp(incs, grid, gI))
-            x = [t.clone().requires_grad_(i < 3) for i, t in enumerate(s32)]
-            L = log_intensity_loss(po_intensity(x, incs, grid, depth_used, sv32), target)
-            L.backward()
-            g32 = (torch.stack([x[c].grad for c in range(3)]).double() * nrm).sum(0)
-        rel_tree32 = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
-        out["ms"].update({"tree_forward_fp32": ms_fwd32, "tree_adjoint_fp32": ms_vjp32})
-
-        def loss64(surf):
-            Ix = sv.solve_fp64_direct(surf, incs, grid)
-            return float(((torch.log(Ix) - torch.log(target.double())) ** 2).sum().item())
-
-        rng = np.random.default_rng(2)
-        idx = torch.from_numpy(rng.choice(n, 64, replace=False)).cuda()
-        v = torch.zeros(n, dtype=torch.float64, device="cuda")
-        v[idx] = torch.from_numpy(rng.standard_normal(64)).cuda()
-        # central differences at h = 1e-4 um and h/2, each compared with the
-        # gradient contracted with the displacement actually applied (float32
-        # rounding of the coordinates); the ratio FD / prediction is
-        # extrapolated (Richardson) to remove the O(h^2) curvature term
-        ratios = {"fp64": [], "tree": []}
-        fds = []
-        base = torch.stack(s32[:3]).double()
-        for h in (1e-10, 0.5e-10):
-            # +d rounded to float32, then -d taken exactly: the pair is centred
-            # on the float32 coordinates the gradient was taken at
-            sp = (base + h * v * nrm).float().double()
-            d = sp - base
-            sm = base - d
-            assert torch.equal(sm.float().double(), sm)
-            fdh = (loss64([sp[c].float() for c in range(3)] + s32[3:])
-                   - loss64([sm[c].float() for c in range(3)] + s32[3:]))
-            dx = 2 * d
-            fds.append(fdh)
-            for name in ratios:
-                ratios[name].append(fdh / float((gxs[name] * dx).sum().item()))
-        fd = fds[0]
-        pred64 = fd / ratios["fp64"][0]
-        pred_tree = fd / ratios["tree"][0]
-        rich = {name: (4 * r[1] - r[0]) / 3 for name, r in ratios.items()}
-
-        # cuticle tilt.  dL/dtau from the adjoint through the differentiable
-        # surface recipe (autograd), and an FD check of the FP64 adjoint along
-        # the tilt tangent t = dS/dtau of all seven inputs (float64 central
-        # difference of the recipe, dtau = 1e-4 deg): steps d = fl32(S0 + h t)
-        # - S0 for h = 0.002 and 0.001 deg, exactly centred (elements where
-        # S0 - d is not a float32 value, and points whose lit state w.n <= 0
-        # differs between S0 + d and S0 - d, get d = 0), ratio FD / (2 g.d)
-        # Richardson-extrapolated.  dL/dtau is a small residue of large
-        # per-point terms, so steps large enough to swamp the float32
-        # rounding of h t leave the linear range of the log loss; these two
-        # are the compromise.
+        ms["fp64_vjp"], _ = sync_time(lambda: sv.vjp(incs, grid, gI, mode="direct64"))
+        out["ms"] = ms
+        out["depth"] = {"autotuned_best": best, "used": depth}
+        print(f"[bench] adjoint ms: " + ", ".join(f"{k} {v:.1f}" for k, v in ms.items()),
+              file=sys.stderr, flush=True)
+        grads = {m: gc.input_gradients(sv, s32, incs, grid, target, m, 0 if m == "direct64" else depth)
+                 for m in ("direct64", "tree64", "tree32")}
+        gd = {m: (torch.stack(g[:3]) * nrm).sum(0) for m, g in grads.items()}
+        rel = {m: float(torch.linalg.norm(gd[m] - gd["direct64"]) / torch.linalg.norm(gd["direct64"]))
+               for m in ("tree64", "tree32")}
+        rn = gc.fd_normal_ratio(sv, s32, nrm, incs, grid, target, grads)
+        rt = gc.fd_tangent_ratio(sv, gc.c5_surface, 3.0, incs, grid, target, grads)
+        # dL/dtau itself, through the differentiable recipe (tree64)
         tau = torch.tensor(3.0, dtype=torch.float64, device="cuda", requires_grad=True)
-        L = log_intensity_loss(po_intensity([t.float() for t in surface(tau)], incs, grid, 0, sv,
-                                            fp64=True), target)
+        L = log_intensity_loss(po_intensity([t.float() for t in gc.c5_surface(tau)], incs, grid,
+                                            depth, sv, mode="tree64"), target)
         L.backward()
-        g_tau = float(tau.grad.item())
-        x = [t.clone().requires_grad_(True) for t in s32]
-        L = log_intensity_loss(po_intensity(x, incs, grid, 0, sv, fp64=True), target)
-        L.backward()
-        g7 = [t.grad.double() for t in x]
-        dtau = 1e-4
-        up = surface(torch.tensor(3.0 + dtau, dtype=torch.float64, device="cuda"))
-        dn = surface(torch.tensor(3.0 - dtau, dtype=torch.float64, device="cuda"))
-        tdir = [(u.detach() - v_.detach()) / (2 * dtau) for u, v_ in zip(up, dn)]
-        S0 = [t.double() for t in s32]
-        wdir = torch.tensor([[math.cos(i.theta) * math.cos(i.phi), math.cos(i.theta) * math.sin(i.phi),
-                              math.sin(i.theta)] for i in incs], dtype=torch.float64, device="cuda")
-        tr, zeroed = [], 0
-        for h in (0.002, 0.001):
-            d = [(S0[c] + h * tdir[c]).float().double() - S0[c] for c in range(7)]
-            for c in range(7):
-                m = (S0[c] - d[c]).float().double() != (S0[c] - d[c])
-                zeroed += int(m.sum().item())
-                d[c][m] = 0
-            sp = [S0[c] + d[c] for c in range(7)]
-            sm = [S0[c] - d[c] for c in range(7)]
-            flip = ((torch.stack(sp[3:6], 1) @ wdir.T <= 0)
-                    != (torch.stack(sm[3:6], 1) @ wdir.T <= 0)).any(1)
-            zeroed += int(flip.sum().item())
-            for c in range(7):
-                d[c][flip] = 0
-            fdt = (loss64([(S0[c] + d[c]).float() for c in range(7)])
-                   - loss64([(S0[c] - d[c]).float() for c in range(7)]))
-            pred = sum(float((g7[c] * 2 * d[c]).sum().item()) for c in range(7))
-            tr.append(fdt / pred)
-        tilt_ratio = (4 * tr[1] - tr[0]) / 3
-        g_tau_t = sum(float((g7[c] * tdir[c]).sum().item()) for c in range(7))
-        fd_tau = tilt_ratio * g_tau_t
     out["gradient"] = {
-        "tree_vs_fp64_rel_l2": rel_tree, "tree_beta8_vs_fp64_rel_l2": rel_tree8,
-        "tree_fp32_mfma_vs_fp64_rel_l2": rel_tree32,
-        "fd_64pts_h1e-4um": fd, "fp64_adjoint_pred": pred64, "tree_adjoint_pred": pred_tree,
-        "fp64_vs_fd_rel_h": abs(pred64 - fd) / abs(fd),
-        "tree_vs_fd_rel_h": abs(pred_tree - fd) / abs(fd),
-        "fp64_vs_fd_rel": abs(rich["fp64"] - 1.0),
-        "tree_vs_fd_rel": abs(rich["tree"] - 1.0),
-        "tilt_adjoint": g_tau, "tilt_adjoint_via_tangent": g_tau_t,
-        "tilt_fd_along_tangent": fd_tau, "tilt_steps_zeroed": zeroed,
-        "tilt_rel": abs(g_tau - fd_tau) / abs(fd_tau),
+        "vs_fp64_gradient_rel_l2": rel,
+        "vs_fd_normal_64pts_rel": {m: abs(r - 1.0) for m, r in rn.items()},
+        "vs_fd_tilt_tangent_rel": {m: abs(r - 1.0) for m, r in rt.items()},
+        "dL_dtau_tree64": float(tau.grad.item()),
         "gate": 1e-4,
+        "pass_tree64": bool(abs(rn["tree64"] - 1) <= 1e-4 and abs(rt["tree64"] - 1) <= 1e-4),
     }
-    print(f"[bench] gradients: tree vs fp64 {rel_tree:.2e} (beta 8: {rel_tree8:.2e}, FP32 MFMA: {rel_tree32:.2e}); FD 64 pts: fp64 "
-          f"{out['gradient']['fp64_vs_fd_rel']:.2e}, tree {out['gradient']['tree_vs_fd_rel']:.2e}; "
-          f"tilt {out['gradient']['tilt_rel']:.2e} (dL/dtau {g_tau:.6g})", file=sys.stderr, flush=True)
+    g = out["gradient"]
+    print(f"[bench] gradients vs FP64 gradient: tree64 {rel['tree64']:.2e}, tree32 {rel['tree32']:.2e}; "
+          f"vs FD (normals): " + ", ".join(f"{m} {v:.2e}" for m, v in g["vs_fd_normal_64pts_rel"].items())
+          + "; vs FD (tilt): " + ", ".join(f"{m} {v:.2e}" for m, v in g["vs_fd_tilt_tangent_rel"].items()),
+          file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
     return 0
-
 
 if __name__ == "__main__":
     sys.exit(main())

@@ -229,11 +229,20 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
  * grad_x = dL/dx_n and grad_n = dL/dn_n ([3][n] each) and grad_a = dL/da_n
  * ([n]), all in the caller's original point order (any may be null), and the
  * forward intensity (nullable).  The forward pass is recomputed; the reverse
- * pass needs no stored per-level fields.  fp64 = 1 (or depth 0) runs the FP64
- * direct sum forward and backward (the reference gradient); otherwise the
- * tree and its exact adjoint.  Normal derivatives of the Fresnel currents use
- * FP64 central differences (h = 1e-6); they are one-sided quantities at the
- * lit/shadow boundary and at exact normal incidence. */
+ * pass needs no stored per-level fields.  fp64 selects the arithmetic:
+ *   1  the FP64 direct sum forward and backward (the reference gradient; also
+ *      used on a depth-0 geometry);
+ *   2  the tree with every pattern, table and accumulator in FP64 and its
+ *      exact adjoint (gradients of log-intensity losses, which weight the
+ *      dimmest directions by 1/I, need FP64 patterns to reach ~1e-4);
+ *   0  the tree with FP32 patterns: FP32 FFT level transitions and FP32 MFMA
+ *      leaf sums (whatever the handle's tune and precision) and their exact
+ *      adjoints.
+ * grad_I = NULL with intensity non-null computes only the forward intensity
+ * in the selected arithmetic, so that a loss and its gradient can be
+ * evaluated on the same forward.  Normal derivatives of the Fresnel currents
+ * are analytic (chain rule through the tangent-plane formulas); they are
+ * one-sided at the lit/shadow boundary and at exact normal incidence. */
 wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
                     const float* grad_I, float* intensity, float* grad_x, float* grad_n,
                     float* grad_a, int32_t fp64, int32_t memspace, void* stream);
@@ -276,8 +285,10 @@ typedef struct wf_tune {
                              rows and the LDS FFT above; 1 = the original LDS FFT kernels */
   int32_t agg_fft_rows;   /* see agg_kernel (default 600) */
   int32_t calibrated;     /* 1 after wf_autotune (or a wf_set_tune of a calibrated tune) */
-  int32_t agg_fused;      /* 1: internal MFMA level transitions whose grids fit run as one
-                             kernel (phi output kept in LDS); 0 (default): two passes */
+  int32_t agg_fused;      /* internal MFMA level transitions whose grids fit run as one
+                             kernel, the phi output kept in LDS: 1 (phi operands read from
+                             global memory) or 2 (phi matrix and child rows staged in LDS,
+                             the next child loaded during the current one); 0: two passes */
 } wf_tune;
 
 wf_status wf_get_tune(wf_handle* h, wf_tune* out);

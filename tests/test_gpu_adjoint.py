@@ -6,10 +6,12 @@
   applied after float32 rounding of the coordinates, so the step is exact.
   With every point displaced, a Richardson extrapolation over h and h/2
   removes the curvature term.
-* Tree gradient (FP32 leaf, exact adjoint of the tree) vs the FP64 gradient.
+* Tree gradients (FP64 patterns, the default, and FP32 patterns; each the
+  exact adjoint of its own forward) vs the FP64 gradient.
 * Cuticle-tilt gradient through the differentiable fiber generator vs a
   Richardson-extrapolated central difference of the FP64 forward.
-Gate: 1e-4 relative for gradient-vs-FD, 1e-3 for tree-vs-FP64 gradients.
+Gate: 1e-4 relative for gradient-vs-FD and for the FP64-pattern tree vs the
+FP64 gradient, 1e-3 for the FP32-pattern tree.
 """
 import numpy as np
 import pytest
@@ -126,22 +128,48 @@ def test_normal_displacement_gradient():
         assert torch.allclose(g, (gx * nrm).sum(0), rtol=1e-5, atol=1e-6 * g.abs().max().item())
 
 
-@pytest.mark.parametrize("depth", [2, 3])
-def test_tree_vjp_matches_fp64_gradient(depth):
+@pytest.mark.parametrize("mode,gate", [("tree64", 1e-4), ("tree32", 1e-3)])
+@pytest.mark.parametrize("depth,beta", [(2, 4.0), (3, 6.0)])
+def test_tree_vjp_matches_fp64_gradient(depth, beta, mode, gate):
+    """The tree gradient with FP64 patterns (default) and with FP32 patterns
+    against the FP64 direct-sum gradient, every input (x, y, z, n, area).
+    The log loss weights every direction by 1/I, so the dim directions set
+    the gradient's error: at depth 3 this 3 um sphere has leaves of
+    k R = 4.1, where beta = 4 leaves a truncation error of 3e-4 in both
+    modes alike; beta = 6 takes it below the gate."""
     s, dev, target = _setup(6000, 3)
-    print(f"case: tree vjp depth={depth}", flush=True)
-    with wf.POSolver(device=0, precision="fp32") as sv:
+    print(f"case: tree vjp depth={depth} beta={beta} mode={mode}", flush=True)
+    with wf.POSolver(device=0, beta=beta) as sv:
         grads = {}
-        for name, kw in (("fp64", dict(depth=0, fp64=True)), ("tree", dict(depth=depth, fp64=False))):
+        for name, kw in (("fp64", dict(depth=0, mode="direct64")), ("tree", dict(depth=depth, mode=mode))):
             x = [t.clone().requires_grad_(True) for t in dev]
-            L = log_intensity_loss(po_intensity(x, INCS, GRID, kw["depth"], sv, fp64=kw["fp64"]), target)
+            L = log_intensity_loss(po_intensity(x, INCS, GRID, kw["depth"], sv, mode=kw["mode"]), target)
             L.backward()
             grads[name] = [t.grad.double() for t in x]
     for c in range(7):
         a, b = grads["tree"][c], grads["fp64"][c]
         rel = (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
         print(f"  component {c}: rel {rel:.2e}", flush=True)
-        assert rel < 1e-3
+        assert rel < gate
+
+
+def test_forward_only_matches_vjp_intensity():
+    """wf_po_vjp with grad_I = NULL returns the same forward intensity as a
+    full reverse call in the same mode, and the tree64 forward agrees with the
+    FP64 direct sum far inside the tree gate."""
+    s, dev, target = _setup(4000, 4)
+    print("case: forward-only vjp", flush=True)
+    with wf.POSolver(device=0) as sv:
+        I64 = sv.solve_fp64_direct(dev, INCS, GRID)
+        sv.set_geometry(dev, 3)
+        for mode in ("tree64", "tree32"):
+            I_f = sv.grad_forward(INCS, GRID, mode=mode)
+            gI = torch.ones_like(I_f)
+            _, _, _, I_v = sv.vjp(INCS, GRID, gI, mode=mode, return_I=True)
+            assert torch.equal(I_f, I_v)
+            rel = float(torch.linalg.norm(I_f.double() - I64) / torch.linalg.norm(I64))
+            print(f"  {mode}: rel-L2 vs FP64 direct {rel:.2e}", flush=True)
+            assert rel < (1e-5 if mode == "tree64" else 1e-4)
 
 
 def test_tilt_gradient_vs_fd():

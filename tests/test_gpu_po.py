@@ -215,26 +215,30 @@ def test_tree_precision_modes(precision):
         assert rel(I[k], I[2]) < 1e-6  # identical incidences in other field slots
 
 
+@pytest.mark.parametrize("version", [1, 2])
 @pytest.mark.parametrize("depth", [3, 4])
-def test_fused_transition_matches_two_pass(depth):
-    """The one-kernel internal level transition (phi output in LDS) and the
-    two-pass MFMA transition use the same split-BF16 matrices: their trees
-    agree to FP32 rounding, both meet the oracle, and the fused form runs no
-    separate phi pass below the root."""
+def test_fused_transition_matches_two_pass(depth, version):
+    """The one-kernel internal level transitions (phi output in LDS; version
+    2 also stages the phi matrix and the child rows in LDS) and the two-pass
+    MFMA transition use the same split-BF16 matrices: their trees agree to
+    FP32 rounding, both meet the oracle, and the fused form runs no separate
+    phi pass below the root."""
     s = wf.fibonacci_sphere(60000, 4e-6)
     nt, np_ = 60, 120
     incs = INCS + [wf.Incidence(0.1 * k, 0.3 * k, 0.5e-6, "ME"[k % 2], 1.5 + 0.05j) for k in range(4)]
-    print(f"case: fused vs two-pass transitions, sphere n=60000 depth={depth}", flush=True)
+    print(f"case: fused v{version} vs two-pass transitions, sphere n=60000 depth={depth}",
+          flush=True)
     out = {}
-    for fused in (1, 0):
+    for fused in (version, 0):
         with wf.POSolver(device=0, solve_streams=1) as sv:
             sv.tune = {"agg_fused": fused, "agg_fft_rows": 100000}
             sv.set_geometry(s, depth)
             out[fused] = (sv.solve(incs, (nt, np_)), sv.stats())
     Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs], nt, np_)
     for k in range(len(incs)):
-        assert rel(out[1][0][k], out[0][0][k]) < 2e-6, (k, rel(out[1][0][k], out[0][0][k]))
-        assert rel(out[1][0][k], Iref[k]) < 1e-4
-    phi_fused = out[1][1]["kernels"].get("phi_pass", {"launches": 0})["launches"]
+        e = rel(out[version][0][k], out[0][0][k])
+        assert e < 2e-6, (k, e)
+        assert rel(out[version][0][k], Iref[k]) < 1e-4
+    phi_fused = out[version][1]["kernels"].get("phi_pass", {"launches": 0})["launches"]
     phi_2pass = out[0][1]["kernels"]["phi_pass"]["launches"]
     assert phi_fused < phi_2pass

@@ -18,14 +18,16 @@ from .po import POSolver
 
 class _POIntensity(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, y, z, nx, ny, nz, area, solver: POSolver, incs, grid, depth, fp64):
+    def forward(ctx, x, y, z, nx, ny, nz, area, solver: POSolver, incs, grid, depth, mode):
         surf = [t.detach().to(torch.float32).contiguous() for t in (x, y, z, nx, ny, nz, area)]
-        if fp64:
+        if mode == "direct64":
             I = solver.solve_fp64_direct(surf, incs, grid).to(torch.float32)
         else:
+            # the forward in the arithmetic of the reverse pass, so that the loss
+            # and its gradient see the same intensities
             solver.set_geometry(surf, depth)
-            I = solver.solve(incs, grid, out="torch")
-        ctx.solver, ctx.incs, ctx.grid, ctx.depth, ctx.fp64 = solver, list(incs), grid, depth, fp64
+            I = solver.grad_forward(incs, grid, mode=mode)
+        ctx.solver, ctx.incs, ctx.grid, ctx.depth, ctx.mode = solver, list(incs), grid, depth, mode
         ctx.save_for_backward(*surf)
         return I
 
@@ -33,18 +35,26 @@ class _POIntensity(torch.autograd.Function):
     def backward(ctx, gI):
         surf = list(ctx.saved_tensors)
         solver = ctx.solver
-        solver.set_geometry(surf, 0 if ctx.fp64 else ctx.depth)
-        gx, gn, ga = solver.vjp(ctx.incs, ctx.grid, gI.contiguous(), fp64=ctx.fp64)
+        solver.set_geometry(surf, 0 if ctx.mode == "direct64" else ctx.depth)
+        gx, gn, ga = solver.vjp(ctx.incs, ctx.grid, gI.contiguous(), mode=ctx.mode)
         return gx[0], gx[1], gx[2], gn[0], gn[1], gn[2], ga, None, None, None, None, None
 
 
-def po_intensity(surface, incidences, grid, depth: int, solver: POSolver, fp64: bool = False):
+def po_intensity(surface, incidences, grid, depth: int, solver: POSolver, fp64: bool = False,
+                 mode: str | None = None):
     """Differentiable far-field intensity [n_inc, Nt, Np] of ``surface``
-    (sequence of 7 CUDA tensors).  fp64=True uses the FP64 direct sum forward
-    and backward (the reference gradient)."""
+    (sequence of 7 CUDA tensors).  mode: "tree64" (default: the tree with FP64
+    patterns, forward and backward), "tree32" (FP32 patterns) or "direct64"
+    (the FP64 direct sum, the reference gradient; also fp64=True)."""
     x, y, z, nx, ny, nz, area = surface
+    if fp64:
+        mode = "direct64"
+    if mode is None:
+        mode = "tree64"
+    if mode not in POSolver.GRAD_MODES:
+        raise ValueError(f"mode must be one of {sorted(POSolver.GRAD_MODES)}")
     return _POIntensity.apply(x, y, z, nx, ny, nz, area, solver, list(incidences), tuple(grid),
-                              depth, fp64)
+                              depth, mode)
 
 
 def log_intensity_loss(I, target, eps: float = 0.0):

@@ -75,6 +75,7 @@ struct FusedK {
   int nkp, nkt;                 // K steps of the phi and theta GEMMs in use
   int mirror;
   int64_t nitems, used;
+  int64_t nparents_f;  // parents x fields
   const int32_t* cb;
   const int32_t* ce;
   const uint32_t* child_key;
@@ -283,6 +284,327 @@ __global__ __launch_bounds__(MAXE == 1 ? 768 : 512) void level_fused_mfma_kernel
   }
 }
 
+// ---------------------------------------------------------------------------
+// Second form of the same transition, built to hide memory latency: a block
+// keeps one pair tile for its life, with that tile's phi-matrix fragments in
+// LDS; the child patterns go through an LDS buffer filled one child ahead
+// (every thread loads its share of the next child into registers at the top
+// of an iteration and stores it after the theta GEMM), and the translation
+// factors of the current child are loaded at the top of the iteration, so the
+// only global accesses on the critical path of a child are the ones issued a
+// whole iteration earlier.  X staging needs ntc * (npc + 2) <= kXMax.
+constexpr int kXMax = 2688;  // complex samples of the LDS child buffer (21.5 KB)
+constexpr int kXRegs = 4;    // float4 per thread for the prefetch
+
+template <int NKP, int NKT, int KMAX, int MAXE, bool STAGE, int LB>
+__global__ __launch_bounds__(LB) void level_fused2_kernel(const FusedK P) {
+  constexpr int NCOL = 16 * KMAX;
+  __shared__ bf16x8 Gs[NKT * 4 * NCOL * 4];
+  __shared__ bf16x8 Ms[NKP * 2 * 2 * NCOL * 4];
+  __shared__ float4 Xs[STAGE ? kXMax / 2 : 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int NW = blockDim.x >> 6;
+  const int c16 = lane & 15, g = lane >> 4;
+  const int64_t nb = gridDim.x, per = (nb + 7) / 8;
+  const int64_t w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (w >= P.used) return;  // whole block
+  const int F = P.F, ntc = P.ntc, npc = P.npc, ntp = P.ntp, npp = P.npp;
+  const int half = P.half, h8 = P.h8, n2p = 2 * ntp;
+  const int npt = P.npt;
+  const int pt = (int)(w % npt);
+  const int64_t ub = P.used / npt;  // blocks per pair tile
+  const int kbase = pt * NCOL;
+  const int nk_here = min(KMAX, (half - kbase + 15) >> 4);
+  const int npcs = npc + 2;  // padded LDS row (16 B aligned, spreads rows over banks)
+  {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
+    for (int i = threadIdx.x; i < NKT * 4 * NCOL * 4; i += blockDim.x) Gs[i] = z;
+    // phi-matrix B fragments of this pair tile: unit (ks, hi/lo, column, k group)
+    for (int i = threadIdx.x; i < NKP * 2 * 2 * NCOL * 4; i += blockDim.x) {
+      const int gg = i & 3, col = (i >> 2) % (2 * NCOL), hl = (i / (8 * NCOL)) & 1,
+                ks = i / (16 * NCOL);
+      const int cc = col % NCOL;
+      const int kk = kbase + cc;
+      bf16x8 v = z;
+      if (kk < half && ks < P.nkp) {
+        const int q = (col < NCOL) ? kk : kk + half;
+        const bf16x8* M = hl ? P.Mphi_lo : P.Mphi_hi;
+        v = M[(((int64_t)q * P.kpad_phi) >> 3) + ks * 4 + gg];
+      }
+      Ms[i] = v;
+    }
+  }
+  bf16x8 ah[MAXE][NKT], al[MAXE][NKT];
+#pragma unroll
+  for (int a = 0; a < MAXE; ++a) {
+    const int et = wave + a * NW;
+    const int row = 16 * et + c16;
+    const int rr = row < P.mrows_th ? row : 0;
+    const bf16x8* ph = P.Mth_hi + (((int64_t)rr * P.kpad_th) >> 3);
+    const bf16x8* pl = P.Mth_lo + (((int64_t)rr * P.kpad_th) >> 3);
+#pragma unroll
+    for (int ks = 0; ks < NKT; ++ks) {
+      const int kq = ks < P.nkt ? ks * 4 + g : 0;
+      ah[a][ks] = ph[kq];
+      al[a][ks] = pl[kq];
+    }
+  }
+  // child rows as float4 (two complex samples): nx4 per child
+  const int nx4 = ntc * (npc >> 1);
+  const int nthr = blockDim.x;
+  float4 xr[kXRegs];
+  // child c, field f: its plane starts at (c F + f) ntc npc
+  auto prefetch_cf = [&](int c, int f) {
+    const float4* src =
+        reinterpret_cast<const float4*>(P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc));
+#pragma unroll
+    for (int r = 0; r < kXRegs; ++r) {
+      const int i = threadIdx.x + r * nthr;
+      xr[r] = i < nx4 ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int r = 0; r < kXRegs; ++r) {
+      const int i = threadIdx.x + r * nthr;
+      if (i < nx4) {
+        const int row = i / (npc >> 1), c2 = i - row * (npc >> 1);
+        Xs[(row * npcs >> 1) + c2] = xr[r];
+      }
+    }
+  };
+  const int64_t nitems = (int64_t)P.nparents_f;
+  const int nunits_max = P.nth * 2 * KMAX;
+  int64_t pf = w / npt;
+  if (STAGE && pf < nitems) {
+    const int p0 = (int)(pf / F), f0 = (int)(pf - (int64_t)p0 * F);
+    if (P.cb[p0] < P.ce[p0]) {
+      prefetch_cf(P.cb[p0], f0);
+      stash();
+    }
+  }
+  __syncthreads();
+  for (; pf < nitems; pf += ub) {
+    const int p = (int)(pf / F), f = (int)(pf - (int64_t)p * F);
+    float2 acc[MAXE][KMAX][4];
+#pragma unroll
+    for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+      for (int b = 0; b < KMAX; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[a][b][j] = make_float2(0.f, 0.f);
+    const int c_begin = P.cb[p], c_end = P.ce[p];
+    if (STAGE && c_begin == c_end && pf + ub < nitems) {  // no children: stage the next item's
+      const int64_t pn = pf + ub;
+      const int p2 = (int)(pn / F), f2 = (int)(pn - (int64_t)p2 * F);
+      if (P.cb[p2] < P.ce[p2]) {
+        prefetch_cf(P.cb[p2], f2);
+        stash();
+      }
+      __syncthreads();
+    }
+    for (int c = c_begin; c < c_end; ++c) {
+      // next (child, field) of this block: the next child of the item, else
+      // the first child of the block's next item
+      if (STAGE) {
+        if (c + 1 < c_end) {
+          prefetch_cf(c + 1, f);
+        } else if (pf + ub < nitems) {
+          const int64_t pn = pf + ub;
+          const int p2 = (int)(pn / F), f2 = (int)(pn - (int64_t)p2 * F);
+          if (P.cb[p2] < P.ce[p2]) prefetch_cf(P.cb[p2], f2);
+        }
+      }
+      const int oct = (int)(P.child_key[c] & 7u);
+      const float2* To = P.Text + (int64_t)oct * (n2p + 1) * half;
+      const float2* Xc = P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc);
+      // ---- phi GEMM into LDS
+      for (int u = wave; u < nunits_max; u += NW) {
+        const int t = u / (2 * KMAX), n = u - t * (2 * KMAX);
+        const int nt16 = n % KMAX;
+        if (nt16 >= nk_here) continue;
+        const bool second = n >= KMAX;
+        const int kk = kbase + 16 * nt16 + c16;
+        const bool kok = kk < half;
+        const int th = 16 * t + c16;
+        const bool rok = th < ntc;
+        floatx4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < NKP; ++ks) {
+          if (ks < P.nkp) {
+            const int j0 = 32 * ks + 8 * g;
+            float vr[8], vi[8];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+              const int jj = j0 + 2 * h;
+              float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+              if (rok && jj < npc) {
+                if (STAGE)
+                  v = Xs[(th * npcs + jj) >> 1];
+                else
+                  v = *reinterpret_cast<const float4*>(Xc + (int64_t)th * npc + jj);
+              }
+              vr[2 * h] = v.x;
+              vi[2 * h] = v.y;
+              vr[2 * h + 1] = v.z;
+              vi[2 * h + 1] = v.w;
+            }
+            bf16x8 arh, arl, aih, ail;
+            split8(vr, arh, arl);
+            split8(vi, aih, ail);
+            const int colM = 16 * n + c16;
+            const bf16x8 bh = Ms[((ks * 2 + 0) * 2 * NCOL + colM) * 4 + g];
+            const bf16x8 bl = Ms[((ks * 2 + 1) * 2 * NCOL + colM) * 4 + g];
+            dr = mfma3(arh, arl, bh, bl, dr);
+            di = mfma3(aih, ail, bh, bl, di);
+          }
+        }
+        const int th0 = 16 * t + 4 * g;
+        if (kok && th0 < h8) {
+          const int jg = (second ? h8 : 0) + th0;
+          const int ks = jg >> 5, g8 = (jg >> 3) & 3, hoff = (jg >> 2) & 1;
+          const int col = 16 * nt16 + c16;
+          bf16x4 rh, rl, ih, il;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const __bf16 hr = (__bf16)dr[j], hi_ = (__bf16)di[j];
+            rh[j] = hr;
+            rl[j] = (__bf16)(dr[j] - (float)hr);
+            ih[j] = hi_;
+            il[j] = (__bf16)(di[j] - (float)hi_);
+          }
+          bf16x4* base = reinterpret_cast<bf16x4*>(Gs);
+          const int unit = ((ks * 4) * NCOL + col) * 4 + g8;
+          base[2 * unit + hoff] = rh;
+          base[2 * (unit + 1 * NCOL * 4) + hoff] = rl;
+          base[2 * (unit + 2 * NCOL * 4) + hoff] = ih;
+          base[2 * (unit + 3 * NCOL * 4) + hoff] = il;
+        }
+      }
+      __syncthreads();
+      // ---- theta GEMM, translation, child sum
+#pragma unroll
+      for (int b = 0; b < KMAX; ++b) {
+        if (b < nk_here) {
+          const int kk = kbase + 16 * b + c16;
+          float2 tf[MAXE][4];
+#pragma unroll
+          for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int e = 16 * (wave + a * NW) + 4 * g + j;
+              tf[a][j] = (kk < half && e < n2p) ? To[(int64_t)e * half + kk] : make_float2(0.f, 0.f);
+            }
+          floatx4 ur[MAXE], ui[MAXE];
+#pragma unroll
+          for (int a = 0; a < MAXE; ++a) {
+            ur[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+            ui[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+          }
+#pragma unroll
+          for (int ks = 0; ks < NKT; ++ks) {
+            if (ks < P.nkt) {
+              const int col = 16 * b + c16;
+              const bf16x8 brh = Gs[((ks * 4 + 0) * NCOL + col) * 4 + g];
+              const bf16x8 brl = Gs[((ks * 4 + 1) * NCOL + col) * 4 + g];
+              const bf16x8 bih = Gs[((ks * 4 + 2) * NCOL + col) * 4 + g];
+              const bf16x8 bil = Gs[((ks * 4 + 3) * NCOL + col) * 4 + g];
+#pragma unroll
+              for (int a = 0; a < MAXE; ++a) {
+                ur[a] = mfma3(ah[a][ks], al[a][ks], brh, brl, ur[a]);
+                ui[a] = mfma3(ah[a][ks], al[a][ks], bih, bil, ui[a]);
+              }
+            }
+          }
+#pragma unroll
+          for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float2 tu = cmul(tf[a][j], make_float2(ur[a][j], ui[a][j]));
+              acc[a][b][j].x += tu.x;
+              acc[a][b][j].y += tu.y;
+            }
+        }
+      }
+      if (STAGE) stash();  // every wave is past this child's phi GEMM (barrier above)
+      __syncthreads();
+    }
+    float2* O = P.out + ((int64_t)p * F + f) * ((int64_t)ntp * npp);
+#pragma unroll
+    for (int a = 0; a < MAXE; ++a)
+#pragma unroll
+      for (int b = 0; b < KMAX; ++b) {
+        const int kk = kbase + 16 * b + c16;
+        if (b < nk_here && kk < half) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int e = 16 * (wave + a * NW) + 4 * g + j;
+            if (e < n2p) {
+              int row = e, col = kk;
+              if (e >= ntp) {
+                row = P.mirror - e;
+                col = kk + half;
+              }
+              if (row >= 0 && row < ntp) O[(int64_t)row * npp + col] = acc[a][b][j];
+            }
+          }
+        }
+      }
+  }
+}
+
+template <int NKP, int NKT, int KMAX, int MAXE, bool STAGE, int LB>
+hipError_t launch_fused2_t(const FusedK& P, int nw, hipStream_t st) {
+  constexpr int lds = (NKT * 4 * 16 * KMAX * 4 + NKP * 2 * 2 * 16 * KMAX * 4) * 16 +
+                      (STAGE ? kXMax * 8 : 16);
+  int bpc = std::max(1, (160 * 1024) / lds);
+  bpc = std::min(bpc, std::max(1, (LB / 64) * 2 / nw));
+  FusedK Q = P;
+  const int64_t pf_items = (int64_t)P.nparents_f;
+  int64_t ub = std::max<int64_t>(1, std::min<int64_t>(256LL * bpc / P.npt, pf_items));
+  Q.used = ub * P.npt;
+  const int64_t nb = ((Q.used + 7) / 8) * 8;
+  hipLaunchKernelGGL((level_fused2_kernel<NKP, NKT, KMAX, MAXE, STAGE, LB>), dim3((unsigned)nb),
+                     dim3(64 * nw), 0, st, Q);
+  return hipGetLastError();
+}
+
+template <int NKP, int NKT, int LB>
+hipError_t launch_fused2_kt(const FusedK& P, int kmax, int maxe, bool stage, int nw, hipStream_t st) {
+  // the LDS child buffer fits beside the operands of the smaller shapes only
+  constexpr bool can_stage = NKP <= 3 && NKT <= 3;
+  if (maxe == 2) return launch_fused2_t<NKP, NKT, 4, 2, false, 512>(P, nw, st);
+  if (kmax == 3) {
+    if constexpr (can_stage) {
+      if (stage) return launch_fused2_t<NKP, NKT, 3, 1, true, LB>(P, nw, st);
+    }
+    return launch_fused2_t<NKP, NKT, 3, 1, false, LB>(P, nw, st);
+  }
+  if constexpr (can_stage) {
+    if (stage) return launch_fused2_t<NKP, NKT, 4, 1, true, LB>(P, nw, st);
+  }
+  return launch_fused2_t<NKP, NKT, 4, 1, false, LB>(P, nw, st);
+}
+
+template <int NKP, int LB>
+hipError_t launch_fused2_kl(const FusedK& P, int nkt, int kmax, int maxe, bool stage, int nw,
+                            hipStream_t st) {
+  if (nkt <= 2) return launch_fused2_kt<NKP, 2, LB>(P, kmax, maxe, stage, nw, st);
+  if (nkt == 3) return launch_fused2_kt<NKP, 3, LB>(P, kmax, maxe, stage, nw, st);
+  if (nkt == 4) return launch_fused2_kt<NKP, 4, LB>(P, kmax, maxe, stage, nw, st);
+  return launch_fused2_kt<NKP, 5, LB>(P, kmax, maxe, stage, nw, st);
+}
+
+// blocks of up to 8 waves get 256 VGPRs (launch bound 512), up to 12 waves 168
+template <int NKP>
+hipError_t launch_fused2_k(const FusedK& P, int nkt, int kmax, int maxe, bool stage, int nw,
+                           hipStream_t st) {
+  if (nw <= 8) return launch_fused2_kl<NKP, 512>(P, nkt, kmax, maxe, stage, nw, st);
+  return launch_fused2_kl<NKP, 768>(P, nkt, kmax, maxe, stage, nw, st);
+}
+
 template <int NKP, int NKT, int MAXE>
 hipError_t launch_fused_t(const FusedK& P, int nw, hipStream_t st) {
   // resident blocks: LDS NKT * 16 KB per block, waves per SIMD by register use
@@ -363,9 +685,22 @@ hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st) {
   P.Mth_lo = reinterpret_cast<const bf16x8*>(a.Mth_lo);
   P.kpad_th = a.kpad_th;
   P.mrows_th = a.mrows_th;
+  P.nparents_f = (int64_t)a.nparents * a.F;
   // one e-tile per wave up to 12 waves, else two
   const int maxe = P.net <= 12 ? 1 : 2;
   const int nw = (P.net + maxe - 1) / maxe;
+  if (a.version == 2) {
+    // pair tiles of 48 columns when they cover the grid, else 64 (two
+    // e-tiles per wave run the 64-column form only)
+    const int kmax = (maxe == 1 && P.half <= 48) ? 3 : 4;
+    P.npt = (P.half + 16 * kmax - 1) / (16 * kmax);
+    P.nitems = P.nparents_f * P.npt;
+    const bool stage = maxe == 1 && a.ntc * (a.npc + 2) <= kXMax &&
+                       (int64_t)a.ntc * (a.npc / 2) <= (int64_t)kXRegs * 64 * nw;
+    if (nkp <= 2) return launch_fused2_k<2>(P, nkt, kmax, maxe, stage, nw, st);
+    if (nkp == 3) return launch_fused2_k<3>(P, nkt, kmax, maxe, stage, nw, st);
+    return launch_fused2_k<4>(P, nkt, kmax, maxe, stage, nw, st);
+  }
   if (nkp <= 2) return launch_fused_k<2>(P, nkt, maxe, nw, st);
   if (nkp == 3) return launch_fused_k<3>(P, nkt, maxe, nw, st);
   return launch_fused_k<4>(P, nkt, maxe, nw, st);

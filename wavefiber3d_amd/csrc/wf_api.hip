@@ -19,6 +19,7 @@
 #include "wf_fft.h"
 #include "wf_interp.h"
 #include "wf_octree.h"
+#include "wf_tree64.h"
 
 namespace {
 
@@ -105,6 +106,7 @@ struct wf_handle {
   std::map<int, FftEntry> ffts;
   std::map<std::pair<std::pair<int, int>, uint64_t>, MatEntry> mats;
   std::map<int, Fft2Entry> ffts2;
+  std::map<int, wf::Fft64Plan> ffts64;  // FP64 twiddles (owned: plan.tw)
   std::map<std::pair<int, uint64_t>, float2*> facs;
   // geometry
   bool has_geom = false;
@@ -261,6 +263,34 @@ wf_status get_fft(wf_handle* h, int n, FftPlanDev* out) {
 // shifted by `shift` (radians of the period-2pi parameter), cached per handle.
 // Interpolation matrix nc -> np samples (gc: the theta matrix of the MFMA
 // pass, nc = 2 ntc, columns in the phi pass's great-circle row order).
+// FP64 Stockham plan of length n (radices as get_fft, twiddles in FP64).
+wf_status get_fft64(wf_handle* h, int n, wf::Fft64Plan* out) {
+  auto it = h->ffts64.find(n);
+  if (it != h->ffts64.end()) {
+    *out = it->second;
+    return WF_OK;
+  }
+  FftPlanDev f;
+  if (!factor_len(n, f))
+    return set_err(WF_ERR_UNSUPPORTED, "FFT length %d has a prime factor above 13", n);
+  std::vector<double2> tw(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * WF_PI * (double)m / (double)n;
+    tw[m] = make_double2(std::cos(a), std::sin(a));
+  }
+  double2* d = nullptr;
+  WF_HIP(hipMalloc(&d, sizeof(double2) * n));
+  WF_HIP(hipMemcpy(d, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  wf::Fft64Plan p;
+  p.n = n;
+  p.nstages = f.nstages;
+  p.radix_packed = f.radix_packed;
+  p.tw = d;
+  h->ffts64[n] = p;
+  *out = p;
+  return WF_OK;
+}
+
 wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out, bool gc = false) {
   uint64_t sb;
   memcpy(&sb, &shift, sizeof sb);
@@ -673,6 +703,8 @@ void wf_destroy(wf_handle* h) {
   }
   for (auto& kv : h->ffts2)
     if (kv.second.tw) hipFree(kv.second.tw);
+  for (auto& kv : h->ffts64)
+    if (kv.second.tw) hipFree(const_cast<double2*>(kv.second.tw));
   for (auto& kv : h->facs)
     if (kv.second) hipFree(kv.second);
   if (h->inc_host) hipHostFree(h->inc_host);
@@ -1415,6 +1447,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       fa.kpad_th = mt.kpad;
       fa.mrows_th = mt.rows_pad;
       fa.mirror = mirror;
+      fa.version = h->tune.agg_fused;
       const double cbytes = 8.0 * planes * gcg.nt * gcg.np;
       const double pbytes = 8.0 * F * (double)npar * gp.nt * gp.np;
       WF_TRY(timed(h, st, WF_K_THETA, cbytes + pbytes,
@@ -1550,6 +1583,178 @@ wf_status backward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const
   for (int q = 0; q < S; ++q)
     WF_TRY(timed(h, st, WF_K_CURRENTS, 0.0, [&] {
       return wf::launch_grad_assemble(h->pts, n, Q, false, F, 6 * q, dinc + order[c0 + q], acc, st);
+    }));
+  return WF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 tree (gradients): the FP32 FFT path's steps with every pattern, table
+// and twiddle in FP64 (wf_tree64.hip).  Levels 0..depth all kept; the root
+// transition evaluates level 1 on the output grid.
+struct Tree64 {
+  std::vector<int64_t> toff;  // plain translation tables [level][8][nt][np]
+  double2* T = nullptr;
+  double2 *patA = nullptr, *patB = nullptr, *G = nullptr;
+};
+
+wf_status tree64_prepare(wf_handle* h, hipStream_t st, const GroupCtx& gc, int F, Tree64& tr) {
+  const int depth = h->depth;
+  tr.toff.assign(depth + 1, 0);
+  int64_t tt = 0;
+  for (int l = 0; l < depth; ++l) {
+    tr.toff[l] = tt;
+    tt += 8LL * gc.g[l].nt * gc.g[l].np;
+  }
+  WF_TRY(dalloc_t(h, "t64_trans", (size_t)std::max<int64_t>(tt, 1), &tr.T));
+  for (int l = 0; l < depth; ++l) {
+    const Grid gl = gc.g[l];
+    const double hc[3] = {0.5 * (h->box[3] - h->box[0]) / std::ldexp(1.0, l + 1),
+                          0.5 * (h->box[4] - h->box[1]) / std::ldexp(1.0, l + 1),
+                          0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
+    const double k0 = gc.k0;
+    WF_TRY(timed(h, st, WF_K_TRANS, 16.0 * 8 * gl.nt * gl.np, [&] {
+      return wf::launch_trans64(tr.T + tr.toff[l], gl.nt, gl.np, theta_origin(l, gl.nt), hc, k0,
+                                st);
+    }));
+  }
+  size_t pat_max = 16, g_max = 16;
+  for (int l = 1; l <= depth; ++l)
+    pat_max = std::max(pat_max, (size_t)h->levels[l].count * F * gc.g[l].nt * gc.g[l].np);
+  for (int l = 0; l < depth; ++l)
+    g_max = std::max(g_max, (size_t)h->levels[l + 1].count * F * gc.g[l + 1].nt * gc.g[l].np);
+  WF_TRY(dalloc_t(h, "t64_patA", pat_max, &tr.patA));
+  WF_TRY(dalloc_t(h, "t64_patB", pat_max, &tr.patB));
+  WF_TRY(dalloc_t(h, "t64_G", g_max, &tr.G));
+  return WF_OK;
+}
+
+wf::Fft64Plan fft64_or_die(wf_handle* h, int n, wf_status* err) {
+  wf::Fft64Plan p{};
+  if (*err == WF_OK) *err = get_fft64(h, n, &p);
+  return p;
+}
+
+// FP64 forward of one chunk: currents (FP64) -> leaf sums -> levels -> F0.
+wf_status forward_chunk64(wf_handle* h, hipStream_t st, const GroupCtx& gc, const WfIncDev* dinc,
+                          const int* order, int c0, int S, int NT, int NP, Tree64& tr,
+                          double2* F0) {
+  const int depth = h->depth;
+  const int64_t n = h->n;
+  const int F = 6 * S;
+  const float* px = h->pts;
+  double2* C = nullptr;
+  WF_TRY(dalloc_t(h, "t64_C", (size_t)n * F, &C));
+  for (int q = 0; q < S; ++q)
+    WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 96.0) * n, [&] {
+      return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                 px + 6 * n, n, dinc + order[c0 + q], 1, 1, C + 6 * q, F, 0, st);
+    }));
+  double2* cur = (depth % 2) ? tr.patA : tr.patB;
+  const LevelDev& LL = h->levels[depth];
+  wf::Leaf64Args la;
+  la.px = px;
+  la.py = px + n;
+  la.pz = px + 2 * n;
+  la.C = C;
+  la.F = F;
+  la.leaf_begin = LL.begin;
+  la.leaf_end = LL.end;
+  la.leaf_key = LL.key;
+  la.nleaves = LL.count;
+  la.depth = depth;
+  for (int q = 0; q < 3; ++q) {
+    la.box_lo[q] = h->box[q];
+    la.leaf_size[q] = (h->box[q + 3] - h->box[q]) / std::ldexp(1.0, depth);
+  }
+  la.k0 = gc.k0;
+  la.nt = gc.g[depth].nt;
+  la.np = gc.g[depth].np;
+  la.out = cur;
+  const double dleaf = (double)la.nt * la.np;
+  WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F, [&] { return wf::launch_leaf64(la, st); }));
+  for (int l = depth - 1; l >= 0; --l) {
+    if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
+    const Grid gcg = gc.g[l + 1], gp = gc.g[l];
+    wf_status err = WF_OK;
+    const wf::Fft64Plan pc = fft64_or_die(h, gcg.np, &err), pp = fft64_or_die(h, gp.np, &err),
+                        tc = fft64_or_die(h, 2 * gcg.nt, &err),
+                        tp = fft64_or_die(h, 2 * gp.nt, &err);
+    WF_TRY(err);
+    const int nchild = h->levels[l + 1].count, npar = h->levels[l].count;
+    const double off_c = theta_origin(l + 1, gcg.nt), off_p = theta_origin(l, gp.nt);
+    const int mirror = mirror_of(gp.nt, off_p);
+    WF_TRY(timed(h, st, WF_K_PHI, 16.0 * nchild * F * gcg.nt * ((double)gcg.np + gp.np), [&] {
+      return wf::launch_phi64(cur, tr.G, (int64_t)nchild * F, gcg.nt, gcg.np, gp.np, pc, pp, st);
+    }));
+    double2* nxt = (l == 0) ? F0 : ((cur == tr.patA) ? tr.patB : tr.patA);
+    const int np_out = (l == 0) ? 1 : npar;
+    WF_TRY(timed(h, st, WF_K_THETA, 16.0 * F * ((double)nchild * gcg.nt * gp.np + (double)np_out * gp.nt * gp.np), [&] {
+      return wf::launch_theta64(tr.G, F, gcg.nt, gp.np, gp.nt, np_out, h->levels[l].cb,
+                                h->levels[l].ce, h->levels[l + 1].key, tr.T + tr.toff[l],
+                                off_p - off_c, mirror, tc, tp, nxt, st);
+    }));
+    cur = nxt;
+  }
+  return WF_OK;
+}
+
+// FP64 reverse of one chunk: gF [F][D] -> leaf cogradients -> Q -> acc.
+wf_status backward_chunk64(wf_handle* h, hipStream_t st, const GroupCtx& gc, const WfIncDev* dinc,
+                           const int* order, int c0, int S, Tree64& tr, const double2* gF,
+                           double* acc) {
+  const int depth = h->depth;
+  const int64_t n = h->n;
+  const int F = 6 * S;
+  const double2* gcur = gF;  // cogradient of the current level's patterns
+  for (int l = 0; l < depth; ++l) {
+    const Grid gp = gc.g[l], gch = gc.g[l + 1];
+    wf_status err = WF_OK;
+    const wf::Fft64Plan pc = fft64_or_die(h, gch.np, &err), pp = fft64_or_die(h, gp.np, &err),
+                        tc = fft64_or_die(h, 2 * gch.nt, &err),
+                        tp = fft64_or_die(h, 2 * gp.nt, &err);
+    WF_TRY(err);
+    const int nchild = h->levels[l + 1].count;
+    const double off_c = theta_origin(l + 1, gch.nt), off_p = theta_origin(l, gp.nt);
+    const int mirror = mirror_of(gp.nt, off_p);
+    WF_TRY(timed(h, st, WF_K_THETA, 0.0, [&] {
+      return wf::launch_theta_adj64(gcur, F, gch.nt, gp.np, gp.nt, nchild,
+                                    l == 0 ? nullptr : h->levels[l + 1].parent,
+                                    h->levels[l + 1].key, tr.T + tr.toff[l], off_p - off_c,
+                                    mirror, tp, tc, tr.G, st);
+    }));
+    double2* nxt = (gcur == tr.patA) ? tr.patB : tr.patA;
+    WF_TRY(timed(h, st, WF_K_PHI, 0.0, [&] {
+      return wf::launch_phi_adj64(tr.G, nxt, (int64_t)nchild * F, gch.nt, gch.np, gp.np, pp, pc,
+                                  st);
+    }));
+    gcur = nxt;
+  }
+  double2* Q = nullptr;
+  WF_TRY(dalloc_t(h, "t64_Q", (size_t)n * F * 4, &Q));
+  const LevelDev& LL = h->levels[depth];
+  wf::LeafAdj64Args aa;
+  aa.px = h->pts;
+  aa.py = h->pts + n;
+  aa.pz = h->pts + 2 * n;
+  aa.leaf_begin = LL.begin;
+  aa.leaf_end = LL.end;
+  aa.leaf_key = LL.key;
+  aa.nleaves = LL.count;
+  aa.depth = depth;
+  for (int q = 0; q < 3; ++q) {
+    aa.box_lo[q] = h->box[q];
+    aa.leaf_size[q] = (h->box[q + 3] - h->box[q]) / std::ldexp(1.0, depth);
+  }
+  aa.k0 = gc.k0;
+  aa.nt = gc.g[depth].nt;
+  aa.np = gc.g[depth].np;
+  aa.F = F;
+  aa.g = gcur;
+  aa.Q = Q;
+  WF_TRY(timed(h, st, WF_K_LEAF, 0.0, [&] { return wf::launch_leaf_adj64(aa, st); }));
+  for (int q = 0; q < S; ++q)
+    WF_TRY(timed(h, st, WF_K_CURRENTS, 0.0, [&] {
+      return wf::launch_grad_assemble(h->pts, n, Q, true, F, 6 * q, dinc + order[c0 + q], acc, st);
     }));
   return WF_OK;
 }
@@ -1838,7 +2043,9 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
                     float* grad_a, int32_t fp64, int32_t memspace, void* stream) {
   if (!h) return set_err(WF_ERR_INVALID_ARG, "handle is null");
   if (!h->has_geom) return set_err(WF_ERR_NO_GEOMETRY, "wf_po_vjp: no geometry");
-  if (!grad_I) return set_err(WF_ERR_INVALID_ARG, "grad_I is null");
+  // grad_I null: the forward intensity only, in the arithmetic of the reverse pass
+  const bool fwd_only = grad_I == nullptr;
+  if (fwd_only && !intensity) return set_err(WF_ERR_INVALID_ARG, "grad_I and intensity are null");
   if (grid.n_theta < 1 || grid.n_phi < 2) return set_err(WF_ERR_INVALID_ARG, "grid too small");
   if (memspace != WF_MEM_HOST && memspace != WF_MEM_DEVICE)
     return set_err(WF_ERR_INVALID_ARG, "memspace unknown");
@@ -1846,7 +2053,9 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
   WF_TRY(set_device(h));
   hipStream_t st = as_stream(stream);
   const int64_t n = h->n;
-  const bool direct = fp64 || h->depth_max == 0;
+  if (fp64 < 0 || fp64 > 2) return set_err(WF_ERR_INVALID_ARG, "wf_po_vjp: fp64 must be 0, 1 or 2");
+  const bool direct = fp64 == 1 || h->depth_max == 0;
+  const bool tree64 = !direct && fp64 == 2;
   const int NT = grid.n_theta, NP = grid.n_phi;
   const int64_t D = (int64_t)NT * NP;
   if (!direct) WF_TRY(check_tree_grid(grid));
@@ -1856,10 +2065,12 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
   float* I_dev = intensity;
   float *gx = grad_x, *gn = grad_n, *ga = grad_a;
   if (memspace == WF_MEM_HOST) {
-    float* t = nullptr;
-    WF_TRY(dalloc_t(h, "vjp_gI", (size_t)n_inc * D, &t));
-    WF_HIP(hipMemcpyAsync(t, grad_I, sizeof(float) * n_inc * D, hipMemcpyHostToDevice, st));
-    gI = t;
+    if (!fwd_only) {
+      float* t = nullptr;
+      WF_TRY(dalloc_t(h, "vjp_gI", (size_t)n_inc * D, &t));
+      WF_HIP(hipMemcpyAsync(t, grad_I, sizeof(float) * n_inc * D, hipMemcpyHostToDevice, st));
+      gI = t;
+    }
     if (intensity) WF_TRY(dalloc_t(h, "vjp_I", (size_t)n_inc * D, &I_dev));
     if (grad_x) WF_TRY(dalloc_t(h, "vjp_gx", (size_t)3 * n, &gx));
     if (grad_n) WF_TRY(dalloc_t(h, "vjp_gn", (size_t)3 * n, &gn));
@@ -1916,7 +2127,11 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
       WF_TRY(dalloc_t(h, "F0", (size_t)D * F, &F0));
       WF_TRY(dalloc_t(h, "vjp_gF", (size_t)D * F, &gF));
       TreeBufs tb;
-      if (direct) {
+      Tree64 t64;
+      if (tree64) {
+        WF_TRY(tree64_prepare(h, st, gc, F, t64));
+        WF_TRY(forward_chunk64(h, st, gc, dinc, order.data(), c0, S, NT, NP, t64, F0));
+      } else if (direct) {
         double2 *C = nullptr, *part = nullptr;
         WF_TRY(dalloc_t(h, "direct_C", (size_t)n * F, &C));
         const int64_t part_elems = (int64_t)wf::direct_split(D) * F * D;
@@ -1931,6 +2146,16 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
       } else {
         WF_TRY(tree_buffers(h, gc, F, NP, tb));
         WF_TRY(forward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, F0));
+      }
+      if (fwd_only) {
+        for (int q = 0; q < S; ++q) {
+          const int slot = order[c0 + q];
+          WF_TRY(timed(h, st, WF_K_EPILOGUE, 0.0, [&] {
+            return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP,
+                                        I_dev + (int64_t)slot * D, nullptr, nullptr, nullptr, st);
+          }));
+        }
+        continue;
       }
       for (int q = 0; q < S; ++q) {
         const int slot = order[c0 + q];
@@ -1950,13 +2175,15 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
         WF_TRY(timed(h, st, WF_K_CURRENTS, 0.0, [&] {
           return wf::launch_grad_assemble(px, n, Qd, true, 6, 0, dinc + order[c0], acc, st);
         }));
+      } else if (tree64) {
+        WF_TRY(backward_chunk64(h, st, gc, dinc, order.data(), c0, S, t64, gF, acc));
       } else {
         WF_TRY(backward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, gF, acc));
       }
     }
     i0 = i1;
   }
-  WF_HIP(wf::launch_grad_scatter(acc, h->perm, n, gx, gn, ga, st));
+  if (!fwd_only) WF_HIP(wf::launch_grad_scatter(acc, h->perm, n, gx, gn, ga, st));
   WF_TRY(prof_event(h, st, &h->ev_t1));
   if (memspace == WF_MEM_HOST) {
     if (intensity)
@@ -2009,9 +2236,9 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
   if (t->chunk_solves < 0 || t->chunk_solves > 8)
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: chunk_solves must be in [0, 8]");
   if (t->agg_kernel < 0 || t->agg_kernel > 1 || t->agg_fft_rows < 0 || t->agg_fused < 0 ||
-      t->agg_fused > 1)
+      t->agg_fused > 2)
     return set_err(WF_ERR_INVALID_ARG,
-                   "wf_tune: agg_kernel and agg_fused must be 0 or 1, agg_fft_rows >= 0");
+                   "wf_tune: agg_kernel must be 0 or 1, agg_fused 0, 1 or 2, agg_fft_rows >= 0");
   h->tune = *t;
   h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
   return WF_OK;

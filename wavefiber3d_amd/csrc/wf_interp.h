@@ -105,6 +105,7 @@ struct AggFusedArgs {
   const uint16_t* Mth_lo;
   int kpad_th, mrows_th;
   int mirror;
+  int version = 1;  // 1: phi operands from global memory; 2: LDS-staged, one child ahead
 };
 bool agg_fused_fits(int ntc, int npc, int ntp, int npp);
 hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st);

@@ -370,10 +370,16 @@ class POSolver:
                               out.data_ptr(), torch.cuda.current_stream().cuda_stream))
         return out
 
-    def vjp(self, incidences, grid, grad_I, fp64: bool = False, return_I: bool = False):
+    GRAD_MODES = {"tree32": 0, "direct64": 1, "tree64": 2}
+
+    def vjp(self, incidences, grid, grad_I, fp64: bool = False, return_I: bool = False,
+            mode: str | None = None):
         """Reverse mode on the current geometry: dL/dI -> (dL/dx [3, n], dL/dn [3, n],
         dL/da [n]) in the original point order.  grad_I: numpy (host) or torch CUDA
-        [n_inc, Nt, Np].  fp64 selects the FP64 direct-sum gradient."""
+        [n_inc, Nt, Np].  mode: "tree64" (default: the tree with FP64 patterns),
+        "tree32" (FP32 patterns) or "direct64" (the FP64 direct sum, also
+        fp64=True)."""
+        code = self._grad_mode(fp64, mode)
         arr, k = _incs(incidences)
         nt, np_ = grid
         g = _lib.Grid(nt, np_)
@@ -389,7 +395,7 @@ class POSolver:
             I = torch.empty((k, nt, np_), dtype=torch.float32, device=dev) if return_I else None
             check(lib().wf_po_vjp(self._h, arr, k, g, gI.data_ptr(),
                                   I.data_ptr() if I is not None else None, gx.data_ptr(),
-                                  gn.data_ptr(), ga.data_ptr(), int(fp64), WF_MEM_DEVICE,
+                                  gn.data_ptr(), ga.data_ptr(), code, WF_MEM_DEVICE,
                                   torch.cuda.current_stream(dev).cuda_stream))
         else:
             gI = np.ascontiguousarray(grad_I, dtype=np.float32)
@@ -399,10 +405,39 @@ class POSolver:
             I = np.empty((k, nt, np_), np.float32) if return_I else None
             check(lib().wf_po_vjp(self._h, arr, k, g, gI.ctypes.data,
                                   I.ctypes.data if I is not None else None, gx.ctypes.data,
-                                  gn.ctypes.data, ga.ctypes.data, int(fp64), WF_MEM_HOST, None))
+                                  gn.ctypes.data, ga.ctypes.data, code, WF_MEM_HOST, None))
         if return_I:
             return gx, gn, ga, I
         return gx, gn, ga
+
+    def _grad_mode(self, fp64: bool, mode: str | None) -> int:
+        if fp64:
+            mode = "direct64"
+        if mode is None:
+            mode = "tree64"
+        if mode not in self.GRAD_MODES:
+            raise ValueError(f"mode must be one of {sorted(self.GRAD_MODES)}")
+        return self.GRAD_MODES[mode]
+
+    def grad_forward(self, incidences, grid, mode: str = "tree64", out: str = "torch"):
+        """The forward intensity [n_inc, Nt, Np] in the arithmetic of ``vjp(mode=...)``
+        (wf_po_vjp with grad_I = NULL), so that a loss and its gradient see the same
+        forward."""
+        code = self._grad_mode(False, mode)
+        arr, k = _incs(incidences)
+        nt, np_ = grid
+        g = _lib.Grid(nt, np_)
+        if out == "torch":
+            import torch
+
+            I = torch.empty((k, nt, np_), dtype=torch.float32, device="cuda")
+            check(lib().wf_po_vjp(self._h, arr, k, g, None, I.data_ptr(), None, None, None, code,
+                                  WF_MEM_DEVICE, torch.cuda.current_stream().cuda_stream))
+            return I
+        I = np.empty((k, nt, np_), np.float32)
+        check(lib().wf_po_vjp(self._h, arr, k, g, None, I.ctypes.data, None, None, None, code,
+                              WF_MEM_HOST, None))
+        return I
 
     def stats(self) -> dict:
         """Per-kernel HIP-event timings of the last solve (waits for its events)."""

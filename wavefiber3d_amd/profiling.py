@@ -5,12 +5,19 @@ prints a markdown table of per-kernel time (calls, total, average, share;
 static LDS and scratch per block from the trace when given).
 
 ``python -m wavefiber3d_amd.profiling traffic --fetch DIR --write DIR
---fetch-copy DIR --write-copy DIR --copy-bytes B --kernel SUBSTR [...]``
-reads the counter-collection CSVs of four ``rocprofv3 --pmc`` runs (FETCH_SIZE
-and WRITE_SIZE of the workload, the same two counters of a device copy of B
-bytes), derives the bytes-per-KiB factor of each counter from the copy
-(bytes read = bytes written = B) and prints, as JSON, the HBM traffic per
-dispatch of the kernels whose names contain SUBSTR.
+--fetch-copy DIR --write-copy DIR --copy-bytes B --kernel SUBSTR[,SUBSTR..]
+[--units U]`` reads the counter-collection CSVs of four ``rocprofv3 --pmc``
+runs (FETCH_SIZE and WRITE_SIZE of the workload, the same two counters of a
+device copy of B bytes), derives the bytes-per-KiB factor of each counter from
+the copy (bytes read = bytes written = B) and prints, as JSON, the HBM traffic
+per dispatch of the kernels whose names contain one of the SUBSTRs -- or, with
+--units, the traffic of all those dispatches divided by U (e.g. the level
+transitions of the profiled solves, whose kernels are several dispatches).
+
+``python -m wavefiber3d_amd.profiling counters DIR [DIR ...] [--kernel SUBSTR,..]``
+prints a markdown table of the per-dispatch averages of every counter found
+in the counter-collection CSVs of the given ``rocprofv3 --pmc`` runs, one row
+per kernel.
 """
 from __future__ import annotations
 
@@ -82,8 +89,12 @@ def _counter(dirname: str, counter: str) -> dict[str, list[float]]:
     return out
 
 
+def _matches(name: str, kernel: str) -> bool:
+    return any(k and k in name for k in kernel.split(","))
+
+
 def traffic(fetch: str, write: str, fetch_copy: str, write_copy: str, copy_bytes: float,
-            kernel: str) -> dict:
+            kernel: str, units: float | None = None) -> dict:
     fc = _counter(fetch_copy, "FETCH_SIZE")
     wc = _counter(write_copy, "WRITE_SIZE")
     fk = max(max(v) for k, v in fc.items() if "copyBuffer" in k)
@@ -92,16 +103,20 @@ def traffic(fetch: str, write: str, fetch_copy: str, write_copy: str, copy_bytes
     w_factor = copy_bytes / (wk * 1024.0)
     fw = _counter(fetch, "FETCH_SIZE")
     ww = _counter(write, "WRITE_SIZE")
-    fv = [x for k, v in fw.items() if kernel in k for x in v]
-    wv = [x for k, v in ww.items() if kernel in k for x in v]
+    fv = [x for k, v in fw.items() if _matches(k, kernel) for x in v]
+    wv = [x for k, v in ww.items() if _matches(k, kernel) for x in v]
     if not fv or not wv:
         raise SystemExit(f"no dispatches of kernels matching {kernel!r}")
-    f_kib, w_kib = sum(fv) / len(fv), sum(wv) / len(wv)
+    if units:
+        f_kib, w_kib = sum(fv) / units, sum(wv) / units
+    else:
+        f_kib, w_kib = sum(fv) / len(fv), sum(wv) / len(wv)
     return {
         "kernel": kernel,
         "dispatches": [len(fv), len(wv)],
-        "fetch_kib_per_dispatch": f_kib,
-        "write_kib_per_dispatch": w_kib,
+        "units": units or None,
+        "fetch_kib_per_dispatch" if not units else "fetch_kib_per_unit": f_kib,
+        "write_kib_per_dispatch" if not units else "write_kib_per_unit": w_kib,
         "copy_bytes": copy_bytes,
         "copy_fetch_kib": fk,
         "copy_write_kib": wk,
@@ -109,6 +124,35 @@ def traffic(fetch: str, write: str, fetch_copy: str, write_copy: str, copy_bytes
         "write_bytes_per_kib": w_factor * 1024.0,
         "traffic_bytes_per_launch": f_kib * 1024.0 * f_factor + w_kib * 1024.0 * w_factor,
     }
+
+
+def counter_table(dirs: list[str], kernel: str | None = None) -> str:
+    """kernel -> per-dispatch average of every counter in the runs' CSVs."""
+    vals = defaultdict(lambda: defaultdict(list))
+    order = []
+    for d in dirs:
+        path = os.path.join(d, "run_counter_collection.csv")
+        per = defaultdict(lambda: defaultdict(float))
+        names = {}
+        for r in csv.DictReader(open(path)):
+            per[r["Dispatch_Id"]][r["Counter_Name"]] += float(r["Counter_Value"])
+            names[r["Dispatch_Id"]] = short_name(r["Kernel_Name"])
+        for disp, cv in per.items():
+            for c, v in cv.items():
+                if c not in order:
+                    order.append(c)
+                vals[names[disp]][c].append(v)
+    rows = sorted(k for k in vals if kernel is None or _matches(k, kernel))
+    out = ["| kernel | dispatches | " + " | ".join(order) + " |",
+           "|---|---|" + "---|" * len(order)]
+    for k in rows:
+        n = max(len(v) for v in vals[k].values())
+        cells = []
+        for c in order:
+            v = vals[k].get(c)
+            cells.append(f"{sum(v) / len(v):.4g}" if v else "")
+        out.append(f"| `{k}` | {n} | " + " | ".join(cells) + " |")
+    return "\n".join(out)
 
 
 def main(argv=None) -> int:
@@ -125,15 +169,21 @@ def main(argv=None) -> int:
     for a in ("--fetch", "--write", "--fetch-copy", "--write-copy", "--kernel"):
         t.add_argument(a, required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
+    t.add_argument("--units", type=float, default=None)
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--kernel", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         print(kernel_table(a.stats, a.trace, a.top))
     elif a.cmd == "group":
         calls, avg = group_average_us(a.stats, a.substr)
         print(json.dumps({"kernel": a.substr, "calls": calls, "avg_us": avg}))
+    elif a.cmd == "counters":
+        print(counter_table(a.dirs, a.kernel))
     else:
         print(json.dumps(traffic(a.fetch, a.write, a.fetch_copy, a.write_copy, a.copy_bytes,
-                                 a.kernel), indent=1))
+                                 a.kernel, a.units), indent=1))
     return 0
 
 
