@@ -164,7 +164,11 @@ def main():
     ap.add_argument("--agg-fft-rows", type=int, default=None,
                     help="tune.agg_fft_rows: levels with 2 nt above it use the FFT transition")
     ap.add_argument("--agg-fused", type=int, default=None, choices=[0, 1, 2],
-                    help="tune.agg_fused: one-kernel internal level transitions (1) or two passes")
+                    help="tune.agg_fused: one-kernel internal level transitions (2, 1) or two passes")
+    ap.add_argument("--theta-tile", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="tune.theta_tile: block tile of the two-pass theta pass of 6-9 K steps")
+    ap.add_argument("--phi-rows", type=int, default=None, choices=[0, 16, 32],
+                    help="tune.phi_rows: rows per staged tile of the two-pass phi pass of long rows")
     ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
     ap.add_argument("--streams", type=int, default=4, help="fibers256: worker streams")
     ap.add_argument("--graph", action="store_true", help="capture solves into hipGraphs")
@@ -221,6 +225,10 @@ def main():
         solver.tune = {"agg_fft_rows": args.agg_fft_rows}
     if args.agg_fused is not None:
         solver.tune = {"agg_fused": args.agg_fused}
+    if args.theta_tile is not None:
+        solver.tune = {"theta_tile": args.theta_tile}
+    if args.phi_rows is not None:
+        solver.tune = {"phi_rows": args.phi_rows}
     t0 = time.perf_counter()
     solver.set_geometry(dev, W["depth"])
     torch.cuda.synchronize()

@@ -286,9 +286,15 @@ typedef struct wf_tune {
   int32_t agg_fft_rows;   /* see agg_kernel (default 600) */
   int32_t calibrated;     /* 1 after wf_autotune (or a wf_set_tune of a calibrated tune) */
   int32_t agg_fused;      /* internal MFMA level transitions whose grids fit run as one
-                             kernel, the phi output kept in LDS: 1 (phi operands read from
-                             global memory) or 2 (phi matrix and child rows staged in LDS,
-                             the next child loaded during the current one); 0: two passes */
+                             kernel, the phi output kept in LDS: 2 (default: phi matrix and
+                             child rows staged in LDS, the next child loaded during the
+                             current one) or 1 (phi operands read from global memory);
+                             0: two passes */
+  int32_t theta_tile;     /* two-pass MFMA theta pass of 6-9 K steps (161-288 great-circle
+                             samples): 0 or 1 = 128 rows x 32 column pairs per block,
+                             2 = 128 x 16, 3 = 64 x 16 */
+  int32_t phi_rows;       /* two-pass MFMA phi pass of rows longer than 160 samples: rows
+                             per staged tile, 0 or 32 (default) or 16 */
 } wf_tune;
 
 wf_status wf_get_tune(wf_handle* h, wf_tune* out);

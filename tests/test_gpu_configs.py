@@ -118,6 +118,28 @@ def test_fiber_663k_depth6_32_incidences():
     assert e64 < 1e-12
 
 
+def test_fiber_two_pass_tiles_agree():
+    """The two-pass transition of 6-9 K steps (the fiber's 125 x 240 ->
+    240 x 480 level) with each theta block tile and phi row tile gives the
+    default's result: the tiles only change which block computes a sample."""
+    s = wf.fiber_segment(40e-6, 30e-6, 30e-6, 0.1e-6, seed=0)
+    incs = [wf.Incidence(wf.reference_incidence_theta(t), 0.0, 0.4e-6, "M", 1.55 + 0.02j)
+            for t in (60, 75, 90, 105)]
+    grid = (360, 720)
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in s.arrays()]
+    out = {}
+    with wf.POSolver(device=0, beta=BETA) as sv:
+        sv.set_geometry(dev, 6)
+        for tt, pr in ((0, 0), (2, 16), (3, 16), (3, 0)):
+            print(f"case: fiber depth 6, theta_tile {tt}, phi_rows {pr}", flush=True)
+            sv.tune = {"theta_tile": tt, "phi_rows": pr}
+            out[(tt, pr)] = sv.solve(incs, grid, out="torch").cpu().numpy()
+    for k, I in out.items():
+        e = max(rel(I[j], out[(0, 0)][j]) for j in range(len(incs)))
+        print(f"  {k}: max rel-L2 vs default tiles {e:.2e}", flush=True)
+        assert e < 1e-6
+
+
 def test_uneven_batch_8_segments_graph():
     from wavefiber3d_amd.geometry import fiber_family_segment
 

@@ -90,13 +90,14 @@ def test_octree_matches_reference_partition(solver, depth):
     np.testing.assert_array_equal(ne, re_)
 
 
-@pytest.fixture(params=[(0, 100000, 1), (0, 100000, 0), (0, 0, 1), (1, 288, 1)],
-                ids=["agg_mfma_fused", "agg_mfma_2pass", "agg_fft2", "agg_fft"])
+@pytest.fixture(params=[(0, 100000, 2), (0, 100000, 1), (0, 100000, 0), (0, 0, 2), (1, 288, 2)],
+                ids=["agg_mfma_fused", "agg_mfma_fused1", "agg_mfma_2pass", "agg_fft2", "agg_fft"])
 def agg_solver(request):
     """A solver with the level-transition kernels selected: dense split-BF16
     MFMA interpolation on every level (internal levels in the one-kernel fused
-    form, or as the phi and theta passes), the large-grid LDS FFT kernels on
-    every level (5-smooth grids), or the original LDS FFT kernels."""
+    form, staged (the default) or not, or as the phi and theta passes), the
+    large-grid LDS FFT kernels on every level (5-smooth grids), or the
+    original LDS FFT kernels."""
     import wavefiber3d_amd as wfm
 
     s = wfm.POSolver(device=0)

@@ -31,7 +31,7 @@ def test_traffic_per_unit_and_counter_table(tmp_path):
                             (3, "phi_tile_kernel", "WRITE_SIZE", 10), (4, "theta_mfma_kernel", "WRITE_SIZE", 30),
                             (5, "leaf_f32_kernel", "WRITE_SIZE", 9000)])
     t = profiling.traffic(str(tmp_path / "f"), str(tmp_path / "w"), str(tmp_path / "fc"),
-                          str(tmp_path / "wc"), float(gib), "phi_,theta_", units=2)
+                          str(tmp_path / "wc"), float(gib), "phi_|theta_", units=2)
     assert t["dispatches"] == [4, 4]
     assert abs(t["fetch_bytes_per_kib"] - 2048.0) < 1e-9
     assert abs(t["write_bytes_per_kib"] - 1024.0) < 1e-9
@@ -40,7 +40,7 @@ def test_traffic_per_unit_and_counter_table(tmp_path):
     t1 = profiling.traffic(str(tmp_path / "f"), str(tmp_path / "w"), str(tmp_path / "fc"),
                            str(tmp_path / "wc"), float(gib), "theta_")
     assert abs(t1["traffic_bytes_per_launch"] - (300 * 2048 + 30 * 1024)) < 1e-6
-    tab = profiling.counter_table([str(tmp_path / "f"), str(tmp_path / "w")], "phi_,leaf")
+    tab = profiling.counter_table([str(tmp_path / "f"), str(tmp_path / "w")], "phi_|leaf")
     lines = tab.splitlines()
     assert lines[0] == "| kernel | dispatches | FETCH_SIZE | WRITE_SIZE |"
     assert lines[2] == "| `leaf_f32_kernel` | 1 | 7000 | 9000 |"

@@ -125,6 +125,8 @@ class Tune(ct.Structure):
         ("agg_fft_rows", ct.c_int32),
         ("calibrated", ct.c_int32),
         ("agg_fused", ct.c_int32),
+        ("theta_tile", ct.c_int32),
+        ("phi_rows", ct.c_int32),
     ]
 
 

@@ -285,23 +285,29 @@ __global__ __launch_bounds__(MAXE == 1 ? 768 : 512) void level_fused_mfma_kernel
 }
 
 // ---------------------------------------------------------------------------
-// Second form of the same transition, built to hide memory latency: a block
-// keeps one pair tile for its life, with that tile's phi-matrix fragments in
-// LDS; the child patterns go through an LDS buffer filled one child ahead
-// (every thread loads its share of the next child into registers at the top
-// of an iteration and stores it after the theta GEMM), and the translation
-// factors of the current child are loaded at the top of the iteration, so the
-// only global accesses on the critical path of a child are the ones issued a
-// whole iteration earlier.  X staging needs ntc * (npc + 2) <= kXMax.
-constexpr int kXMax = 2688;  // complex samples of the LDS child buffer (21.5 KB)
-constexpr int kXRegs = 4;    // float4 per thread for the prefetch
+// Second form of the same transition, built for occupancy and few VALU
+// instructions per MFMA: a block keeps one pair tile for its life, with that
+// tile's phi-matrix fragments in LDS.  For child grids of up to RT 16-row
+// tiles the child pattern is staged in LDS already split into BF16 hi / lo
+// planes in the layout of the phi GEMM's A operand, so each sample is split
+// once per (child, field) instead of once per output column tile; every
+// thread loads its chunk (one row, 8 samples) of the next (child, field) into
+// registers at the top of an iteration and splits and stores it after the
+// theta GEMM, so the global loads of a child are issued a whole iteration
+// before their use.  Larger child grids read the A operand from global
+// memory and split it per unit (RT = 0).
+constexpr int kChunk = 8;  // complex samples of a child row per staging chunk
 
-template <int NKP, int NKT, int KMAX, int MAXE, bool STAGE, int LB>
-__global__ __launch_bounds__(LB) void level_fused2_kernel(const FusedK P) {
+template <int NKP, int NKT, int KMAX, int MAXE, int RT, int LB, int WPE>
+__global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
+  constexpr bool STAGE = RT > 0;
   constexpr int NCOL = 16 * KMAX;
+  constexpr int R16 = 16 * (RT > 0 ? RT : 1);  // staged rows (zero beyond ntc)
   __shared__ bf16x8 Gs[NKT * 4 * NCOL * 4];
   __shared__ bf16x8 Ms[NKP * 2 * 2 * NCOL * 4];
-  __shared__ float4 Xs[STAGE ? kXMax / 2 : 1];
+  // [ks][plane: re hi, re lo, im hi, im lo][row][k group]: an A lane (row
+  // c16, k group g) reads 16 B, a wave 1 KB contiguously
+  __shared__ bf16x8 Xs[STAGE ? NKP * 4 * R16 * 4 : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int NW = blockDim.x >> 6;
   const int c16 = lane & 15, g = lane >> 4;
@@ -315,12 +321,13 @@ __global__ __launch_bounds__(LB) void level_fused2_kernel(const FusedK P) {
   const int64_t ub = P.used / npt;  // blocks per pair tile
   const int kbase = pt * NCOL;
   const int nk_here = min(KMAX, (half - kbase + 15) >> 4);
-  const int npcs = npc + 2;  // padded LDS row (16 B aligned, spreads rows over banks)
   {
     bf16x8 z;
 #pragma unroll
     for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
     for (int i = threadIdx.x; i < NKT * 4 * NCOL * 4; i += blockDim.x) Gs[i] = z;
+    if (STAGE)
+      for (int i = threadIdx.x; i < NKP * 4 * R16 * 4; i += blockDim.x) Xs[i] = z;
     // phi-matrix B fragments of this pair tile: unit (ks, hi/lo, column, k group)
     for (int i = threadIdx.x; i < NKP * 2 * 2 * NCOL * 4; i += blockDim.x) {
       const int gg = i & 3, col = (i >> 2) % (2 * NCOL), hl = (i / (8 * NCOL)) & 1,
@@ -351,33 +358,45 @@ __global__ __launch_bounds__(LB) void level_fused2_kernel(const FusedK P) {
       al[a][ks] = pl[kq];
     }
   }
-  // child rows as float4 (two complex samples): nx4 per child
-  const int nx4 = ntc * (npc >> 1);
-  const int nthr = blockDim.x;
-  float4 xr[kXRegs];
+  // staging chunk of this thread: row cr, samples kChunk cq .. kChunk cq + 7
+  const int nq = (npc + kChunk - 1) / kChunk;
+  const bool has_chunk = STAGE && (int)threadIdx.x < ntc * nq;
+  const int cr = (int)threadIdx.x / nq, cq = (int)threadIdx.x - cr * nq;
+  float4 xr[kChunk / 2];
   // child c, field f: its plane starts at (c F + f) ntc npc
   auto prefetch_cf = [&](int c, int f) {
-    const float4* src =
-        reinterpret_cast<const float4*>(P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc));
+    if (has_chunk) {
+      const float4* src = reinterpret_cast<const float4*>(
+          P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc) + (int64_t)cr * npc + kChunk * cq);
 #pragma unroll
-    for (int r = 0; r < kXRegs; ++r) {
-      const int i = threadIdx.x + r * nthr;
-      xr[r] = i < nx4 ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int h = 0; h < kChunk / 2; ++h)
+        xr[h] = (kChunk * cq + 2 * h < npc) ? src[h] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   };
   auto stash = [&]() {
+    if (has_chunk) {
+      float re[8], im[8];
 #pragma unroll
-    for (int r = 0; r < kXRegs; ++r) {
-      const int i = threadIdx.x + r * nthr;
-      if (i < nx4) {
-        const int row = i / (npc >> 1), c2 = i - row * (npc >> 1);
-        Xs[(row * npcs >> 1) + c2] = xr[r];
+      for (int h = 0; h < 4; ++h) {
+        re[2 * h] = xr[h].x;
+        im[2 * h] = xr[h].y;
+        re[2 * h + 1] = xr[h].z;
+        im[2 * h + 1] = xr[h].w;
       }
+      bf16x8 rh, rl, ih, il;
+      split8(re, rh, rl);
+      split8(im, ih, il);
+      const int base = (((cq >> 2) * 4) * R16 + cr) * 4 + (cq & 3);
+      Xs[base] = rh;
+      Xs[base + R16 * 4] = rl;
+      Xs[base + 2 * R16 * 4] = ih;
+      Xs[base + 3 * R16 * 4] = il;
     }
   };
   const int64_t nitems = (int64_t)P.nparents_f;
   const int nunits_max = P.nth * 2 * KMAX;
   int64_t pf = w / npt;
+  __syncthreads();  // LDS zeroed before the first child is stored
   if (STAGE && pf < nitems) {
     const int p0 = (int)(pf / F), f0 = (int)(pf - (int64_t)p0 * F);
     if (P.cb[p0] < P.ce[p0]) {
@@ -429,31 +448,34 @@ __global__ __launch_bounds__(LB) void level_fused2_kernel(const FusedK P) {
         const int kk = kbase + 16 * nt16 + c16;
         const bool kok = kk < half;
         const int th = 16 * t + c16;
-        const bool rok = th < ntc;
         floatx4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < NKP; ++ks) {
           if (ks < P.nkp) {
-            const int j0 = 32 * ks + 8 * g;
-            float vr[8], vi[8];
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-              const int jj = j0 + 2 * h;
-              float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-              if (rok && jj < npc) {
-                if (STAGE)
-                  v = Xs[(th * npcs + jj) >> 1];
-                else
-                  v = *reinterpret_cast<const float4*>(Xc + (int64_t)th * npc + jj);
-              }
-              vr[2 * h] = v.x;
-              vi[2 * h] = v.y;
-              vr[2 * h + 1] = v.z;
-              vi[2 * h + 1] = v.w;
-            }
             bf16x8 arh, arl, aih, ail;
-            split8(vr, arh, arl);
-            split8(vi, aih, ail);
+            if constexpr (STAGE) {
+              const int ib = ((ks * 4) * R16 + th) * 4 + g;
+              arh = Xs[ib];
+              arl = Xs[ib + R16 * 4];
+              aih = Xs[ib + 2 * R16 * 4];
+              ail = Xs[ib + 3 * R16 * 4];
+            } else {
+              const int j0 = 32 * ks + 8 * g;
+              const bool rok = th < ntc;
+              float vr[8], vi[8];
+#pragma unroll
+              for (int h = 0; h < 4; ++h) {
+                const int jj = j0 + 2 * h;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (rok && jj < npc) v = *reinterpret_cast<const float4*>(Xc + (int64_t)th * npc + jj);
+                vr[2 * h] = v.x;
+                vi[2 * h] = v.y;
+                vr[2 * h + 1] = v.z;
+                vi[2 * h + 1] = v.w;
+              }
+              split8(vr, arh, arl);
+              split8(vi, aih, ail);
+            }
             const int colM = 16 * n + c16;
             const bf16x8 bh = Ms[((ks * 2 + 0) * 2 * NCOL + colM) * 4 + g];
             const bf16x8 bl = Ms[((ks * 2 + 1) * 2 * NCOL + colM) * 4 + g];
@@ -555,54 +577,65 @@ __global__ __launch_bounds__(LB) void level_fused2_kernel(const FusedK P) {
   }
 }
 
-template <int NKP, int NKT, int KMAX, int MAXE, bool STAGE, int LB>
+// LDS bytes of one block of the second form
+constexpr int fused2_lds(int nkp, int nkt, int kmax, int rt) {
+  return (nkt * 4 * 16 * kmax * 4 + nkp * 2 * 2 * 16 * kmax * 4 + (rt > 0 ? nkp * 4 * 16 * rt * 4 : 1)) *
+         16;
+}
+
+template <int NKP, int NKT, int KMAX, int MAXE, int RT, int LB, int WPE>
 hipError_t launch_fused2_t(const FusedK& P, int nw, hipStream_t st) {
-  constexpr int lds = (NKT * 4 * 16 * KMAX * 4 + NKP * 2 * 2 * 16 * KMAX * 4) * 16 +
-                      (STAGE ? kXMax * 8 : 16);
+  constexpr int lds = fused2_lds(NKP, NKT, KMAX, RT);
+  static_assert(lds <= 160 * 1024, "LDS of the fused transition");
+  // resident blocks per CU: by LDS, and by waves (WPE waves per SIMD)
   int bpc = std::max(1, (160 * 1024) / lds);
-  bpc = std::min(bpc, std::max(1, (LB / 64) * 2 / nw));
+  bpc = std::min(bpc, std::max(1, (4 * WPE) / nw));
   FusedK Q = P;
   const int64_t pf_items = (int64_t)P.nparents_f;
   int64_t ub = std::max<int64_t>(1, std::min<int64_t>(256LL * bpc / P.npt, pf_items));
   Q.used = ub * P.npt;
   const int64_t nb = ((Q.used + 7) / 8) * 8;
-  hipLaunchKernelGGL((level_fused2_kernel<NKP, NKT, KMAX, MAXE, STAGE, LB>), dim3((unsigned)nb),
+  hipLaunchKernelGGL((level_fused2_kernel<NKP, NKT, KMAX, MAXE, RT, LB, WPE>), dim3((unsigned)nb),
                      dim3(64 * nw), 0, st, Q);
   return hipGetLastError();
 }
 
+// rt: staged row tiles (0: A operand from global memory).  Staged shapes
+// cap registers at 168 (3 waves per SIMD), so two blocks of up to 6 waves or
+// one of up to 12 share a CU.
 template <int NKP, int NKT, int LB>
-hipError_t launch_fused2_kt(const FusedK& P, int kmax, int maxe, bool stage, int nw, hipStream_t st) {
-  // the LDS child buffer fits beside the operands of the smaller shapes only
-  constexpr bool can_stage = NKP <= 3 && NKT <= 3;
-  if (maxe == 2) return launch_fused2_t<NKP, NKT, 4, 2, false, 512>(P, nw, st);
-  if (kmax == 3) {
-    if constexpr (can_stage) {
-      if (stage) return launch_fused2_t<NKP, NKT, 3, 1, true, LB>(P, nw, st);
+hipError_t launch_fused2_kt(const FusedK& P, int kmax, int maxe, int rt, int nw, hipStream_t st) {
+  if (maxe == 2) return launch_fused2_t<NKP, NKT, 4, 2, 0, 512, 1>(P, nw, st);
+  if constexpr (fused2_lds(NKP, NKT, 3, 2) <= 80 * 1024) {
+    if (kmax == 3 && rt == 2) return launch_fused2_t<NKP, NKT, 3, 1, 2, LB, 3>(P, nw, st);
+  }
+  if constexpr (fused2_lds(NKP, NKT, 4, 2) <= 80 * 1024) {
+    if (kmax == 4 && rt == 2) return launch_fused2_t<NKP, NKT, 4, 1, 2, LB, 3>(P, nw, st);
+  }
+  if constexpr (fused2_lds(NKP, NKT, 4, 3) <= 160 * 1024) {
+    if (rt > 0) {
+      if (kmax == 3) return launch_fused2_t<NKP, NKT, 3, 1, 3, LB, 3>(P, nw, st);
+      return launch_fused2_t<NKP, NKT, 4, 1, 3, LB, 3>(P, nw, st);
     }
-    return launch_fused2_t<NKP, NKT, 3, 1, false, LB>(P, nw, st);
   }
-  if constexpr (can_stage) {
-    if (stage) return launch_fused2_t<NKP, NKT, 4, 1, true, LB>(P, nw, st);
-  }
-  return launch_fused2_t<NKP, NKT, 4, 1, false, LB>(P, nw, st);
+  if (kmax == 3) return launch_fused2_t<NKP, NKT, 3, 1, 0, LB, 1>(P, nw, st);
+  return launch_fused2_t<NKP, NKT, 4, 1, 0, LB, 1>(P, nw, st);
 }
 
 template <int NKP, int LB>
-hipError_t launch_fused2_kl(const FusedK& P, int nkt, int kmax, int maxe, bool stage, int nw,
+hipError_t launch_fused2_kl(const FusedK& P, int nkt, int kmax, int maxe, int rt, int nw,
                             hipStream_t st) {
-  if (nkt <= 2) return launch_fused2_kt<NKP, 2, LB>(P, kmax, maxe, stage, nw, st);
-  if (nkt == 3) return launch_fused2_kt<NKP, 3, LB>(P, kmax, maxe, stage, nw, st);
-  if (nkt == 4) return launch_fused2_kt<NKP, 4, LB>(P, kmax, maxe, stage, nw, st);
-  return launch_fused2_kt<NKP, 5, LB>(P, kmax, maxe, stage, nw, st);
+  if (nkt <= 2) return launch_fused2_kt<NKP, 2, LB>(P, kmax, maxe, rt, nw, st);
+  if (nkt == 3) return launch_fused2_kt<NKP, 3, LB>(P, kmax, maxe, rt, nw, st);
+  if (nkt == 4) return launch_fused2_kt<NKP, 4, LB>(P, kmax, maxe, rt, nw, st);
+  return launch_fused2_kt<NKP, 5, LB>(P, kmax, maxe, rt, nw, st);
 }
 
-// blocks of up to 8 waves get 256 VGPRs (launch bound 512), up to 12 waves 168
 template <int NKP>
-hipError_t launch_fused2_k(const FusedK& P, int nkt, int kmax, int maxe, bool stage, int nw,
+hipError_t launch_fused2_k(const FusedK& P, int nkt, int kmax, int maxe, int rt, int nw,
                            hipStream_t st) {
-  if (nw <= 8) return launch_fused2_kl<NKP, 512>(P, nkt, kmax, maxe, stage, nw, st);
-  return launch_fused2_kl<NKP, 768>(P, nkt, kmax, maxe, stage, nw, st);
+  if (nw <= 8) return launch_fused2_kl<NKP, 512>(P, nkt, kmax, maxe, rt, nw, st);
+  return launch_fused2_kl<NKP, 768>(P, nkt, kmax, maxe, rt, nw, st);
 }
 
 template <int NKP, int NKT, int MAXE>
@@ -695,11 +728,13 @@ hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st) {
     const int kmax = (maxe == 1 && P.half <= 48) ? 3 : 4;
     P.npt = (P.half + 16 * kmax - 1) / (16 * kmax);
     P.nitems = P.nparents_f * P.npt;
-    const bool stage = maxe == 1 && a.ntc * (a.npc + 2) <= kXMax &&
-                       (int64_t)a.ntc * (a.npc / 2) <= (int64_t)kXRegs * 64 * nw;
-    if (nkp <= 2) return launch_fused2_k<2>(P, nkt, kmax, maxe, stage, nw, st);
-    if (nkp == 3) return launch_fused2_k<3>(P, nkt, kmax, maxe, stage, nw, st);
-    return launch_fused2_k<4>(P, nkt, kmax, maxe, stage, nw, st);
+    // stage the child in LDS when its rows fit 2 or 3 tiles and every
+    // staging chunk has a thread
+    const int nq = (a.npc + kChunk - 1) / kChunk;
+    const int rt = (maxe == 1 && P.nth <= 3 && a.ntc * nq <= 64 * nw) ? std::max(2, P.nth) : 0;
+    if (nkp <= 2) return launch_fused2_k<2>(P, nkt, kmax, maxe, rt, nw, st);
+    if (nkp == 3) return launch_fused2_k<3>(P, nkt, kmax, maxe, rt, nw, st);
+    return launch_fused2_k<4>(P, nkt, kmax, maxe, rt, nw, st);
   }
   if (nkp <= 2) return launch_fused_k<2>(P, nkt, maxe, nw, st);
   if (nkp == 3) return launch_fused_k<3>(P, nkt, maxe, nw, st);

@@ -70,7 +70,8 @@ typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 // at a theta that is a multiple of 8.  Block = 4 waves side by side in q
 // (wave: QT 16-column tiles, block: 64 QT columns); every wave keeps its
 // matrix fragments (QT tiles x NKS K steps, hi and lo) in registers.  The
-// block walks 32-row tiles with stride ng; each tile (32 rows x npc,
+// block walks RT-row tiles (32, or 16 to halve the LDS so that two blocks
+// share a CU) with stride ng; each tile (RT rows x npc,
 // contiguous in X within a plane) is loaded coalesced, split into BF16 hi/lo
 // and staged in LDS [buffer][k step][plane][k group][row] (MFMA lanes read
 // consecutive 16 B), double buffered with one barrier per tile and the next
@@ -78,14 +79,14 @@ typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 // column q, i.e. half of an 8-sample group; it splits them into BF16 hi/lo
 // and writes 8 B into each of the group's 4 planes (re hi, re lo, im hi,
 // im lo) of the pre-split great-circle row of q.
-template <int NKS, int QT, int D>
+template <int NKS, int QT, int D, int RT>
 __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict__ X,
                                                        bf16x4* __restrict__ GT, int planes,
                                                        int ntc, int npc, int npp,
                                                        const bf16x8* __restrict__ Mhi,
                                                        const bf16x8* __restrict__ Mlo, int kpad,
                                                        int mrows, int nqb, int ng, int64_t used) {
-  constexpr int RT = 32;
+  constexpr int MT = RT / 16;          // 16-row MFMA tiles per row tile
   constexpr int NITEM = RT * NKS * 4;  // (row, 8-sample group) items per tile
   constexpr int LI = (NITEM + 255) / 256;
   __shared__ bf16x8 As[2][NKS][4][4][RT];
@@ -180,9 +181,9 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
       __syncthreads();
       if (ltile < nrt) load(d, ltile);
       ltile += ng;
-      floatx4 accr[2][QT], acci[2][QT];
+      floatx4 accr[MT][QT], acci[MT][QT];
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < QT; ++n) {
           accr[m][n] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
       for (int ks = 0; ks < NKS; ++ks) {
         if (ks < nks) {
 #pragma unroll
-          for (int m = 0; m < 2; ++m) {
+          for (int m = 0; m < MT; ++m) {
             const int row = 16 * m + c16;
             const bf16x8 arh = As[buf][ks][0][g][row], arl = As[buf][ks][1][g][row],
                          aih = As[buf][ks][2][g][row], ail = As[buf][ks][3][g][row];
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void phi_tile_kernel(const float2* __restrict_
       }
       // D lane: column q, rows 4g..4g+3 of each M tile = half of one group
 #pragma unroll
-      for (int m = 0; m < 2; ++m) {
+      for (int m = 0; m < MT; ++m) {
         const int r4 = tile * RT + 16 * m + 4 * g;  // first of the lane's 4 rows
         if (r4 < rows) {
           const int cf = (int)((uint32_t)r4 / (uint32_t)ntc8), th = r4 - cf * ntc8;
@@ -799,17 +800,17 @@ void build_theta_matrix_gc(int ntc, int n2p, double shift, std::vector<uint16_t>
     }
 }
 
-template <int NKS>
+template <int NKS, int RT>
 hipError_t launch_phi_tile(const float2* X, float2* GT, int64_t planes, int ntc, int npc, int npp,
                            const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st) {
   constexpr int QT = 2, D = 1, BPC = 2;
   const int nqb = (npp + 64 * QT - 1) / (64 * QT);
-  const int64_t nrt = (planes * gc_half_len(ntc) + 31) / 32;
+  const int64_t nrt = (planes * gc_half_len(ntc) + RT - 1) / RT;
   int64_t ng = std::max<int64_t>(1, 256LL * BPC / nqb);
   ng = std::min<int64_t>(ng, nrt);
   const int64_t used = (int64_t)nqb * ng;
   const int64_t nb = ((used + 7) / 8) * 8;
-  hipLaunchKernelGGL((phi_tile_kernel<NKS, QT, D>), dim3((unsigned)nb), dim3(256), 0, st, X,
+  hipLaunchKernelGGL((phi_tile_kernel<NKS, QT, D, RT>), dim3((unsigned)nb), dim3(256), 0, st, X,
                      reinterpret_cast<bf16x4*>(GT), (int)planes, ntc, npc, npp,
                      reinterpret_cast<const bf16x8*>(Mhi), reinterpret_cast<const bf16x8*>(Mlo),
                      kpad, interp_rows_pad(npp), nqb, (int)ng, used);
@@ -817,7 +818,8 @@ hipError_t launch_phi_tile(const float2* X, float2* GT, int64_t planes, int ntc,
 }
 
 hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
-                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st) {
+                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st,
+                           int row_tile) {
   if (rows <= 0) return hipSuccess;
   if (ntc < 1 || npc < 2 || (npc & 1) || npc > kPhiMfmaMaxK || npp < 2 || (npp & 1) ||
       kpad < npc || (kpad & 31) || rows % ntc)
@@ -827,13 +829,21 @@ hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, i
       (int64_t)(npp / 2) * gc_row_len(ntc) >= (1LL << 31) ||
       (int64_t)interp_rows_pad(npp) * kpad >= (1LL << 31))
     return hipErrorInvalidValue;
+  if (row_tile != 0 && row_tile != 16 && row_tile != 32) return hipErrorInvalidValue;
   const int nks = (npc + 31) / 32;
-  if (nks <= 2) return launch_phi_tile<2>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
-  if (nks == 3) return launch_phi_tile<3>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
-  if (nks == 4) return launch_phi_tile<4>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
-  if (nks == 5) return launch_phi_tile<5>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
-  if (nks <= 7) return launch_phi_tile<7>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
-  return launch_phi_tile<9>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks <= 2) return launch_phi_tile<2, 32>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks == 3) return launch_phi_tile<3, 32>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks == 4) return launch_phi_tile<4, 32>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  if (nks == 5) return launch_phi_tile<5, 32>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  // long rows: 16-row tiles on request (half the LDS)
+  if (nks <= 7) {
+    if (row_tile == 16)
+      return launch_phi_tile<7, 16>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+    return launch_phi_tile<7, 32>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  }
+  if (row_tile == 16)
+    return launch_phi_tile<9, 16>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
+  return launch_phi_tile<9, 32>(X, GT, planes, ntc, npc, npp, Mhi, Mlo, kpad, st);
 }
 
 hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
@@ -847,19 +857,30 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
       (int64_t)(a.npp / 2) * gc_row_len(a.ntc) >= (1LL << 31) ||
       (int64_t)(2 * a.ntp + 1) * 8 * (a.npp / 2) >= (1LL << 31))
     return hipErrorInvalidValue;
+  if (a.tile < 0 || a.tile > 3) return hipErrorInvalidValue;
   // tile by K steps (32 great-circle samples each), measured on the sphere
-  // and fiber level shapes
+  // and fiber level shapes; 6-9 K steps also take an explicit tile (rows x
+  // column pairs): 1 = 128 x 32 (the default), 2 = 128 x 16, 3 = 64 x 16
+  // (the two smaller ones leave LDS for two blocks per CU)
   const int nks = (gc_row_len(a.ntc) + 31) / 32;
   if (a.out_root != nullptr) {
     if (nks <= 4) return launch_theta_t<2, 1, 4, 2, true, 1>(a, 2, st);
-    if (nks <= 9) return launch_theta_t<2, 2, 9, 1, true, 1>(a, 2, st);
+    if (nks <= 9) {
+      if (a.tile == 2) return launch_theta_t<2, 1, 9, 1, true, 1>(a, 2, st);
+      if (a.tile == 3) return launch_theta_t<1, 1, 9, 1, true, 2>(a, 2, st);
+      return launch_theta_t<2, 2, 9, 1, true, 1>(a, 2, st);
+    }
     return launch_theta_t<1, 1, 16, 1, true, 1>(a, 1, st);
   }
   if (nks <= 2) return launch_theta_t<2, 1, 2, 2, false, 2>(a, 2, st);
   if (nks == 3) return launch_theta_t<2, 1, 3, 2, false, 2>(a, 2, st);
   if (nks == 4) return launch_theta_t<2, 1, 4, 2, false, 2>(a, 2, st);
   if (nks == 5) return launch_theta_t<1, 1, 5, 2, false, 2>(a, 4, st);
-  if (nks <= 9) return launch_theta_t<2, 2, 9, 1, false, 1>(a, 2, st);
+  if (nks <= 9) {
+    if (a.tile == 2) return launch_theta_t<2, 1, 9, 1, false, 1>(a, 2, st);
+    if (a.tile == 3) return launch_theta_t<1, 1, 9, 1, false, 2>(a, 2, st);
+    return launch_theta_t<2, 2, 9, 1, false, 1>(a, 2, st);
+  }
   return launch_theta_t<1, 1, 16, 1, false, 1>(a, 1, st);
 }
 

@@ -512,7 +512,9 @@ void default_tune(wf_tune* t) {
   t->agg_kernel = 0;
   t->agg_fft_rows = 600;
   t->calibrated = 0;
-  t->agg_fused = 0;
+  t->agg_fused = 2;
+  t->theta_tile = 0;
+  t->phi_rows = 0;
 }
 
 // Deepest useful level of a WF_DEPTH_AUTO build: about one octant digit per
@@ -1463,7 +1465,8 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true));
       const int64_t rows = (int64_t)nchild * F * gcg.nt;
       WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
-        return wf::launch_phi_mfma(cur, G, rows, gcg.nt, gcg.np, gp.np, mp.hi, mp.lo, mp.kpad, st);
+        return wf::launch_phi_mfma(cur, G, rows, gcg.nt, gcg.np, gp.np, mp.hi, mp.lo, mp.kpad, st,
+                                   h->tune.phi_rows);
       }));
       float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
       wf::ThetaMfmaArgs ta;
@@ -1484,6 +1487,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       ta.mirror = mirror;
       ta.out = (l == 0) ? nullptr : nxt;
       ta.out_root = (l == 0) ? F0 : nullptr;
+      ta.tile = h->tune.theta_tile;
       const double wbytes = (l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np;
       WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * gp.np + wbytes,
                    [&] { return wf::launch_theta_mfma(ta, st); }));
@@ -2236,9 +2240,11 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
   if (t->chunk_solves < 0 || t->chunk_solves > 8)
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: chunk_solves must be in [0, 8]");
   if (t->agg_kernel < 0 || t->agg_kernel > 1 || t->agg_fft_rows < 0 || t->agg_fused < 0 ||
-      t->agg_fused > 2)
+      t->agg_fused > 2 || t->theta_tile < 0 || t->theta_tile > 3 ||
+      (t->phi_rows != 0 && t->phi_rows != 16 && t->phi_rows != 32))
     return set_err(WF_ERR_INVALID_ARG,
-                   "wf_tune: agg_kernel must be 0 or 1, agg_fused 0, 1 or 2, agg_fft_rows >= 0");
+                   "wf_tune: agg_kernel must be 0 or 1, agg_fused 0, 1 or 2, agg_fft_rows >= 0, "
+                   "theta_tile 0..3, phi_rows 0, 16 or 32");
   h->tune = *t;
   h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
   return WF_OK;

@@ -50,8 +50,11 @@ int interp_kpad(int k);
 // zero where theta >= ntc.  Each group of 8 samples is stored as 4 planes of
 // 8 BF16 (re hi, re lo, im hi, im lo; hi + lo = the FP32 value), i.e.
 // 64 B per group and 8 B per sample.  npc <= kPhiMfmaMaxK, npp even.
+// row_tile: rows per staged tile for rows of more than 160 samples (0 or 32:
+// 32; 16: half the LDS per block)
 hipError_t launch_phi_mfma(const float2* X, float2* GT, int64_t rows, int ntc, int npc, int npp,
-                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st);
+                           const uint16_t* Mhi, const uint16_t* Mlo, int kpad, hipStream_t st,
+                           int row_tile = 0);
 inline int gc_half_len(int ntc) { return ((ntc + 7) / 8) * 8; }
 inline int gc_row_len(int ntc) { return 2 * gc_half_len(ntc); }
 // samples (8 B each) of the phi-pass output for `planes` = nodes*F
@@ -82,6 +85,7 @@ struct ThetaMfmaArgs {
   int mirror;  // 2 ntp - 1 (half-sample grid) or 2 ntp (sample at the pole)
   float2* out;       // [parent][F][ntp][npp], or
   double2* out_root; // FP64 accumulation target of the root level
+  int tile = 0;      // 6-9 K steps: 0/1 = 128 x 32, 2 = 128 x 16, 3 = 64 x 16
 };
 hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
 

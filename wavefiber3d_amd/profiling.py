@@ -5,7 +5,7 @@ prints a markdown table of per-kernel time (calls, total, average, share;
 static LDS and scratch per block from the trace when given).
 
 ``python -m wavefiber3d_amd.profiling traffic --fetch DIR --write DIR
---fetch-copy DIR --write-copy DIR --copy-bytes B --kernel SUBSTR[,SUBSTR..]
+--fetch-copy DIR --write-copy DIR --copy-bytes B --kernel SUBSTR[|SUBSTR..]
 [--units U]`` reads the counter-collection CSVs of four ``rocprofv3 --pmc``
 runs (FETCH_SIZE and WRITE_SIZE of the workload, the same two counters of a
 device copy of B bytes), derives the bytes-per-KiB factor of each counter from
@@ -14,7 +14,7 @@ per dispatch of the kernels whose names contain one of the SUBSTRs -- or, with
 --units, the traffic of all those dispatches divided by U (e.g. the level
 transitions of the profiled solves, whose kernels are several dispatches).
 
-``python -m wavefiber3d_amd.profiling counters DIR [DIR ...] [--kernel SUBSTR,..]``
+``python -m wavefiber3d_amd.profiling counters DIR [DIR ...] [--kernel SUBSTR|..]``
 prints a markdown table of the per-dispatch averages of every counter found
 in the counter-collection CSVs of the given ``rocprofv3 --pmc`` runs, one row
 per kernel.
@@ -90,7 +90,7 @@ def _counter(dirname: str, counter: str) -> dict[str, list[float]]:
 
 
 def _matches(name: str, kernel: str) -> bool:
-    return any(k and k in name for k in kernel.split(","))
+    return any(k and k in name for k in kernel.split("|"))
 
 
 def traffic(fetch: str, write: str, fetch_copy: str, write_copy: str, copy_bytes: float,
