@@ -296,7 +296,7 @@ __global__ __launch_bounds__(MAXE == 1 ? 768 : 512) void level_fused_mfma_kernel
 // theta GEMM, so the global loads of a child are issued a whole iteration
 // before their use.  Larger child grids read the A operand from global
 // memory and split it per unit (RT = 0).
-constexpr int kChunk = 8;  // complex samples of a child row per staging chunk
+constexpr int kChunk = 4;  // complex samples of a child row per staging chunk
 
 template <int NKP, int NKT, int KMAX, int MAXE, int RT, int LB, int WPE>
 __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
@@ -305,8 +305,10 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
   constexpr int R16 = 16 * (RT > 0 ? RT : 1);  // staged rows (zero beyond ntc)
   __shared__ bf16x8 Gs[NKT * 4 * NCOL * 4];
   __shared__ bf16x8 Ms[NKP * 2 * 2 * NCOL * 4];
-  // [ks][plane: re hi, re lo, im hi, im lo][row][k group]: an A lane (row
-  // c16, k group g) reads 16 B, a wave 1 KB contiguously
+  // LDS operands are [..][k group][column or row]: MFMA lanes with consecutive
+  // c16 read consecutive 16 B (no bank conflicts).  Gs: [ks][plane: re hi,
+  // re lo, im hi, im lo][k group][column]; Ms: [ks][hi, lo][k group][column
+  // of both halves]; Xs: [ks][plane][k group][row]
   __shared__ bf16x8 Xs[STAGE ? NKP * 4 * R16 * 4 : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int NW = blockDim.x >> 6;
@@ -328,9 +330,9 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
     for (int i = threadIdx.x; i < NKT * 4 * NCOL * 4; i += blockDim.x) Gs[i] = z;
     if (STAGE)
       for (int i = threadIdx.x; i < NKP * 4 * R16 * 4; i += blockDim.x) Xs[i] = z;
-    // phi-matrix B fragments of this pair tile: unit (ks, hi/lo, column, k group)
+    // phi-matrix B fragments of this pair tile: unit (ks, hi/lo, k group, column)
     for (int i = threadIdx.x; i < NKP * 2 * 2 * NCOL * 4; i += blockDim.x) {
-      const int gg = i & 3, col = (i >> 2) % (2 * NCOL), hl = (i / (8 * NCOL)) & 1,
+      const int col = i % (2 * NCOL), gg = (i / (2 * NCOL)) & 3, hl = (i / (8 * NCOL)) & 1,
                 ks = i / (16 * NCOL);
       const int cc = col % NCOL;
       const int kk = kbase + cc;
@@ -358,10 +360,12 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
       al[a][ks] = pl[kq];
     }
   }
-  // staging chunk of this thread: row cr, samples kChunk cq .. kChunk cq + 7
+  // staging chunk of this thread: row cr, samples kChunk cq .. kChunk cq + 3
   const int nq = (npc + kChunk - 1) / kChunk;
   const bool has_chunk = STAGE && (int)threadIdx.x < ntc * nq;
-  const int cr = (int)threadIdx.x / nq, cq = (int)threadIdx.x - cr * nq;
+  // consecutive threads take consecutive rows, so their LDS stores are
+  // consecutive 16 B
+  const int cq = (int)threadIdx.x / ntc, cr = (int)threadIdx.x - cq * ntc;
   float4 xr[kChunk / 2];
   // child c, field f: its plane starts at (c F + f) ntc npc
   auto prefetch_cf = [&](int c, int f) {
@@ -375,22 +379,26 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
   };
   auto stash = [&]() {
     if (has_chunk) {
-      float re[8], im[8];
+      // samples k = 4 cq + j: K step k / 32, k group (k / 8) % 4, half (k / 4) % 2
+      bf16x4 rh, rl, ih, il;
 #pragma unroll
-      for (int h = 0; h < 4; ++h) {
-        re[2 * h] = xr[h].x;
-        im[2 * h] = xr[h].y;
-        re[2 * h + 1] = xr[h].z;
-        im[2 * h + 1] = xr[h].w;
+      for (int h = 0; h < 2; ++h) {
+        const float v[4] = {xr[h].x, xr[h].y, xr[h].z, xr[h].w};  // re, im, re, im
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const __bf16 hr = (__bf16)v[2 * q], hi_ = (__bf16)v[2 * q + 1];
+          rh[2 * h + q] = hr;
+          rl[2 * h + q] = (__bf16)(v[2 * q] - (float)hr);
+          ih[2 * h + q] = hi_;
+          il[2 * h + q] = (__bf16)(v[2 * q + 1] - (float)hi_);
+        }
       }
-      bf16x8 rh, rl, ih, il;
-      split8(re, rh, rl);
-      split8(im, ih, il);
-      const int base = (((cq >> 2) * 4) * R16 + cr) * 4 + (cq & 3);
-      Xs[base] = rh;
-      Xs[base + R16 * 4] = rl;
-      Xs[base + 2 * R16 * 4] = ih;
-      Xs[base + 3 * R16 * 4] = il;
+      const int base = (((cq >> 3) * 4) * 4 + ((cq >> 1) & 3)) * R16 + cr;
+      bf16x4* X4 = reinterpret_cast<bf16x4*>(Xs);
+      X4[2 * base + (cq & 1)] = rh;
+      X4[2 * (base + 4 * R16) + (cq & 1)] = rl;
+      X4[2 * (base + 2 * 4 * R16) + (cq & 1)] = ih;
+      X4[2 * (base + 3 * 4 * R16) + (cq & 1)] = il;
     }
   };
   const int64_t nitems = (int64_t)P.nparents_f;
@@ -454,11 +462,11 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
           if (ks < P.nkp) {
             bf16x8 arh, arl, aih, ail;
             if constexpr (STAGE) {
-              const int ib = ((ks * 4) * R16 + th) * 4 + g;
+              const int ib = ((ks * 4) * 4 + g) * R16 + th;
               arh = Xs[ib];
-              arl = Xs[ib + R16 * 4];
-              aih = Xs[ib + 2 * R16 * 4];
-              ail = Xs[ib + 3 * R16 * 4];
+              arl = Xs[ib + 4 * R16];
+              aih = Xs[ib + 2 * 4 * R16];
+              ail = Xs[ib + 3 * 4 * R16];
             } else {
               const int j0 = 32 * ks + 8 * g;
               const bool rok = th < ntc;
@@ -477,8 +485,8 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
               split8(vi, aih, ail);
             }
             const int colM = 16 * n + c16;
-            const bf16x8 bh = Ms[((ks * 2 + 0) * 2 * NCOL + colM) * 4 + g];
-            const bf16x8 bl = Ms[((ks * 2 + 1) * 2 * NCOL + colM) * 4 + g];
+            const bf16x8 bh = Ms[((ks * 2 + 0) * 4 + g) * 2 * NCOL + colM];
+            const bf16x8 bl = Ms[((ks * 2 + 1) * 4 + g) * 2 * NCOL + colM];
             dr = mfma3(arh, arl, bh, bl, dr);
             di = mfma3(aih, ail, bh, bl, di);
           }
@@ -498,18 +506,19 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
             il[j] = (__bf16)(di[j] - (float)hi_);
           }
           bf16x4* base = reinterpret_cast<bf16x4*>(Gs);
-          const int unit = ((ks * 4) * NCOL + col) * 4 + g8;
+          const int unit = ((ks * 4) * 4 + g8) * NCOL + col;
           base[2 * unit + hoff] = rh;
-          base[2 * (unit + 1 * NCOL * 4) + hoff] = rl;
-          base[2 * (unit + 2 * NCOL * 4) + hoff] = ih;
-          base[2 * (unit + 3 * NCOL * 4) + hoff] = il;
+          base[2 * (unit + 1 * 4 * NCOL) + hoff] = rl;
+          base[2 * (unit + 2 * 4 * NCOL) + hoff] = ih;
+          base[2 * (unit + 3 * 4 * NCOL) + hoff] = il;
         }
       }
       __syncthreads();
-      // ---- theta GEMM, translation, child sum
+      // ---- theta GEMM, translation, child sum (waves past the last e-tile,
+      // there to fill the block to a multiple of 4 waves, skip it)
 #pragma unroll
       for (int b = 0; b < KMAX; ++b) {
-        if (b < nk_here) {
+        if (b < nk_here && 16 * wave < n2p) {
           const int kk = kbase + 16 * b + c16;
           float2 tf[MAXE][4];
 #pragma unroll
@@ -529,10 +538,10 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
           for (int ks = 0; ks < NKT; ++ks) {
             if (ks < P.nkt) {
               const int col = 16 * b + c16;
-              const bf16x8 brh = Gs[((ks * 4 + 0) * NCOL + col) * 4 + g];
-              const bf16x8 brl = Gs[((ks * 4 + 1) * NCOL + col) * 4 + g];
-              const bf16x8 bih = Gs[((ks * 4 + 2) * NCOL + col) * 4 + g];
-              const bf16x8 bil = Gs[((ks * 4 + 3) * NCOL + col) * 4 + g];
+              const bf16x8 brh = Gs[((ks * 4 + 0) * 4 + g) * NCOL + col];
+              const bf16x8 brl = Gs[((ks * 4 + 1) * 4 + g) * NCOL + col];
+              const bf16x8 bih = Gs[((ks * 4 + 2) * 4 + g) * NCOL + col];
+              const bf16x8 bil = Gs[((ks * 4 + 3) * 4 + g) * NCOL + col];
 #pragma unroll
               for (int a = 0; a < MAXE; ++a) {
                 ur[a] = mfma3(ah[a][ks], al[a][ks], brh, brl, ur[a]);
@@ -600,17 +609,17 @@ hipError_t launch_fused2_t(const FusedK& P, int nw, hipStream_t st) {
   return hipGetLastError();
 }
 
-// rt: staged row tiles (0: A operand from global memory).  Staged shapes
-// cap registers at 168 (3 waves per SIMD), so two blocks of up to 6 waves or
-// one of up to 12 share a CU.
+// rt: staged row tiles (0: A operand from global memory).  Staged blocks
+// have a multiple of 4 waves; two blocks of up to 8 waves in 80 KB of LDS
+// each cap registers at 128 (4 waves per SIMD), one block of up to 12 at 168.
 template <int NKP, int NKT, int LB>
 hipError_t launch_fused2_kt(const FusedK& P, int kmax, int maxe, int rt, int nw, hipStream_t st) {
   if (maxe == 2) return launch_fused2_t<NKP, NKT, 4, 2, 0, 512, 1>(P, nw, st);
-  if constexpr (fused2_lds(NKP, NKT, 3, 2) <= 80 * 1024) {
-    if (kmax == 3 && rt == 2) return launch_fused2_t<NKP, NKT, 3, 1, 2, LB, 3>(P, nw, st);
+  if constexpr (fused2_lds(NKP, NKT, 3, 2) <= 80 * 1024 && LB == 512) {
+    if (kmax == 3 && rt == 2) return launch_fused2_t<NKP, NKT, 3, 1, 2, 512, 4>(P, nw, st);
   }
-  if constexpr (fused2_lds(NKP, NKT, 4, 2) <= 80 * 1024) {
-    if (kmax == 4 && rt == 2) return launch_fused2_t<NKP, NKT, 4, 1, 2, LB, 3>(P, nw, st);
+  if constexpr (fused2_lds(NKP, NKT, 4, 2) <= 80 * 1024 && LB == 512) {
+    if (kmax == 4 && rt == 2) return launch_fused2_t<NKP, NKT, 4, 1, 2, 512, 4>(P, nw, st);
   }
   if constexpr (fused2_lds(NKP, NKT, 4, 3) <= 160 * 1024) {
     if (rt > 0) {
@@ -721,17 +730,23 @@ hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st) {
   P.nparents_f = (int64_t)a.nparents * a.F;
   // one e-tile per wave up to 12 waves, else two
   const int maxe = P.net <= 12 ? 1 : 2;
-  const int nw = (P.net + maxe - 1) / maxe;
+  int nw = (P.net + maxe - 1) / maxe;
   if (a.version == 2) {
     // pair tiles of 48 columns when they cover the grid, else 64 (two
-    // e-tiles per wave run the 64-column form only)
+    // e-tiles per wave run the 64-column form only; a 16-wave block with one
+    // e-tile per wave and 32-pair tiles measured 8 % slower on 72 x 128 ->
+    // 125 x 240)
     const int kmax = (maxe == 1 && P.half <= 48) ? 3 : 4;
     P.npt = (P.half + 16 * kmax - 1) / (16 * kmax);
     P.nitems = P.nparents_f * P.npt;
     // stage the child in LDS when its rows fit 2 or 3 tiles and every
     // staging chunk has a thread
     const int nq = (a.npc + kChunk - 1) / kChunk;
-    const int rt = (maxe == 1 && P.nth <= 3 && a.ntc * nq <= 64 * nw) ? std::max(2, P.nth) : 0;
+    const int nw4 = (nw + 3) & ~3;  // waves of a staged block
+    const int rt = (maxe == 1 && P.nth <= 3 && nw4 <= 16 && a.ntc * nq <= 64 * nw4)
+                       ? std::max(2, P.nth)
+                       : 0;
+    if (rt > 0) nw = nw4;
     if (nkp <= 2) return launch_fused2_k<2>(P, nkt, kmax, maxe, rt, nw, st);
     if (nkp == 3) return launch_fused2_k<3>(P, nkt, kmax, maxe, rt, nw, st);
     return launch_fused2_k<4>(P, nkt, kmax, maxe, rt, nw, st);
