@@ -102,12 +102,33 @@ __device__ inline double2* lines64(double2* a, double2* b, int nlines, int lstri
   return a;
 }
 
+// The plan with its twiddle table copied into LDS at dst (p.n entries): the
+// butterflies then read twiddles at LDS latency.  Call from every thread; a
+// barrier must follow before the plan is used.
+__device__ inline wf::Fft64Plan plan_in_lds(const wf::Fft64Plan& p, double2* dst) {
+  for (int t = threadIdx.x; t < p.n; t += blockDim.x) dst[t] = p.tw[t];
+  wf::Fft64Plan q = p;
+  q.tw = dst;
+  return q;
+}
+
+// Phase factors of the origin shift for the band of nc samples:
+// ph[f + nc/2] = e^{i sign f shift}, f = -(nc/2) .. (nc-1)/2 (+ nc/2 for an even
+// nc), into LDS; one FP64 sine/cosine per frequency instead of one per output
+// sample and child.  Call from every thread; a barrier must follow.
+__device__ inline void shift_phases64(double2* ph, int nc, double shift, double sign) {
+  const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
+  for (int t = threadIdx.x; t <= fmax - fmin; t += blockDim.x)
+    ph[t] = zcis_rev(sign * shift * (double)(fmin + t) / WF_TWO_PI);
+}
+
 // Spectral pad/fold of spectra X (length nc) onto np bins with the origin
 // shift: Y[q] = scale sum_{f = q mod np, f in band(nc)} w_f e^{i f shift} X[f mod nc]
-// (w_f = 1/2 on the two Nyquist halves of an even nc, else 1).
+// (w_f = 1/2 on the two Nyquist halves of an even nc, else 1); ph: the
+// factors of shift_phases64 (sign +1), or null for no shift.
 __device__ inline void resample64(const double2* __restrict__ X, double2* __restrict__ Y,
-                                  int nlines, int xs, int ys, int nc, int np, double shift,
-                                  double scale) {
+                                  int nlines, int xs, int ys, int nc, int np,
+                                  const double2* __restrict__ ph, double scale) {
   const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
   const int total = nlines * np;
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
@@ -120,17 +141,18 @@ __device__ inline void resample64(const double2* __restrict__ X, double2* __rest
       const int bin = f < 0 ? f + nc : f;
       double2 v = x[bin];
       if ((nc % 2 == 0) && (f == nc / 2 || f == -(nc / 2))) v = zscale(v, 0.5);
-      if (shift != 0.0) v = zmul(v, zcis_rev(shift * (double)f / WF_TWO_PI));
+      if (ph) v = zmul(v, ph[f - fmin]);
       acc = zadd(acc, v);
     }
     Y[line * ys + q] = zscale(acc, scale);
   }
 }
 
-// Hermitian adjoint of resample64 (times scale).
+// Hermitian adjoint of resample64 (times scale); ph: shift_phases64 with
+// sign -1, or null.
 __device__ inline void resample64_adj(const double2* __restrict__ Y, double2* __restrict__ X,
-                                      int nlines, int ys, int xs, int nc, int np, double shift,
-                                      double scale) {
+                                      int nlines, int ys, int xs, int nc, int np,
+                                      const double2* __restrict__ ph, double scale) {
   const int total = nlines * nc;
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int line = t / nc, m = t - line * nc;
@@ -145,7 +167,7 @@ __device__ inline void resample64_adj(const double2* __restrict__ Y, double2* __
       if (q < 0) q += np;
       double2 v = y[q];
       if (nyq) v = zscale(v, 0.5);
-      if (shift != 0.0) v = zmul(v, zcis_rev(-shift * (double)f / WF_TWO_PI));
+      if (ph) v = zmul(v, ph[f + nc / 2]);
       acc = zadd(acc, v);
     }
     X[line * xs + m] = zscale(acc, scale);
@@ -217,7 +239,8 @@ __global__ __launch_bounds__(kT) void leaf64_kernel(wf::Leaf64Args a) {
       sc[p][f] = a.C[(int64_t)(base + p) * a.F + fb + f];
     }
     __syncthreads();
-    for (int p = 0; p < cnt; ++p) {
+#pragma unroll 2
+    for (int p = 0; p < cnt; ++p) {  // two independent phase chains in flight
       const double3 r = sr[p];
       const double2 ph = zcis_rev(s[0] * r.x + s[1] * r.y + s[2] * r.z);
 #pragma unroll
@@ -233,12 +256,14 @@ __global__ __launch_bounds__(kT) void leaf64_kernel(wf::Leaf64Args a) {
 // phi pass: [plane][ntc][npc] -> [plane][ntc][npp], rows_per_block rows per block.
 __global__ __launch_bounds__(kT) void phi64_kernel(const double2* __restrict__ in,
                                                    double2* __restrict__ out, int ntc, int npc,
-                                                   int npp, int rows_per_block, wf::Fft64Plan pc,
-                                                   wf::Fft64Plan pp) {
+                                                   int npp, int rows_per_block, wf::Fft64Plan pc0,
+                                                   wf::Fft64Plan pp0) {
   extern __shared__ double2 sm64[];
   const int lmax = npc > npp ? npc : npp;
   double2* a = sm64;
   double2* b = sm64 + rows_per_block * lmax;
+  const wf::Fft64Plan pc = plan_in_lds(pc0, b + rows_per_block * lmax);
+  const wf::Fft64Plan pp = plan_in_lds(pp0, b + rows_per_block * lmax + pc0.n);
   const int rblocks = (ntc + rows_per_block - 1) / rows_per_block;
   const int64_t plane = blockIdx.x / rblocks;
   const int row0 = (int)(blockIdx.x - plane * rblocks) * rows_per_block;
@@ -251,7 +276,7 @@ __global__ __launch_bounds__(kT) void phi64_kernel(const double2* __restrict__ i
   __syncthreads();
   double2* X = lines64(a, b, nrows, lmax, pc, 0);
   double2* Y = (X == a) ? b : a;
-  resample64(X, Y, nrows, lmax, lmax, npc, npp, 0.0, 1.0 / npc);
+  resample64(X, Y, nrows, lmax, lmax, npc, npp, nullptr, 1.0 / npc);
   __syncthreads();
   double2* Z = lines64(Y, X, nrows, lmax, pp, 1);
   double2* dst = out + (plane * ntc + row0) * (int64_t)npp;
@@ -269,12 +294,16 @@ __global__ __launch_bounds__(kT) void theta64_kernel(
     const double2* __restrict__ G, int F, int ntc, int npp, int ntp, int W,
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
     const uint32_t* __restrict__ child_key, const double2* __restrict__ T, double shift,
-    int mirror_base, wf::Fft64Plan pc, wf::Fft64Plan pp, double2* __restrict__ out) {
+    int mirror_base, wf::Fft64Plan pc0, wf::Fft64Plan pp0, double2* __restrict__ out) {
   extern __shared__ double2 sm64[];
   const int n2c = 2 * ntc, n2p = 2 * ntp;
   const int lmax = n2c > n2p ? n2c : n2p;
   double2* a = sm64;
   double2* b = sm64 + W * lmax;
+  double2* ph = sm64 + 2 * W * lmax;  // n2c + 1 shift factors, then the twiddles
+  shift_phases64(ph, n2c, shift, 1.0);  // the child loop's first barrier follows
+  const wf::Fft64Plan pc = plan_in_lds(pc0, ph + n2c + 1);
+  const wf::Fft64Plan pp = plan_in_lds(pp0, ph + n2c + 1 + pc0.n);
   const int half = npp / 2;
   const int ntiles = (half + W - 1) / W;
   const int tile = (int)(blockIdx.x % ntiles);
@@ -299,7 +328,7 @@ __global__ __launch_bounds__(kT) void theta64_kernel(
     __syncthreads();
     double2* X = lines64(a, b, wcnt, lmax, pc, 0);
     double2* Y = (X == a) ? b : a;
-    resample64(X, Y, wcnt, lmax, lmax, n2c, n2p, shift, 1.0 / n2c);
+    resample64(X, Y, wcnt, lmax, lmax, n2c, n2p, ph, 1.0 / n2c);
     __syncthreads();
     double2* Z = lines64(Y, X, wcnt, lmax, pp, 1);
     const int oct = child_key ? (int)(child_key[c] & 7u) : 0;
@@ -366,13 +395,17 @@ __global__ __launch_bounds__(kT) void trans64_kernel(double2* __restrict__ T, in
 __global__ __launch_bounds__(kT) void theta_adj64_kernel(
     const double2* __restrict__ gP, int F, int ntc, int npp, int ntp, int W,
     const int32_t* __restrict__ parent_idx, const uint32_t* __restrict__ child_key,
-    const double2* __restrict__ T, double shift, int mirror_base, wf::Fft64Plan pp,
-    wf::Fft64Plan pc, double2* __restrict__ gG) {
+    const double2* __restrict__ T, double shift, int mirror_base, wf::Fft64Plan pp0,
+    wf::Fft64Plan pc0, double2* __restrict__ gG) {
   extern __shared__ double2 sm64[];
   const int n2c = 2 * ntc, n2p = 2 * ntp;
   const int lmax = n2c > n2p ? n2c : n2p;
   double2* a = sm64;
   double2* b = sm64 + W * lmax;
+  double2* ph = sm64 + 2 * W * lmax;  // n2c + 1 shift factors (conjugate), then the twiddles
+  shift_phases64(ph, n2c, shift, -1.0);  // the barrier after the load orders it
+  const wf::Fft64Plan pp = plan_in_lds(pp0, ph + n2c + 1);
+  const wf::Fft64Plan pc = plan_in_lds(pc0, ph + n2c + 1 + pp0.n);
   const int half = npp / 2;
   const int ntiles = (half + W - 1) / W;
   const int tile = (int)(blockIdx.x % ntiles);
@@ -405,7 +438,7 @@ __global__ __launch_bounds__(kT) void theta_adj64_kernel(
   __syncthreads();
   double2* X = lines64(a, b, wcnt, lmax, pp, 0);
   double2* Y = (X == a) ? b : a;
-  resample64_adj(X, Y, wcnt, lmax, lmax, n2c, n2p, shift, 1.0 / n2c);
+  resample64_adj(X, Y, wcnt, lmax, lmax, n2c, n2p, ph, 1.0 / n2c);
   __syncthreads();
   double2* Z = lines64(Y, X, wcnt, lmax, pc, 1);
   double2* g = gG + ((int64_t)c * F + f) * ntc * (int64_t)npp;
@@ -420,11 +453,13 @@ __global__ __launch_bounds__(kT) void theta_adj64_kernel(
 __global__ __launch_bounds__(kT) void phi_adj64_kernel(const double2* __restrict__ in,
                                                        double2* __restrict__ out, int ntc,
                                                        int npc, int npp, int rows_per_block,
-                                                       wf::Fft64Plan pp, wf::Fft64Plan pc) {
+                                                       wf::Fft64Plan pp0, wf::Fft64Plan pc0) {
   extern __shared__ double2 sm64[];
   const int lmax = npc > npp ? npc : npp;
   double2* a = sm64;
   double2* b = sm64 + rows_per_block * lmax;
+  const wf::Fft64Plan pp = plan_in_lds(pp0, b + rows_per_block * lmax);
+  const wf::Fft64Plan pc = plan_in_lds(pc0, b + rows_per_block * lmax + pp0.n);
   const int rblocks = (ntc + rows_per_block - 1) / rows_per_block;
   const int64_t plane = blockIdx.x / rblocks;
   const int row0 = (int)(blockIdx.x - plane * rblocks) * rows_per_block;
@@ -437,7 +472,7 @@ __global__ __launch_bounds__(kT) void phi_adj64_kernel(const double2* __restrict
   __syncthreads();
   double2* X = lines64(a, b, nrows, lmax, pp, 0);
   double2* Y = (X == a) ? b : a;
-  resample64_adj(X, Y, nrows, lmax, lmax, npc, npp, 0.0, 1.0 / npc);
+  resample64_adj(X, Y, nrows, lmax, lmax, npc, npp, nullptr, 1.0 / npc);
   __syncthreads();
   double2* Z = lines64(Y, X, nrows, lmax, pc, 1);
   double2* dst = out + (plane * ntc + row0) * (int64_t)npc;
@@ -451,6 +486,7 @@ __global__ __launch_bounds__(kT) void phi_adj64_kernel(const double2* __restrict
 __global__ __launch_bounds__(kT) void leaf_adj64_kernel(wf::LeafAdj64Args a) {
   __shared__ double3 ss[kTile];
   __shared__ double2 sg[kTile][kFB];
+  __shared__ double2 red[kT][4];
   const int nfb = (a.F + kFB - 1) / kFB;
   const int leaf = (int)(blockIdx.x / nfb);
   const int fb = (int)(blockIdx.x % nfb) * kFB;
@@ -459,9 +495,17 @@ __global__ __launch_bounds__(kT) void leaf_adj64_kernel(wf::LeafAdj64Args a) {
   double c[3];
   leaf_centre64(a.leaf_key[leaf], a.depth, a.box_lo, a.leaf_size, c);
   const double k_rev = a.k0 / WF_TWO_PI;
-  for (int pb = b; pb < e; pb += blockDim.x) {
-    const int i = pb + threadIdx.x;
-    const bool valid = i < e;
+  // a leaf of PB < 256 points gets S = min(8, 256 / PB) threads per point,
+  // each summing every S-th direction; the S partial sums are added in a
+  // fixed order at the end (deterministic)
+  const int P = e - b;
+  const int PB = P >= kT ? kT : (P > 0 ? P : 1);
+  const int S = min(8, kT / PB);
+  const int pi = (int)threadIdx.x % PB, sl = (int)threadIdx.x / PB;
+  const bool active = sl < S;
+  for (int pb = b; pb < e; pb += PB) {
+    const int i = pb + pi;
+    const bool valid = active && i < e;
     double r[3] = {0.0, 0.0, 0.0};
     if (valid) {
       r[0] = ((double)a.px[i] - c[0]) * k_rev;
@@ -477,9 +521,9 @@ __global__ __launch_bounds__(kT) void leaf_adj64_kernel(wf::LeafAdj64Args a) {
       const int cnt = min(kTile, D - d0);
       __syncthreads();
       for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        double s[3];
-        leaf_dir64(d0 + t, a.nt, a.np, s);
-        ss[t] = make_double3(s[0], s[1], s[2]);
+        double sd[3];
+        leaf_dir64(d0 + t, a.nt, a.np, sd);
+        ss[t] = make_double3(sd[0], sd[1], sd[2]);
       }
       for (int t = threadIdx.x; t < cnt * kFB; t += blockDim.x) {
         const int dd = t / kFB, f = t - dd * kFB;
@@ -488,47 +532,58 @@ __global__ __launch_bounds__(kT) void leaf_adj64_kernel(wf::LeafAdj64Args a) {
       }
       __syncthreads();
       if (valid) {
-        for (int dd = 0; dd < cnt; ++dd) {
-          const double3 s = ss[dd];
-          const double2 ph = zcis_rev(s.x * r[0] + s.y * r[1] + s.z * r[2]);
+#pragma unroll 2
+        for (int dd = sl; dd < cnt; dd += S) {  // two independent phase chains in flight
+          const double3 sv = ss[dd];
+          const double2 ph = zcis_rev(sv.x * r[0] + sv.y * r[1] + sv.z * r[2]);
 #pragma unroll
           for (int f = 0; f < kFB; ++f) {
             const double2 g = sg[dd][f];
             // conj(P) g
             const double2 v = make_double2(ph.x * g.x + ph.y * g.y, ph.x * g.y - ph.y * g.x);
             acc[f][0] = zadd(acc[f][0], v);
-            acc[f][1] = zadd(acc[f][1], zscale(v, s.x));
-            acc[f][2] = zadd(acc[f][2], zscale(v, s.y));
-            acc[f][3] = zadd(acc[f][3], zscale(v, s.z));
+            acc[f][1] = zadd(acc[f][1], zscale(v, sv.x));
+            acc[f][2] = zadd(acc[f][2], zscale(v, sv.y));
+            acc[f][3] = zadd(acc[f][3], zscale(v, sv.z));
           }
         }
       }
     }
-    if (valid) {
+    // add the slices' partial sums, slice 0 first
 #pragma unroll
-      for (int f = 0; f < kFB; ++f) {
-        if (fb + f >= a.F) break;
+    for (int f = 0; f < kFB; ++f) {
+      __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) a.Q[((int64_t)i * a.F + fb + f) * 4 + j] = acc[f][j];
+      for (int j = 0; j < 4; ++j) red[threadIdx.x][j] = acc[f][j];
+      __syncthreads();
+      if (valid && sl == 0 && fb + f < a.F) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          double2 q = red[pi][j];
+          for (int u = 1; u < S; ++u) q = zadd(q, red[pi + u * PB][j]);
+          a.Q[((int64_t)i * a.F + fb + f) * 4 + j] = q;
+        }
       }
     }
   }
 }
 
 // LDS budget of the line kernels: two ping-pong buffers of this many double2
-constexpr int kBatch64 = 1792;          // 2 x 28 KB
+constexpr int kBatch64 = 2048;          // line buffers + tables: 64 KB
 constexpr int kSlots64 = 16;            // parent slots per thread in theta64
-constexpr size_t kLds64 = 64 * 1024;
+constexpr size_t kLds64 = 80 * 1024;  // lines + tables (a table of one long line may exceed 64 KB)
 
-int rows_for(int lmax, int ntc) {
-  int rows = kBatch64 / lmax;
+// Lines per block: two buffers of kBatch64 samples, less the tables
+// (extra samples: twiddles, shift factors) kept beside them.
+int rows_for(int lmax, int ntc, int extra) {
+  int rows = (kBatch64 - extra / 2) / lmax;
   if (rows < 1) rows = 1;
   if (rows > ntc) rows = ntc;
   return rows;
 }
 
-int cols_for(int lmax, int n2p, int half) {
-  int W = std::min(kBatch64 / lmax, (kT * kSlots64) / n2p);
+int cols_for(int lmax, int n2p, int half, int extra) {
+  int W = std::min((kBatch64 - extra / 2) / lmax, (kT * kSlots64) / n2p);
   if (W < 1) W = 1;
   if (W > half) W = half > 0 ? half : 1;
   return W;
@@ -554,10 +609,15 @@ hipError_t launch_phi64(const double2* in, double2* out, int64_t planes, int ntc
   if (planes <= 0) return hipSuccess;
   const int lmax = npc > npp ? npc : npp;
   if (lmax > 2 * kBatch64) return hipErrorInvalidValue;
-  const int rows = rows_for(lmax, ntc);
-  const size_t sh = (size_t)2 * rows * lmax * sizeof(double2);
+  const int extra = pc.n + pp.n;  // twiddle tables
+  const int rows = rows_for(lmax, ntc, extra);
+  const size_t sh = ((size_t)2 * rows * lmax + extra) * sizeof(double2);
   if (sh > kLds64) return hipErrorInvalidValue;
-  hipFuncSetAttribute((const void*)phi64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  {
+    const hipError_t e = hipFuncSetAttribute((const void*)phi64_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
   const int64_t nblk = (int64_t)((ntc + rows - 1) / rows) * planes;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL(phi64_kernel, dim3((unsigned)nblk), dim3(kT), sh, st, in, out, ntc, npc, npp,
@@ -573,14 +633,18 @@ hipError_t launch_theta64(const double2* G, int F, int ntc, int npp, int ntp, in
   const int n2c = 2 * ntc, n2p = 2 * ntp;
   const int lmax = n2c > n2p ? n2c : n2p;
   const int half = npp / 2;
-  const int W = cols_for(lmax, n2p, half);
+  const int extra = n2c + 1 + pc.n + pp.n;  // shift factors, twiddle tables
+  const int W = cols_for(lmax, n2p, half, extra);
   if (W * n2p > kT * kSlots64) return hipErrorInvalidValue;
-  const size_t sh = (size_t)2 * W * lmax * sizeof(double2);
+  const size_t sh = ((size_t)2 * W * lmax + extra) * sizeof(double2);
   if (sh > kLds64) return hipErrorInvalidValue;
   const int64_t nblk = (int64_t)((half + W - 1) / W) * F * nparents;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipFuncSetAttribute((const void*)theta64_kernel<kSlots64>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  {
+    const hipError_t e = hipFuncSetAttribute((const void*)theta64_kernel<kSlots64>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((theta64_kernel<kSlots64>), dim3((unsigned)nblk), dim3(kT), sh, st, G, F, ntc,
                      npp, ntp, W, cb, ce, child_key, T, shift, mirror_base, pc, pp, out);
   return hipGetLastError();
@@ -606,13 +670,17 @@ hipError_t launch_theta_adj64(const double2* gP, int F, int ntc, int npp, int nt
   const int n2c = 2 * ntc, n2p = 2 * ntp;
   const int lmax = n2c > n2p ? n2c : n2p;
   const int half = npp / 2;
-  const int W = cols_for(lmax, n2p, half);
-  const size_t sh = (size_t)2 * W * lmax * sizeof(double2);
+  const int extra = n2c + 1 + pc.n + pp.n;  // shift factors, twiddle tables
+  const int W = cols_for(lmax, n2p, half, extra);
+  const size_t sh = ((size_t)2 * W * lmax + extra) * sizeof(double2);
   if (sh > kLds64) return hipErrorInvalidValue;
   const int64_t nblk = (int64_t)((half + W - 1) / W) * F * nchild;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipFuncSetAttribute((const void*)theta_adj64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)sh);
+  {
+    const hipError_t e = hipFuncSetAttribute((const void*)theta_adj64_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(theta_adj64_kernel, dim3((unsigned)nblk), dim3(kT), sh, st, gP, F, ntc, npp,
                      ntp, W, parent_idx, child_key, T, shift, mirror_base, pp, pc, gG);
   return hipGetLastError();
@@ -622,11 +690,15 @@ hipError_t launch_phi_adj64(const double2* in, double2* out, int64_t planes, int
                             int npp, const Fft64Plan& pp, const Fft64Plan& pc, hipStream_t st) {
   if (planes <= 0) return hipSuccess;
   const int lmax = npc > npp ? npc : npp;
-  const int rows = rows_for(lmax, ntc);
-  const size_t sh = (size_t)2 * rows * lmax * sizeof(double2);
+  const int extra = pc.n + pp.n;  // twiddle tables
+  const int rows = rows_for(lmax, ntc, extra);
+  const size_t sh = ((size_t)2 * rows * lmax + extra) * sizeof(double2);
   if (sh > kLds64) return hipErrorInvalidValue;
-  hipFuncSetAttribute((const void*)phi_adj64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)sh);
+  {
+    const hipError_t e = hipFuncSetAttribute((const void*)phi_adj64_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
   const int64_t nblk = (int64_t)((ntc + rows - 1) / rows) * planes;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL(phi_adj64_kernel, dim3((unsigned)nblk), dim3(kT), sh, st, in, out, ntc, npc,
