@@ -169,6 +169,10 @@ def main():
                     help="tune.theta_tile: block tile of the two-pass theta pass of 6-9 K steps")
     ap.add_argument("--phi-rows", type=int, default=None, choices=[0, 16, 32],
                     help="tune.phi_rows: rows per staged tile of the two-pass phi pass of long rows")
+    ap.add_argument("--chunk-solves", type=int, default=None,
+                    help="tune.chunk_solves: solves per tree pass (1..8)")
+    ap.add_argument("--leaf-quad", type=int, default=None, choices=[0, 1],
+                    help="tune.leaf_quad: quad-symmetric leaf kernel where eligible")
     ap.add_argument("--segments", type=int, default=32, help="fibers256: segments in the batch")
     ap.add_argument("--check-all", action="store_true",
                     help="fibers256: check every segment's first solve against the FP64 direct sum")
@@ -231,6 +235,10 @@ def main():
         solver.tune = {"theta_tile": args.theta_tile}
     if args.phi_rows is not None:
         solver.tune = {"phi_rows": args.phi_rows}
+    if args.chunk_solves is not None:
+        solver.tune = {"chunk_solves": args.chunk_solves}
+    if args.leaf_quad is not None:
+        solver.tune = {"leaf_quad": args.leaf_quad}
     t0 = time.perf_counter()
     solver.set_geometry(dev, W["depth"])
     torch.cuda.synchronize()

@@ -302,8 +302,9 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t);
 
 /* Times solves of `inc` (its first wavelength group, at most 8 solves) on the
  * current geometry at up to three candidate depths <= the built depth, fits
- * the cost model and the leaf kernel choice, and returns the measured fastest
- * depth in *best_depth (nullable).  Synchronous. */
+ * the cost model, picks the leaf kernel and the level-transition kernels
+ * (agg_fused 2 or 0, theta_tile 0 or 2) by measured time, and returns the
+ * measured fastest depth in *best_depth (nullable).  Synchronous. */
 wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
                       int32_t* best_depth, void* stream);
 

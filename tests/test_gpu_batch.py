@@ -81,6 +81,7 @@ def test_autotune_calibrates_and_keeps_accuracy():
         assert t["calibrated"] == 1
         assert t["leaf_s_per_pair"] > 0 and t["agg_s_per_byte"] > 0 and t["level_s"] >= 0
         assert t["chunk_solves"] in (4, 8)
+        assert (t["agg_fused"], t["theta_tile"]) in ((2, 0), (0, 0), (2, 2))
         I = sv.solve(INCS, GRID)
         # a calibration moves between handles
         with wf.POSolver(device=0) as sv2:

@@ -448,51 +448,12 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
       const float2* To = P.Text + (int64_t)oct * (n2p + 1) * half;
       const float2* Xc = P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc);
       // ---- phi GEMM into LDS
-      for (int u = wave; u < nunits_max; u += NW) {
-        const int t = u / (2 * KMAX), n = u - t * (2 * KMAX);
-        const int nt16 = n % KMAX;
-        if (nt16 >= nk_here) continue;
-        const bool second = n >= KMAX;
+      // D lane (column c16, rows 4g..4g+3) of unit (t, nt16, half) -> its
+      // split into BF16 hi/lo, written in the theta GEMM's B layout
+      auto write_g = [&](const floatx4& dr, const floatx4& di, int t, int nt16, bool second) {
         const int kk = kbase + 16 * nt16 + c16;
-        const bool kok = kk < half;
-        const int th = 16 * t + c16;
-        floatx4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < NKP; ++ks) {
-          if (ks < P.nkp) {
-            bf16x8 arh, arl, aih, ail;
-            if constexpr (STAGE) {
-              const int ib = ((ks * 4) * 4 + g) * R16 + th;
-              arh = Xs[ib];
-              arl = Xs[ib + 4 * R16];
-              aih = Xs[ib + 2 * 4 * R16];
-              ail = Xs[ib + 3 * 4 * R16];
-            } else {
-              const int j0 = 32 * ks + 8 * g;
-              const bool rok = th < ntc;
-              float vr[8], vi[8];
-#pragma unroll
-              for (int h = 0; h < 4; ++h) {
-                const int jj = j0 + 2 * h;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (rok && jj < npc) v = *reinterpret_cast<const float4*>(Xc + (int64_t)th * npc + jj);
-                vr[2 * h] = v.x;
-                vi[2 * h] = v.y;
-                vr[2 * h + 1] = v.z;
-                vi[2 * h + 1] = v.w;
-              }
-              split8(vr, arh, arl);
-              split8(vi, aih, ail);
-            }
-            const int colM = 16 * n + c16;
-            const bf16x8 bh = Ms[((ks * 2 + 0) * 4 + g) * 2 * NCOL + colM];
-            const bf16x8 bl = Ms[((ks * 2 + 1) * 4 + g) * 2 * NCOL + colM];
-            dr = mfma3(arh, arl, bh, bl, dr);
-            di = mfma3(aih, ail, bh, bl, di);
-          }
-        }
         const int th0 = 16 * t + 4 * g;
-        if (kok && th0 < h8) {
+        if (kk < half && th0 < h8) {
           const int jg = (second ? h8 : 0) + th0;
           const int ks = jg >> 5, g8 = (jg >> 3) & 3, hoff = (jg >> 2) & 1;
           const int col = 16 * nt16 + c16;
@@ -511,6 +472,71 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
           base[2 * (unit + 1 * 4 * NCOL) + hoff] = rl;
           base[2 * (unit + 2 * 4 * NCOL) + hoff] = ih;
           base[2 * (unit + 3 * 4 * NCOL) + hoff] = il;
+        }
+      };
+      if constexpr (STAGE) {
+        // unit = (row tile, column tile of one half); A from the staged planes
+        for (int u = wave; u < nunits_max; u += NW) {
+          const int t = u / (2 * KMAX), n = u - t * (2 * KMAX);
+          const int nt16 = n % KMAX;
+          if (nt16 >= nk_here) continue;
+          const int th = 16 * t + c16;
+          floatx4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int ks = 0; ks < NKP; ++ks) {
+            if (ks < P.nkp) {
+              const int ib = ((ks * 4) * 4 + g) * R16 + th;
+              const bf16x8 arh = Xs[ib], arl = Xs[ib + 4 * R16], aih = Xs[ib + 2 * 4 * R16],
+                           ail = Xs[ib + 3 * 4 * R16];
+              const int colM = 16 * n + c16;
+              const bf16x8 bh = Ms[((ks * 2 + 0) * 4 + g) * 2 * NCOL + colM];
+              const bf16x8 bl = Ms[((ks * 2 + 1) * 4 + g) * 2 * NCOL + colM];
+              dr = mfma3(arh, arl, bh, bl, dr);
+              di = mfma3(aih, ail, bh, bl, di);
+            }
+          }
+          write_g(dr, di, t, nt16, n >= KMAX);
+        }
+      } else {
+        // unit = (row tile, column tile) for both halves: the A operand,
+        // read from global memory, is split once for the two
+        for (int u = wave; u < P.nth * KMAX; u += NW) {
+          const int t = u / KMAX, nt16 = u - t * KMAX;
+          if (nt16 >= nk_here) continue;
+          const int th = 16 * t + c16;
+          const bool rok = th < ntc;
+          floatx4 dr0 = {0.f, 0.f, 0.f, 0.f}, di0 = dr0, dr1 = dr0, di1 = dr0;
+#pragma unroll
+          for (int ks = 0; ks < NKP; ++ks) {
+            if (ks < P.nkp) {
+              const int j0 = 32 * ks + 8 * g;
+              float vr[8], vi[8];
+#pragma unroll
+              for (int h = 0; h < 4; ++h) {
+                const int jj = j0 + 2 * h;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (rok && jj < npc) v = *reinterpret_cast<const float4*>(Xc + (int64_t)th * npc + jj);
+                vr[2 * h] = v.x;
+                vi[2 * h] = v.y;
+                vr[2 * h + 1] = v.z;
+                vi[2 * h + 1] = v.w;
+              }
+              bf16x8 arh, arl, aih, ail;
+              split8(vr, arh, arl);
+              split8(vi, aih, ail);
+              const int c0 = 16 * nt16 + c16, c1 = c0 + NCOL;
+              const bf16x8 bh0 = Ms[((ks * 2 + 0) * 4 + g) * 2 * NCOL + c0];
+              const bf16x8 bl0 = Ms[((ks * 2 + 1) * 4 + g) * 2 * NCOL + c0];
+              dr0 = mfma3(arh, arl, bh0, bl0, dr0);
+              di0 = mfma3(aih, ail, bh0, bl0, di0);
+              const bf16x8 bh1 = Ms[((ks * 2 + 0) * 4 + g) * 2 * NCOL + c1];
+              const bf16x8 bl1 = Ms[((ks * 2 + 1) * 4 + g) * 2 * NCOL + c1];
+              dr1 = mfma3(arh, arl, bh1, bl1, dr1);
+              di1 = mfma3(aih, ail, bh1, bl1, di1);
+            }
+          }
+          write_g(dr0, di0, t, nt16, false);
+          write_g(dr1, di1, t, nt16, true);
         }
       }
       __syncthreads();

@@ -2334,6 +2334,30 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
     h->tune.leaf_quad = quad ? 1 : 0;
     h->tune.chunk_solves = quad ? 4 : 8;
   }
+  // level-transition kernels at the model's best depth (dense MFMA levels
+  // only): the staged fused form or two passes, and the two-pass theta tile
+  // of long rows; every candidate gives the same result to FP32 rounding
+  if (h->tune.agg_kernel == 0) {
+    static const int32_t cand[3][2] = {{2, 0}, {0, 0}, {2, 2}};  // agg_fused, theta_tile
+    double tbest = 0;
+    int ibest = 0;
+    for (int c = 0; c < 3; ++c) {
+      h->tune.agg_fused = cand[c][0];
+      h->tune.theta_tile = cand[c][1];
+      Run r;
+      e = run(rank[0].second, r);
+      if (e != WF_OK) {
+        h->tune = saved;
+        return e;
+      }
+      if (c == 0 || r.total < tbest) {
+        tbest = r.total;
+        ibest = c;
+      }
+    }
+    h->tune.agg_fused = cand[ibest][0];
+    h->tune.theta_tile = cand[ibest][1];
+  }
   std::vector<Run> runs;
   for (const auto& c : rank) {
     Run r;
