@@ -295,6 +295,9 @@ typedef struct wf_tune {
                              2 = 128 x 16, 3 = 64 x 16 */
   int32_t phi_rows;       /* two-pass MFMA phi pass of rows longer than 160 samples: rows
                              per staged tile, 0 or 32 (default) or 16 */
+  double beta;            /* level sampling constant of the forward solves in place of
+                             opts.beta (0, the default: opts.beta); set by wf_autotune.
+                             wf_po_vjp always uses opts.beta */
 } wf_tune;
 
 wf_status wf_get_tune(wf_handle* h, wf_tune* out);
@@ -303,8 +306,11 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t);
 /* Times solves of `inc` (its first wavelength group, at most 8 solves) on the
  * current geometry at up to three candidate depths <= the built depth, fits
  * the cost model, picks the leaf kernel and the level-transition kernels
- * (agg_fused 2 or 0, theta_tile 0 or 2) by measured time, and returns the
- * measured fastest depth in *best_depth (nullable).  Synchronous. */
+ * (agg_fused 2 or 0, theta_tile 0 or 2) by measured time, picks the level
+ * sampling constant tune.beta: the smallest of 3 and 2 below opts.beta whose
+ * intensities stay within 1e-4 relative L2 (every solve of the group) of
+ * the opts.beta solve, and returns the measured fastest depth in *best_depth
+ * (nullable).  Synchronous. */
 wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_grid grid,
                       int32_t* best_depth, void* stream);
 

@@ -134,7 +134,10 @@ def test_batch_matches_single_solves(use_graph):
     segs = _segments()
     depths = ["auto", 2, "auto", 1, 3, "auto"]
     print(f"case: batch of {len(segs)} segments use_graph={use_graph}", flush=True)
-    with wf.Batch(n_streams=3, device=0, use_graph=use_graph) as b:
+    # beta = 2 is the autotuner's floor, so the batch's calibration (on its
+    # largest auto-depth segment) keeps the sampling of the single-handle
+    # solves below; its kernel choices change results by FP32 rounding only
+    with wf.Batch(n_streams=3, device=0, use_graph=use_graph, beta=2.0) as b:
         for s, d in zip(segs, depths):
             b.add(s, d)
         I = b.solve(INCS, GRID)
@@ -152,7 +155,7 @@ def test_batch_matches_single_solves(use_graph):
         assert r < 1e-3
         if d != "auto":
             assert info["depth"][k] == d
-            with wf.POSolver(device=0) as sv:
+            with wf.POSolver(device=0, beta=2.0) as sv:
                 sv.tune = {"chunk_solves": 4, "leaf_quad": 1}
                 sv.set_geometry(s, d)
                 J = sv.solve(INCS, GRID)

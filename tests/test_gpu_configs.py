@@ -118,6 +118,29 @@ def test_fiber_663k_depth6_32_incidences():
     assert e64 < 1e-12
 
 
+def test_autotune_picks_beta_under_its_gate_on_the_fiber():
+    """At the library defaults (opts.beta = 4) wf_autotune lowers the fiber's
+    sampling constant to 2 (its solves stay within 1e-4 of the beta = 4 ones)
+    and the tuned solve still meets the FP64 direct sum at 1e-4."""
+    s = wf.fiber_segment(40e-6, 30e-6, 30e-6, 0.1e-6, seed=0)
+    incs = [wf.Incidence(wf.reference_incidence_theta(t), wf.reference_incidence_phi(p), 0.4e-6,
+                         "M", 1.55 + 0.02j) for t in (60, 75, 90, 105) for p in (0, 180)]
+    grid = (360, 720)
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in s.arrays()]
+    print(f"case: autotune fiber n={s.n} depth 6, 8 incidences, 360x720", flush=True)
+    with wf.POSolver(device=0) as sv:
+        sv.set_geometry(dev, 6)
+        sv.autotune(incs, grid)
+        t = sv.tune
+        print(f"  tune {t}", flush=True)
+        assert t["beta"] == 2.0
+        I = sv.solve(incs[:2], grid, out="torch").cpu().numpy()
+        I64 = sv.solve_fp64_direct(dev, incs[:2], grid).cpu().numpy()
+    e = max(rel(I[k], I64[k]) for k in range(2))
+    print(f"  rel-L2 vs FP64 {e:.2e}", flush=True)
+    assert e < TREE_GATE
+
+
 def test_fiber_two_pass_tiles_agree():
     """The two-pass transition of 6-9 K steps (the fiber's 125 x 240 ->
     240 x 480 level) with each theta block tile and phi row tile gives the

@@ -127,6 +127,7 @@ class Tune(ct.Structure):
         ("agg_fused", ct.c_int32),
         ("theta_tile", ct.c_int32),
         ("phi_rows", ct.c_int32),
+        ("beta", ct.c_double),
     ]
 
 

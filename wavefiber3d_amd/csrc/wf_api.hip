@@ -493,7 +493,8 @@ Grid level_grid(const wf_handle* h, int level, double lambda_s) {
     g.np = smooth_even_up(std::max(8, N(h->opts.ref_dphi_m)));
     return g;
   }
-  auto L = [&](double r) { return ks * r + beta * std::cbrt(ks * r); };
+  const double bt = h->tune.beta > 0 ? h->tune.beta : beta;  // the tuned constant
+  auto L = [&](double r) { return ks * r + bt * std::cbrt(ks * r); };
   g.nt = smooth_up(std::max(4, (int)std::ceil(L(R))));
   g.np = smooth_even_up(std::max(8, (int)std::ceil(2.0 * L(Rxy))));
   return g;
@@ -515,6 +516,7 @@ void default_tune(wf_tune* t) {
   t->agg_fused = 2;
   t->theta_tile = 0;
   t->phi_rows = 0;
+  t->beta = 0.0;
 }
 
 // Deepest useful level of a WF_DEPTH_AUTO build: about one octant digit per
@@ -2103,6 +2105,7 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
       h->opts.precision = prec;
     }
   } guard{h, h->tune, h->opts.precision};
+  h->tune.beta = 0.0;  // gradients sample at opts.beta, whatever the forward was tuned to
   if (!direct) {
     h->tune.agg_kernel = 1;
     if (h->opts.precision == WF_PREC_BF16X3) h->opts.precision = WF_PREC_FP32;
@@ -2241,10 +2244,11 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
     return set_err(WF_ERR_INVALID_ARG, "wf_tune: chunk_solves must be in [0, 8]");
   if (t->agg_kernel < 0 || t->agg_kernel > 1 || t->agg_fft_rows < 0 || t->agg_fused < 0 ||
       t->agg_fused > 2 || t->theta_tile < 0 || t->theta_tile > 3 ||
-      (t->phi_rows != 0 && t->phi_rows != 16 && t->phi_rows != 32))
+      (t->phi_rows != 0 && t->phi_rows != 16 && t->phi_rows != 32) ||
+      !(t->beta >= 0 && t->beta < 100))
     return set_err(WF_ERR_INVALID_ARG,
                    "wf_tune: agg_kernel must be 0 or 1, agg_fused 0, 1 or 2, agg_fft_rows >= 0, "
-                   "theta_tile 0..3, phi_rows 0, 16 or 32");
+                   "theta_tile 0..3, phi_rows 0, 16 or 32, 0 <= beta < 100");
   h->tune = *t;
   h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
   return WF_OK;
@@ -2309,13 +2313,59 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
   };
   // candidate depths: the model's three cheapest
   std::vector<std::pair<double, int>> rank;
-  for (int d = 1; d <= h->depth_max; ++d)
-    rank.push_back({model_cost(h, h->tune, d, lam_s, NT, NP), d});
-  std::sort(rank.begin(), rank.end());
-  if (rank.size() > 3) rank.resize(3);
+  auto rank_depths = [&]() {
+    rank.clear();
+    for (int d = 1; d <= h->depth_max; ++d)
+      rank.push_back({model_cost(h, h->tune, d, lam_s, NT, NP), d});
+    std::sort(rank.begin(), rank.end());
+    if (rank.size() > 3) rank.resize(3);
+  };
+  rank_depths();
+  // sampling constant: the smallest of 3 and 2 below opts.beta whose
+  // intensities stay within kBetaGate of the opts.beta solve, every solve
+  wf_status e = WF_OK;
+  if (h->opts.ref_dtheta_m <= 0 && h->opts.beta > 2.0) {
+    constexpr double kBetaGate = 1e-4;
+    std::vector<float> Iref((size_t)S * D), Ib((size_t)S * D);
+    auto fetch = [&](std::vector<float>& dst) -> wf_status {
+      WF_HIP(hipMemcpyAsync(dst.data(), I_dev, sizeof(float) * dst.size(), hipMemcpyDeviceToHost,
+                            st));
+      WF_HIP(hipStreamSynchronize(st));
+      return WF_OK;
+    };
+    Run r;
+    h->tune.beta = 0.0;
+    e = run(rank[0].second, r);
+    if (e == WF_OK) e = fetch(Iref);
+    double chosen = 0.0;
+    for (const double b : {3.0, 2.0}) {
+      if (e != WF_OK || b >= h->opts.beta) continue;
+      h->tune.beta = b;
+      e = run(rank[0].second, r);
+      if (e == WF_OK) e = fetch(Ib);
+      if (e != WF_OK) break;
+      double worst = 0.0;
+      for (int i = 0; i < S; ++i) {
+        double num = 0.0, den = 0.0;
+        for (int64_t q = 0; q < D; ++q) {
+          const double a = Ib[(size_t)i * D + q], c = Iref[(size_t)i * D + q];
+          num += (a - c) * (a - c);
+          den += c * c;
+        }
+        worst = std::max(worst, den > 0 ? std::sqrt(num / den) : 0.0);
+      }
+      if (!(worst <= kBetaGate)) break;
+      chosen = b;
+    }
+    if (e != WF_OK) {
+      h->tune = saved;
+      return e;
+    }
+    h->tune.beta = chosen;
+    rank_depths();
+  }
   // kernel variant at the model's best depth (only a BF16x3 choice; groups of
   // <= 4 solves run the quad kernel in either setting)
-  wf_status e = WF_OK;
   if (h->opts.precision == WF_PREC_BF16X3 && S > 4 && h->opts.field_chunk <= 0) {
     Run qa, qb;
     h->tune.leaf_quad = 1;
