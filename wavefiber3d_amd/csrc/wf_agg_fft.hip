@@ -102,8 +102,11 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
     const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int text_per_child,
     int mirror, Fft2Plan pc, Fft2Plan pp, const float2* __restrict__ fac_g,
-    float2* __restrict__ out, double2* __restrict__ out_root) {
+    float2* __restrict__ out, double2* __restrict__ out_root, int flip) {
   typedef typename FftAcc<ROOT>::type acc_t;
+  // transverse components: the second half of every great circle is negated
+  // on input and on output
+  const float sgn2 = flip ? -1.f : 1.f;
   extern __shared__ float2 sm[];
   const int n2c = 2 * ntc, n2p = 2 * ntp, half = npp / 2;
   const int lmax = n2c > n2p ? n2c : n2p;
@@ -146,8 +149,12 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
     for (int t = threadIdx.x; t < nw * n2c; t += blockDim.x) {
       const int w = t / n2c, jp = t - w * n2c;
       const int k = k0 + w;
-      a[w * lmax + jp] = (jp < ntc) ? S[(int64_t)k * ntc + jp]
-                                    : S[(int64_t)(k + half) * ntc + (n2c - 1 - jp)];
+      if (jp < ntc) {
+        a[w * lmax + jp] = S[(int64_t)k * ntc + jp];
+      } else {
+        const float2 v = S[(int64_t)(k + half) * ntc + (n2c - 1 - jp)];
+        a[w * lmax + jp] = make_float2(sgn2 * v.x, sgn2 * v.y);
+      }
     }
     __syncthreads();
     float2* Xs = wffft2::lines(a, b, nw, lmax, pc, twc, 0);
@@ -182,24 +189,26 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
     const int e = (int)fdiv((uint32_t)s, mag_nw), w = s - e * nw;
     const int col = k0 + w;
     int row = e, cc = col;
+    float sg = 1.f;
     if (e >= ntp) {
       const int i2 = mirror - e;  // in [0, ntp]
       row = i2 < ntp ? i2 : -1;
       cc = col + half;
+      sg = sgn2;
     }
     if (row >= 0) {
       const int64_t o = obase + (int64_t)row * npp + cc;
       if (ROOT)
-        out_root[o] = make_double2((double)acc[m].x, (double)acc[m].y);
+        out_root[o] = make_double2(sg * (double)acc[m].x, sg * (double)acc[m].y);
       else
-        out[o] = make_float2((float)acc[m].x, (float)acc[m].y);
+        out[o] = make_float2(sg * (float)acc[m].x, sg * (float)acc[m].y);
     }
     if (m == 0 && dual && e == 0) {
       const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
       if (ROOT)
-        out_root[o2] = make_double2((double)acc2.x, (double)acc2.y);
+        out_root[o2] = make_double2(sgn2 * (double)acc2.x, sgn2 * (double)acc2.y);
       else
-        out[o2] = make_float2((float)acc2.x, (float)acc2.y);
+        out[o2] = make_float2(sgn2 * (float)acc2.x, sgn2 * (float)acc2.y);
     }
   }
 }
@@ -217,7 +226,8 @@ hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_
     if (e != hipSuccess) return e;                                                                \
     hipLaunchKernelGGL((theta_fft_kernel<MS, ROOT, NT>), dim3((unsigned)nblk), dim3(NT), sh, st,   \
                        a.GT, a.F, a.ntc, a.npp, a.ntp, W, a.cb, a.ce, a.child_key, a.Text,         \
-                       a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root);         \
+                       a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root,          \
+                       a.flip);                                                                   \
     return hipGetLastError();                                                                     \
   } while (0)
   if (slots <= 4) WF_THETA_FFT(4);

@@ -271,6 +271,7 @@ struct ThetaK {
   const bf16x8* Mlo;
   float2* out;
   double2* out_root;
+  float dual_sign;  // -1 for transverse components: the pole's second sample
 };
 
 // theta pass + translation + child sum.  Block = 4 waves stacked along the
@@ -546,9 +547,11 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
                   if (a == 0 && j == 0 && dual) {
                     const int64_t o2 = obase + (int64_t)i2_0 * npp + col + half;
                     if (ROOT)
-                      P.out_root[o2] = make_double2((double)acc2[b].x, (double)acc2[b].y);
+                      P.out_root[o2] = make_double2(P.dual_sign * (double)acc2[b].x,
+                                                    P.dual_sign * (double)acc2[b].y);
                     else
-                      P.out[o2] = make_float2((float)acc2[b].x, (float)acc2[b].y);
+                      P.out[o2] = make_float2(P.dual_sign * (float)acc2[b].x,
+                                              P.dual_sign * (float)acc2[b].y);
                   }
                 }
               }
@@ -731,6 +734,7 @@ hipError_t launch_theta_t(const wf::ThetaMfmaArgs& a, int blocks_per_cu, hipStre
   P.Mlo = reinterpret_cast<const bf16x8*>(a.Mlo);
   P.out = a.out;
   P.out_root = a.out_root;
+  P.dual_sign = a.flip ? -1.f : 1.f;
   hipLaunchKernelGGL((theta_mfma_kernel<ET, CT, NKS, D, ROOT, MINB>), dim3((unsigned)nb), dim3(256), 0,
                      st, P);
   return hipGetLastError();
@@ -782,7 +786,7 @@ void build_interp_matrix(int nc, int np, double shift, std::vector<uint16_t>& hi
 }
 
 void build_theta_matrix_gc(int ntc, int n2p, double shift, std::vector<uint16_t>& hi,
-                           std::vector<uint16_t>& lo, int& rows_pad, int& kpad) {
+                           std::vector<uint16_t>& lo, int& rows_pad, int& kpad, bool flip) {
   const int nc = 2 * ntc, h8 = gc_half_len(ntc);
   rows_pad = interp_rows_pad(n2p);
   kpad = interp_kpad(2 * h8);
@@ -793,7 +797,8 @@ void build_theta_matrix_gc(int ntc, int n2p, double shift, std::vector<uint16_t>
       const int t = j < h8 ? j : j - h8;  // theta of the stored sample
       if (t >= ntc) continue;
       const int pos = j < h8 ? t : nc - 1 - t;  // its great-circle position
-      const double v = dirichlet(nc, WF_TWO_PI * q / n2p + shift - WF_TWO_PI * pos / nc);
+      double v = dirichlet(nc, WF_TWO_PI * q / n2p + shift - WF_TWO_PI * pos / nc);
+      if (flip && ((j >= h8) != (q >= n2p / 2))) v = -v;
       const uint16_t hb = bf16_bits((float)v);
       hi[(size_t)q * kpad + j] = hb;
       lo[(size_t)q * kpad + j] = bf16_bits((float)(v - (double)bf16_value(hb)));

@@ -291,10 +291,13 @@ wf_status get_fft64(wf_handle* h, int n, wf::Fft64Plan* out) {
   return WF_OK;
 }
 
-wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out, bool gc = false) {
+wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out, bool gc = false,
+                  bool flip = false) {
   uint64_t sb;
   memcpy(&sb, &shift, sizeof sb);
-  const auto key = std::make_pair(std::make_pair(gc ? -nc : nc, np), sb);
+  // keys: nc for phi matrices, -nc for theta matrices, -nc - 2^20 for the
+  // sign-flipped theta matrices of transverse components
+  const auto key = std::make_pair(std::make_pair(gc ? (flip ? -nc - (1 << 20) : -nc) : nc, np), sb);
   auto it = h->mats.find(key);
   if (it != h->mats.end()) {
     *out = it->second;
@@ -303,7 +306,7 @@ wf_status get_mat(wf_handle* h, int nc, int np, double shift, MatEntry* out, boo
   std::vector<uint16_t> hi, lo;
   MatEntry e;
   if (gc)
-    wf::build_theta_matrix_gc(nc / 2, np, shift, hi, lo, e.rows_pad, e.kpad);
+    wf::build_theta_matrix_gc(nc / 2, np, shift, hi, lo, e.rows_pad, e.kpad, flip);
   else
     wf::build_interp_matrix(nc, np, shift, hi, lo, e.rows_pad, e.kpad);
   WF_HIP(hipMalloc(&e.hi, sizeof(uint16_t) * hi.size()));
@@ -1328,32 +1331,55 @@ wf_status tree_buffers(wf_handle* h, const GroupCtx& gc, int F, int NP, TreeBufs
 // Forward tree pass of one chunk of S solves sharing a wavelength: currents ->
 // leaf sums -> aggregation -> root evaluation on the output grid.  F0 receives
 // the FP64 far-field components [6 S][Nt*Np].
+// transverse: aggregate the two transverse far-field components of each
+// solve instead of its six fields (wf_transverse.hip) when every transition
+// of the chunk runs a kernel that carries the great circle's sign flips;
+// *fps receives the fields per solve of F0 (2 or 6).
 wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const WfIncDev* dinc,
                         const int* order, int c0, int S, int NT, int NP, const TreeBufs& tb,
-                        double2* F0) {
+                        double2* F0, bool transverse, int* fps) {
   const int depth = h->depth;
   const int64_t n = h->n;
   const int64_t D = (int64_t)NT * NP;
-  const int F = 6 * S;
+  const int F6 = 6 * S;
   const float* px = h->pts;
   float2* C = nullptr;
-  WF_TRY(dalloc_t(h, "C", (size_t)n * F, &C));
+  WF_TRY(dalloc_t(h, "C", (size_t)n * F6, &C));
   for (int q = 0; q < S; ++q)
     WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n, [&] {
       return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                 px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F, 0, st);
+                                 px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F6, 0, st);
     }));
+  // the root reads level root_from directly when the levels above it have
+  // grids larger than the output grid (their transitions would only cost)
+  const int r = gc.root_from;
+  if (transverse) {
+    for (int l = depth - 1; l >= 0 && transverse; --l) {
+      if (l > 0 && l < r) continue;
+      const LevelPath lp = level_path(h, gc.g[(l == 0) ? r : l + 1], gc.g[l]);
+      if (!lp.mfma && !lp.fft2) transverse = false;
+    }
+  }
+  const int F = transverse ? 2 * S : F6;  // fields per level pattern
+  *fps = transverse ? 2 : 6;
   float2* cur = (depth % 2) ? tb.patA : tb.patB;
-  const wf::LeafArgs la = leaf_args(h, gc, C, F, cur);
+  const wf::LeafArgs la = leaf_args(h, gc, C, F6, cur);
   const double dleaf = (double)gc.g[depth].nt * gc.g[depth].np;
-  WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F, [&] {
+  WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F6, [&] {
     if (h->opts.precision == WF_PREC_FP32_VALU) return wf::launch_leaf(la, st);
     return wf::launch_leaf_mfma(la, h->opts.precision == WF_PREC_BF16X3, st);
   }));
   h->stats.leaf_pairs += (int64_t)(dleaf * n);
-  // the root reads level root_from directly when the levels above it have
-  // grids larger than the output grid (their transitions would only cost)
-  const int r = gc.root_from;
+  if (transverse) {
+    float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
+    const int nleaf = h->levels[depth].count;
+    const double z0 = std::sqrt((4e-7 * WF_PI) / 8.8541878e-12);  // as the epilogue
+    WF_TRY(timed(h, st, WF_K_TRANS, 8.0 * (F6 + F) * nleaf * dleaf, [&] {
+      return wf::launch_project_transverse(cur, nxt, nleaf, S, gc.g[depth].nt, gc.g[depth].np,
+                                           theta_origin(depth, gc.g[depth].nt), z0, st);
+    }));
+    cur = nxt;
+  }
   for (int l = depth - 1; l >= 0; --l) {
     if (l > 0 && l < r) continue;
     if (cancelled(h)) return set_err(WF_ERR_CANCELLED, "cancelled");
@@ -1416,6 +1442,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       ta.fac = fac_t;
       ta.out = (l == 0) ? nullptr : nxt;
       ta.out_root = (l == 0) ? F0 : nullptr;
+      ta.flip = transverse ? 1 : 0;
       const double wbytes = (l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np;
       WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * gp.np + wbytes,
                    [&] { return wf::launch_theta_fft(ta, st); }));
@@ -1427,7 +1454,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       // one kernel: phi GEMM into LDS, theta GEMM, translation, child sum
       MatEntry mp, mt;
       WF_TRY(get_mat(h, gcg.np, gp.np, 0.0, &mp));
-      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true));
+      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true, transverse));
       float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
       wf::AggFusedArgs fa;
       fa.X = cur;
@@ -1464,7 +1491,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       // translation + child sum -> parent (FP64 at the root)
       MatEntry mp, mt;
       WF_TRY(get_mat(h, gcg.np, gp.np, 0.0, &mp));
-      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true));
+      WF_TRY(get_mat(h, 2 * gcg.nt, 2 * gp.nt, off_p - off_c, &mt, true, transverse));
       const int64_t rows = (int64_t)nchild * F * gcg.nt;
       WF_TRY(timed(h, st, WF_K_PHI, 8.0 * planes * gcg.nt * ((double)gcg.np + gp.np), [&] {
         return wf::launch_phi_mfma(cur, G, rows, gcg.nt, gcg.np, gp.np, mp.hi, mp.lo, mp.kpad, st,
@@ -1490,6 +1517,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       ta.out = (l == 0) ? nullptr : nxt;
       ta.out_root = (l == 0) ? F0 : nullptr;
       ta.tile = h->tune.theta_tile;
+      ta.flip = transverse ? 1 : 0;
       const double wbytes = (l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np;
       WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * gp.np + wbytes,
                    [&] { return wf::launch_theta_mfma(ta, st); }));
@@ -1798,6 +1826,7 @@ wf_status run_chunks(wf_handle* w, hipStream_t st, const std::vector<Chunk>& cs,
     const int depth = w->depth;
     const int c0 = ch.c0, S = ch.S;
     const int F = 6 * S;
+    int fps = 6;  // fields per solve of F0: 6, or 2 transverse components
     if (cancelled(w)) return set_err(WF_ERR_CANCELLED, "cancelled");
     double2* F0 = nullptr;
     WF_TRY(dalloc_t(w, "F0", (size_t)D * F, &F0));
@@ -1819,11 +1848,16 @@ wf_status run_chunks(wf_handle* w, hipStream_t st, const std::vector<Chunk>& cs,
     } else {
       TreeBufs tb;
       WF_TRY(tree_buffers(w, gc, F, NP, tb));
-      WF_TRY(forward_chunk(w, st, gc, dinc, order, c0, S, NT, NP, tb, F0));
+      WF_TRY(forward_chunk(w, st, gc, dinc, order, c0, S, NT, NP, tb, F0, true, &fps));
     }
     for (int q = 0; q < S; ++q) {
       const int slot = order[c0 + q];
       WF_TRY(timed(w, st, WF_K_EPILOGUE, epi_bytes, [&] {
+        if (fps == 2)
+          return wf::launch_intensity_transverse(F0 + (int64_t)q * 2 * D, 1, NT, NP,
+                                                 I_dev + (int64_t)slot * D,
+                                                 E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr,
+                                                 nullptr, nullptr, st);
         return wf::launch_intensity(F0 + (int64_t)q * 6 * D, 1, NT, NP, I_dev + (int64_t)slot * D,
                                     E_dev ? E_dev + (int64_t)slot * 6 * D : nullptr, nullptr,
                                     nullptr, st);
@@ -2152,7 +2186,8 @@ wf_status wf_po_vjp(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_gri
         }));
       } else {
         WF_TRY(tree_buffers(h, gc, F, NP, tb));
-        WF_TRY(forward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, F0));
+        int fps6 = 6;  // gradients: the six fields (the reverse pass needs them)
+        WF_TRY(forward_chunk(h, st, gc, dinc, order.data(), c0, S, NT, NP, tb, F0, false, &fps6));
       }
       if (fwd_only) {
         for (int q = 0; q < S; ++q) {

@@ -116,6 +116,9 @@ hipError_t launch_direct_fp64(const float* x, const float* y, const float* z, in
 
 hipError_t launch_intensity(const double2* F0, int S, int ntheta, int nphi, float* I, float* E,
                             double* Id, double* Ed, hipStream_t st);
+// the same from the transverse components (2 per solve, launch_project_transverse)
+hipError_t launch_intensity_transverse(const double2* F0, int S, int ntheta, int nphi, float* I,
+                                       float* E, double* Id, double* Ed, hipStream_t st);
 
 hipError_t launch_validate(const float* x, const float* y, const float* z, const float* nx,
                            const float* ny, const float* nz, const float* area, int64_t n,

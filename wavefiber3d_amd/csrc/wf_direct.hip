@@ -159,6 +159,38 @@ __global__ __launch_bounds__(256) void intensity_kernel(const double2* __restric
   }
 }
 
+// The intensity from the transverse components a = t.E'', b = p.E'' of each
+// solve (F0 [2 S][D]): I = |a|^2 + |b|^2, E'' = a t + b p with t, p the unit
+// vectors of increasing theta and phi at the output direction.
+__global__ __launch_bounds__(256) void intensity_transverse_kernel(
+    const double2* __restrict__ F0, int S, int ntheta, int nphi, float* __restrict__ If,
+    float* __restrict__ Ef, double* __restrict__ Id, double* __restrict__ Ed) {
+  const int64_t D = (int64_t)ntheta * nphi;
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s_ = blockIdx.y;
+  if (d >= D) return;
+  const double2 a = F0[(int64_t)(2 * s_) * D + d], b = F0[(int64_t)(2 * s_ + 1) * D + d];
+  const double I = a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y;
+  if (If) If[(int64_t)s_ * D + d] = (float)I;
+  if (Id) Id[(int64_t)s_ * D + d] = I;
+  if (Ef || Ed) {
+    const int j = (int)(d / nphi), kk = (int)(d - (int64_t)j * nphi);
+    const double th = -WF_PI / 2 + 1e-10 + WF_PI / ntheta * j;  // as grid_dir
+    const double ph = WF_TWO_PI / nphi * kk;
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    const double t[3] = {-st * cp, -st * sp, ct}, pv[3] = {-sp, cp, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2 o = zadd(zscale(a, t[c]), zscale(b, pv[c]));
+      const int64_t e = ((int64_t)s_ * 3 + c) * D + d;
+      if (Ef) reinterpret_cast<float2*>(Ef)[e] = make_float2((float)o.x, (float)o.y);
+      if (Ed) reinterpret_cast<double2*>(Ed)[e] = o;
+    }
+  }
+}
+
 }  // namespace
 
 namespace wf {
@@ -220,6 +252,15 @@ hipError_t launch_intensity(const double2* F0, int S, int ntheta, int nphi, floa
   dim3 grid((unsigned)((D + 255) / 256), (unsigned)S);
   hipLaunchKernelGGL(intensity_kernel, grid, dim3(256), 0, st, F0, S, ntheta, nphi, z0, I, E, Id,
                      Ed);
+  return hipGetLastError();
+}
+
+hipError_t launch_intensity_transverse(const double2* F0, int S, int ntheta, int nphi, float* I,
+                                       float* E, double* Id, double* Ed, hipStream_t st) {
+  const int64_t D = (int64_t)ntheta * nphi;
+  dim3 grid((unsigned)((D + 255) / 256), (unsigned)S);
+  hipLaunchKernelGGL(intensity_transverse_kernel, grid, dim3(256), 0, st, F0, S, ntheta, nphi, I,
+                     E, Id, Ed);
   return hipGetLastError();
 }
 

@@ -64,8 +64,11 @@ inline int64_t gc_elems(int64_t planes, int ntc, int npp) {
 // the theta matrix in the same column order: [rows_pad >= 2 ntp][kpad >=
 // gc_row_len(ntc)], column j < gc_half_len -> great-circle sample j, column
 // gc_half_len + t -> sample 2 ntc - 1 - t, zero for padding
+// flip: for transverse components (see launch_project_transverse): the
+// columns of the second half of the great circle and the rows e >= n2p/2 are
+// negated (the transverse unit vectors change sign across the pole)
 void build_theta_matrix_gc(int ntc, int n2p, double shift, std::vector<uint16_t>& hi,
-                           std::vector<uint16_t>& lo, int& rows_pad, int& kpad);
+                           std::vector<uint16_t>& lo, int& rows_pad, int& kpad, bool flip = false);
 // largest npc of the phi MFMA pass and largest gc_row_len(ntc) of the theta
 // MFMA pass (their matrix fragments stay in registers for all K steps)
 constexpr int kPhiMfmaMaxK = 288;
@@ -86,6 +89,8 @@ struct ThetaMfmaArgs {
   float2* out;       // [parent][F][ntp][npp], or
   double2* out_root; // FP64 accumulation target of the root level
   int tile = 0;      // 6-9 K steps: 0/1 = 128 x 32, 2 = 128 x 16, 3 = 64 x 16
+  int flip = 0;      // transverse components: the pole's second sample is negated
+                     // (the matrix carries the other signs, build_theta_matrix_gc)
 };
 hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st);
 
@@ -168,8 +173,22 @@ struct ThetaFftArgs {
   const float2* fac;   // resample factors, 2 ntc + 1
   float2* out;
   double2* out_root;
+  int flip = 0;        // transverse components: the second half of every great
+                       // circle is negated on input and on output
 };
 hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st);
+
+// Transverse components of the far field from the six radiated fields of each
+// solve: F [node][6 S][nt][np] (E x, y, z, H x, y, z; level grid theta_j =
+// theta0 + pi j / nt, phi_k = 2 pi k / np) -> P [node][2 S][nt][np] with
+//   P[2s]   = t.E - z0 p.H,   P[2s+1] = p.E + z0 t.H,
+// t = (-sin th cos ph, -sin th sin ph, cos th), p = (-sin ph, cos ph, 0): the
+// components along t and p of E'' = E_perp - z0 s x H_perp, the field whose
+// squared norm is the intensity.  Each is a band-limited function of
+// (theta, phi) of one degree more than E and H, and changes sign with t and p
+// across the pole, which the theta passes' flip handles.
+hipError_t launch_project_transverse(const float2* F, float2* P, int64_t nodes, int S, int nt,
+                                     int np, double theta0, double z0, hipStream_t st);
 
 // Leaf radiation sum: pattern[leaf][F][nt][np] of
 // sum_{n in leaf} C[n][f] exp(+i k0 s.(x_n - c_leaf)) on the leaf grid
