@@ -213,6 +213,193 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
   }
 }
 
+// ---------------------------------------------------------------- wave lines --
+// phi: block = nw waves on nw consecutive theta rows of one (child, field)
+// plane; each wave transforms its row (FFT(npc), resample, inverse FFT(npp))
+// and leaves the npp outputs in its LDS line; the block then writes
+// GT[cf][q][r0 .. r0 + nw) with the rows contiguous.
+template <int NMAX, int RL>
+__global__ __launch_bounds__(512) void phi_fft3_kernel(const float2* __restrict__ X,
+                                                       float2* __restrict__ GT, int ntc, int npc,
+                                                       int npp, Fft2Plan pc, Fft2Plan pp,
+                                                       const float2* __restrict__ fac_g) {
+  extern __shared__ float2 sm[];
+  const int nw = (int)(blockDim.x >> 6), wave = (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int lmax = npc > npp ? npc : npp;
+  float2* twc = sm;
+  float2* twp = twc + pc.twn;
+  float2* fac = twp + pp.twn;
+  float2* lines = fac + (npc + 1);
+  load_table(twc, pc.tw, pc.twn);
+  load_table(twp, pp.tw, pp.twn);
+  load_table(fac, fac_g, npc + 1);
+  __syncthreads();
+  const int rblocks = (ntc + nw - 1) / nw;
+  const int64_t cf = blockIdx.x / rblocks;
+  const int r0 = (int)(blockIdx.x - cf * rblocks) * nw;
+  const int nr = min(nw, ntc - r0);
+  float2* buf = lines + wave * lmax;
+  if (wave < nr) {
+    const float2* row = X + (cf * ntc + r0 + wave) * (int64_t)npc;
+    auto src = [&](int j) { return row[j]; };
+    wffft3::forward<NMAX>(pc, buf, twc, src, lane);
+    float2 o[wffft3::Slots<NMAX, RL>::v];
+    auto keep = [&](int s, int, float2 z) { o[s] = z; };
+    wffft3::inverse<NMAX, RL>(pp, buf, twp, fac, npc, keep, lane);
+    auto put = [&](int s, int e) { buf[e] = o[s]; };
+    wffft3::for_outputs<NMAX, RL>(pp, put, lane);
+  }
+  __syncthreads();
+  float2* dst = GT + cf * (int64_t)npp * ntc + r0;
+  for (int t = threadIdx.x; t < nr * npp; t += blockDim.x) {
+    const int q = t / nr, w = t - q * nr;
+    dst[(int64_t)q * ntc + w] = lines[w * lmax + q];
+  }
+}
+
+// theta: block = kNw waves on kNw consecutive great circles (column pairs k,
+// k + npp/2) of one (parent, field); each wave runs every child's line
+// (FFT(2 ntc), resample with the origin shift, inverse FFT(2 ntp)), multiplies
+// its outputs by the child's translation factors (k-major table: one
+// contiguous row per great circle) and accumulates them in registers (FP64 at
+// the root).  Block b -> (u = parent * nkg + k group, field): the F blocks of
+// one u are consecutive on one XCD (blocks go round-robin over the 8 XCDs),
+// so a root child's table rows are fetched into that XCD's L2 once for all
+// fields.
+constexpr int kNw = 4;
+template <int NMAX, int RL, bool ROOT>
+__global__ __launch_bounds__(64 * kNw) void theta_fft3_kernel(
+    const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, int64_t U, int nkg,
+    const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
+    const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int text_per_child,
+    int mirror, Fft2Plan pc, Fft2Plan pp, const float2* __restrict__ fac_g,
+    float2* __restrict__ out, double2* __restrict__ out_root, int flip) {
+  typedef typename FftAcc<ROOT>::type acc_t;
+  extern __shared__ float2 sm[];
+  const int wave = (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int n2c = 2 * ntc, n2p = 2 * ntp, half = npp / 2;
+  const int lmax = n2c > n2p ? n2c : n2p;
+  float2* twc = sm;
+  float2* twp = twc + pc.twn;
+  float2* fac = twp + pp.twn;
+  float2* lines = fac + (n2c + 1);
+  load_table(twc, pc.tw, pc.twn);
+  load_table(twp, pp.tw, pp.twn);
+  load_table(fac, fac_g, n2c + 1);
+  __syncthreads();
+  const int xcd = (int)(blockIdx.x & 7u);
+  const int64_t j = blockIdx.x >> 3;
+  const int f = (int)(j % F);
+  const int64_t u = (j / F) * 8 + xcd;
+  if (u >= U) return;
+  const int p = (int)(u / nkg);
+  const int k = (int)(u - (int64_t)p * nkg) * kNw + wave;
+  if (k >= half) return;
+  // transverse components: the second half of every great circle negated
+  const float sgn2 = flip ? -1.f : 1.f;
+  const int i2_0 = mirror % n2p;
+  const bool dual = (i2_0 < ntp);
+  constexpr int NS = wffft3::Slots<NMAX, RL>::v;
+  acc_t acc[NS], acc2;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    acc[s].x = 0;
+    acc[s].y = 0;
+  }
+  acc2.x = 0;
+  acc2.y = 0;
+  float2* buf = lines + wave * lmax;
+  const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
+  for (int c = c0; c < c1; ++c) {
+    // the lane index through an empty asm: the stages' lane-derived addresses
+    // are recomputed per child instead of all being hoisted out of the loop
+    int ln = lane;
+    __asm__ volatile("" : "+v"(ln));
+    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
+    const float2* S0 = S + (int64_t)k * ntc;
+    const float2* S1 = S + (int64_t)(k + half) * ntc + (n2c - 1);
+    auto src = [&](int jp) -> float2 {
+      if (jp < ntc) return S0[jp];
+      const float2 v = S1[-jp];
+      return make_float2(sgn2 * v.x, sgn2 * v.y);
+    };
+    wffft3::forward<NMAX>(pc, buf, twc, src, ln);
+    const int oct = text_per_child ? c : (child_key ? (int)(child_key[c] & 7u) : 0);
+    const float2* To = Text ? Text + ((int64_t)oct * half + k) * (n2p + 1) : nullptr;
+    auto sink = [&](int s, int e, float2 z) {
+      const float2 tu = To ? cmul(To[e], z) : z;
+      acc[s].x += tu.x;
+      acc[s].y += tu.y;
+      if (dual && e == 0) {
+        const float2 t2 = To ? cmul(To[n2p], z) : z;
+        acc2.x += t2.x;
+        acc2.y += t2.y;
+      }
+    };
+    wffft3::inverse<NMAX, RL>(pp, buf, twp, fac, n2c, sink, ln);
+  }
+  const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
+  auto put = [&](int s, int e) {
+    int row = e, cc = k;
+    float sg = 1.f;
+    if (e >= ntp) {
+      const int i2 = mirror - e;  // in [0, ntp]
+      row = i2 < ntp ? i2 : -1;
+      cc = k + half;
+      sg = sgn2;
+    }
+    if (row >= 0) {
+      const int64_t o = obase + (int64_t)row * npp + cc;
+      if constexpr (ROOT)
+        out_root[o] = make_double2(sg * (double)acc[s].x, sg * (double)acc[s].y);
+      else
+        out[o] = make_float2(sg * (float)acc[s].x, sg * (float)acc[s].y);
+    }
+    if (dual && e == 0) {
+      const int64_t o2 = obase + (int64_t)i2_0 * npp + k + half;
+      if constexpr (ROOT)
+        out_root[o2] = make_double2(sgn2 * (double)acc2.x, sgn2 * (double)acc2.y);
+      else
+        out[o2] = make_float2(sgn2 * (float)acc2.x, sgn2 * (float)acc2.y);
+    }
+  };
+  wffft3::for_outputs<NMAX, RL>(pp, put, lane);
+}
+
+template <int NMAX, int RL, bool ROOT>
+hipError_t launch_theta_fft3_t(const wf::ThetaFftArgs& a, hipStream_t st) {
+  const int n2c = 2 * a.ntc, n2p = 2 * a.ntp, half = a.npp / 2;
+  const int lmax = n2c > n2p ? n2c : n2p;
+  const size_t sh = sizeof(float2) * ((size_t)a.pc.twn + a.pp.twn + n2c + 1 + (size_t)kNw * lmax);
+  if (sh > 160 * 1024) return hipErrorInvalidValue;
+  const int nkg = (half + kNw - 1) / kNw;
+  const int64_t U = (int64_t)a.nparents * nkg;
+  const int64_t nblk = ((U + 7) / 8) * 8 * a.F;
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void*)theta_fft3_kernel<NMAX, RL, ROOT>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((theta_fft3_kernel<NMAX, RL, ROOT>), dim3((unsigned)nblk), dim3(64 * kNw), sh, st,
+                     a.GT, a.F, a.ntc, a.npp, a.ntp, U, nkg, a.cb, a.ce, a.child_key, a.Text,
+                     a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root, a.flip);
+  return hipGetLastError();
+}
+
+template <bool ROOT>
+hipError_t launch_theta_fft3(const wf::ThetaFftArgs& a, hipStream_t st) {
+  const int lmax = a.ntc > a.ntp ? 2 * a.ntc : 2 * a.ntp;
+  const int rl = fft2_radix(a.pp, a.pp.nstages - 1);
+  if (rl == 4) {
+    if (lmax <= 512) return launch_theta_fft3_t<512, 4, ROOT>(a, st);
+    if (lmax <= 768) return launch_theta_fft3_t<768, 4, ROOT>(a, st);
+    return launch_theta_fft3_t<1024, 4, ROOT>(a, st);
+  }
+  if (rl != 2) return hipErrorInvalidValue;  // even lengths end on radix 4 or 2
+  if (lmax <= 512) return launch_theta_fft3_t<512, 2, ROOT>(a, st);
+  if (lmax <= 768) return launch_theta_fft3_t<768, 2, ROOT>(a, st);
+  return launch_theta_fft3_t<1024, 2, ROOT>(a, st);
+}
+
 template <bool ROOT, int NT>
 hipError_t launch_theta_fft_t(const wf::ThetaFftArgs& a, int W, int slots, size_t sh,
                               hipStream_t st) {
@@ -263,6 +450,35 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
                           hipStream_t st) {
   if (lines_cf <= 0) return hipSuccess;
   const int lmax = npc > npp ? npc : npp;
+  if (pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
+  if (phi_fft_wave(npc, npp)) {
+    if (pp.nstages < 2) return hipErrorInvalidValue;
+    constexpr int NW = 8;  // rows per block: 64 B runs of the transposed writes
+    const size_t sh = sizeof(float2) * ((size_t)pc.twn + pp.twn + npc + 1 + (size_t)NW * lmax);
+    if (sh > 160 * 1024) return hipErrorInvalidValue;
+    const int64_t nblk = lines_cf * ((ntc + NW - 1) / NW);
+    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    const int rl = fft2_radix(pp, pp.nstages - 1);
+    if (rl != 2 && rl != 4) return hipErrorInvalidValue;  // even lengths end on 4 or 2
+#define WF_PHI3(N, RL)                                                                            \
+  do {                                                                                            \
+    hipError_t e = hipFuncSetAttribute((const void*)phi_fft3_kernel<N, RL>,                       \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);      \
+    if (e != hipSuccess) return e;                                                                \
+    hipLaunchKernelGGL((phi_fft3_kernel<N, RL>), dim3((unsigned)nblk), dim3(64 * NW), sh, st, X,  \
+                       GT, ntc, npc, npp, pc, pp, fac);                                           \
+    return hipGetLastError();                                                                     \
+  } while (0)
+    if (rl == 4) {
+      if (lmax <= 512) WF_PHI3(512, 4);
+      if (lmax <= 768) WF_PHI3(768, 4);
+      WF_PHI3(1024, 4);
+    }
+    if (lmax <= 512) WF_PHI3(512, 2);
+    if (lmax <= 768) WF_PHI3(768, 2);
+    WF_PHI3(1024, 2);
+#undef WF_PHI3
+  }
   if (lmax > kLineBuf || pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
   const int NT = fft_threads();
   const size_t tables = sizeof(float2) * ((size_t)pc.twn + pp.twn + npc + 1);
@@ -294,6 +510,13 @@ hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st) {
   if (a.mirror != n2p - 1 && a.mirror != n2p) return hipErrorInvalidValue;
   if ((a.out_root != nullptr) == (a.out != nullptr)) return hipErrorInvalidValue;
   const int lmax = n2c > n2p ? n2c : n2p;
+  if (theta_fft_wave(a.ntc, a.ntp)) {
+    if (a.Text && !a.text_kmajor) return hipErrorInvalidValue;
+    if (a.pp.nstages < 2) return hipErrorInvalidValue;
+    if (a.out_root) return launch_theta_fft3<true>(a, st);
+    return launch_theta_fft3<false>(a, st);
+  }
+  if (a.Text && a.text_kmajor) return hipErrorInvalidValue;
   const int NT = fft_threads();
   if (lmax > kLineBuf || n2p > NT * 16) return hipErrorInvalidValue;
   const size_t tables = sizeof(float2) * ((size_t)a.pc.twn + a.pp.twn + n2c + 1);

@@ -590,16 +590,26 @@ __global__ __launch_bounds__(256, MINB) void theta_mfma_kernel(const ThetaK P) {
 // Extended translation table: Text[o][e][k], e < 2 ntp, k < npp/2, holds the
 // factor of the parent sample fed by great-circle row e of column pair k (zero
 // where none); row 2 ntp holds the second sample fed by e = 0 (see above).
+// kmajor: the same factors stored Text[o][k][e] (e contiguous: the LDS FFT
+// theta kernel reads one great circle per wave).
 __global__ __launch_bounds__(256) void trans_ext_kernel(float2* __restrict__ Text, int ntp, int npp,
                                                         int mirror, double theta0, double hx,
-                                                        double hy, double hz, double k_rev) {
+                                                        double hy, double hz, double k_rev,
+                                                        int kmajor) {
   const int half = npp / 2, n2p = 2 * ntp;
   const int64_t per_o = (int64_t)(n2p + 1) * half;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 8 * per_o;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int o = (int)(t / per_o);
     const int64_t d = t - (int64_t)o * per_o;
-    const int e = (int)(d / half), k = (int)(d - (int64_t)e * half);
+    int e, k;
+    if (kmajor) {
+      k = (int)(d / (n2p + 1));
+      e = (int)(d - (int64_t)k * (n2p + 1));
+    } else {
+      e = (int)(d / half);
+      k = (int)(d - (int64_t)e * half);
+    }
     int row = -1, col = k;
     if (e < n2p && e < ntp) {
       row = e;
@@ -636,14 +646,21 @@ __global__ __launch_bounds__(256) void trans_child_kernel(float2* __restrict__ T
                                                           const double* __restrict__ cx,
                                                           const double* __restrict__ cy,
                                                           const double* __restrict__ cz, int nch,
-                                                          double k_rev) {
+                                                          double k_rev, int kmajor) {
   const int half = npp / 2, n2p = 2 * ntp;
   const int64_t per_c = (int64_t)(n2p + 1) * half;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < per_c * nch;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)(t / per_c);
     const int64_t d = t - (int64_t)c * per_c;
-    const int e = (int)(d / half), k = (int)(d - (int64_t)e * half);
+    int e, k;
+    if (kmajor) {
+      k = (int)(d / (n2p + 1));
+      e = (int)(d - (int64_t)k * (n2p + 1));
+    } else {
+      e = (int)(d / half);
+      k = (int)(d - (int64_t)e * half);
+    }
     int row = -1, col = k;
     if (e < n2p && e < ntp) {
       row = e;
@@ -890,25 +907,25 @@ hipError_t launch_theta_mfma(const ThetaMfmaArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
-                            const double h[3], double k0, hipStream_t st) {
+                            const double h[3], double k0, hipStream_t st, int kmajor) {
   const int64_t tot = 8LL * (2 * ntp + 1) * (npp / 2);
   int64_t blocks = (tot + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   if (blocks < 1) return hipSuccess;
   hipLaunchKernelGGL(trans_ext_kernel, dim3((unsigned)blocks), dim3(256), 0, st, Text, ntp, npp,
-                     mirror, theta0, h[0], h[1], h[2], k0 / WF_TWO_PI);
+                     mirror, theta0, h[0], h[1], h[2], k0 / WF_TWO_PI, kmajor);
   return hipGetLastError();
 }
 
 hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
                               const double* cx, const double* cy, const double* cz, int nch,
-                              double k0, hipStream_t st) {
+                              double k0, hipStream_t st, int kmajor) {
   const int64_t tot = (int64_t)nch * (2 * ntp + 1) * (npp / 2);
   int64_t blocks = (tot + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   if (blocks < 1) return hipSuccess;
   hipLaunchKernelGGL(trans_child_kernel, dim3((unsigned)blocks), dim3(256), 0, st, T, ntp, npp,
-                     mirror, theta0, cx, cy, cz, nch, k0 / WF_TWO_PI);
+                     mirror, theta0, cx, cy, cz, nch, k0 / WF_TWO_PI, kmajor);
   return hipGetLastError();
 }
 

@@ -330,17 +330,32 @@ wf_status get_fft2(wf_handle* h, int n, Fft2Plan* out) {
   e.plan.n = n;
   e.plan.nstages = 0;
   e.plan.radix_packed = 0;
+  memset(e.plan.mag_ns, 0, sizeof e.plan.mag_ns);
   int m = n;
   std::vector<int> rad;
-  while (m % 4 == 0) {
-    rad.push_back(4);
-    m /= 4;
-  }
-  for (int r : {3, 5, 2})
+  // odd radices first, then 4s, then a 2: even lengths end on radix 4 or 2
+  // (the wave-private kernels are compiled per last radix, 2 or 4); n = 4
+  // runs as two radix-2 stages (those kernels need at least two stages)
+  for (int r : {5, 3})
     while (m % r == 0) {
       rad.push_back(r);
       m /= r;
     }
+  while (m % 4 == 0 && m != 4) {
+    rad.push_back(4);
+    m /= 4;
+  }
+  if (m == 4 && rad.empty()) {
+    rad.push_back(2);
+    m = 2;
+  } else if (m == 4) {
+    rad.push_back(4);
+    m = 1;
+  }
+  if (m == 2) {
+    rad.push_back(2);
+    m = 1;
+  }
   if (m != 1 || n < 1 || (int)rad.size() > WF_FFT2_MAX_STAGES)
     return set_err(WF_ERR_UNSUPPORTED, "FFT length %d is not 5-smooth", n);
   std::vector<float2> tw;
@@ -348,6 +363,7 @@ wf_status get_fft2(wf_handle* h, int n, Fft2Plan* out) {
   for (size_t s = 0; s < rad.size(); ++s) {
     const int R = rad[s];
     e.plan.radix_packed |= (uint64_t)R << (4 * s);
+    e.plan.mag_ns[s] = fmagic((uint32_t)Ns);
     for (int k = 0; k < Ns; ++k)
       for (int r = 1; r < R; ++r) {
         const double a = -2.0 * WF_PI * (double)r * k / ((double)Ns * R);
@@ -357,6 +373,7 @@ wf_status get_fft2(wf_handle* h, int n, Fft2Plan* out) {
   }
   e.plan.nstages = (int)rad.size();
   e.plan.twn = (int)tw.size();
+  e.plan.mag_n = fmagic((uint32_t)n);
   WF_HIP(hipMalloc(&e.tw, sizeof(float2) * std::max<size_t>(tw.size(), 1)));
   if (!tw.empty())
     WF_HIP(hipMemcpy(e.tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
@@ -1176,6 +1193,19 @@ LevelPath level_path(const wf_handle* h, Grid gcg, Grid gp) {
   return p;
 }
 
+// The transition child grid -> parent grid runs the LDS FFT kernels (long
+// theta rows, or grids the MFMA kernels do not take).
+bool fft_transition(const wf_handle* h, Grid gcg, Grid gp) {
+  const LevelPath lp = level_path(h, gcg, gp);
+  return lp.fft2 && (2 * gp.nt > h->tune.agg_fft_rows || !lp.mfma);
+}
+
+// Translation-table layout of a transition: k-major for the wave-private LDS
+// FFT theta kernel.
+int table_kmajor(const wf_handle* h, Grid gcg, Grid gp) {
+  return (fft_transition(h, gcg, gp) && wf::theta_fft_wave(gcg.nt, gp.nt)) ? 1 : 0;
+}
+
 // Grids and translation tables of one wavelength group.  allow_skip: the
 // root may read the first level whose grid is no larger than the output grid
 // directly (forward solves; the adjoint keeps every level).
@@ -1215,8 +1245,9 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
                             0.5 * (h->box[5] - h->box[2]) / std::ldexp(1.0, l + 1)};
       float2* Tl = gc.Tx + gc.txoff[l];
       const double k0 = gc.k0;
+      const int kmajor = table_kmajor(h, gc.g[l + 1], gl);
       WF_TRY(timed(h, st, WF_K_TRANS, 8.0 * 8 * (2 * gl.nt + 1) * (gl.np / 2), [&] {
-        return wf::launch_trans_ext(Tl, gl.nt, gl.np, mirror, th0, hc, k0, st);
+        return wf::launch_trans_ext(Tl, gl.nt, gl.np, mirror, th0, hc, k0, st, kmajor);
       }));
     }
   }
@@ -1248,9 +1279,10 @@ wf_status prepare_group(wf_handle* h, hipStream_t st, double lam, int NT, int NP
                                       coff + 2 * nr, st));
       WF_TRY(dalloc_t(h, "trans_root", (size_t)nr * (2 * NT + 1) * (NP / 2), &gc.Troot));
       const double k0 = gc.k0;
+      const int kmajor = table_kmajor(h, gr, gc.g[0]);
       WF_TRY(timed(h, st, WF_K_TRANS, tbytes, [&] {
         return wf::launch_trans_child(gc.Troot, NT, NP, mirror, off_p, coff, coff + nr,
-                                      coff + 2 * nr, nr, k0, st);
+                                      coff + 2 * nr, nr, k0, st, kmajor);
       }));
       gc.root_from = r;
     }
@@ -1403,13 +1435,13 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
     const int mirror = mirror_of(gp.nt, off_p);
     const float2* Tl = gc.T ? gc.T + gc.toff[l] : nullptr;
     const LevelPath lpath = level_path(h, gcg, gp);
-    const bool mfma_ok = lpath.mfma, fft2_ok = lpath.fft2;
+    const bool mfma_ok = lpath.mfma;
     // compulsory bytes of this transition: children read once, parents
     // written once (the FP64 root accumulators at 16 B per sample)
     h->stats.agg_transitions += 1;
     h->stats.agg_bytes += 8.0 * planes * gcg.nt * gcg.np +
                           ((l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np);
-    if (fft2_ok && (2 * gp.nt > h->tune.agg_fft_rows || !mfma_ok)) {
+    if (fft_transition(h, gcg, gp)) {
       // large grids: LDS FFT (each sample read and written once)
       Fft2Plan f_pc, f_pp, f_tc, f_tp;
       WF_TRY(get_fft2(h, gcg.np, &f_pc));
@@ -1443,6 +1475,7 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
       ta.out = (l == 0) ? nullptr : nxt;
       ta.out_root = (l == 0) ? F0 : nullptr;
       ta.flip = transverse ? 1 : 0;
+      ta.text_kmajor = table_kmajor(h, gcg, gp);
       const double wbytes = (l == 0) ? 16.0 * F * D : 8.0 * F * (double)npar * gp.nt * gp.np;
       WF_TRY(timed(h, st, WF_K_THETA, 8.0 * planes * gcg.nt * gp.np + wbytes,
                    [&] { return wf::launch_theta_fft(ta, st); }));

@@ -23,6 +23,8 @@ struct Fft2Plan {
   uint64_t radix_packed;  // 4 bits per stage
   const float2* tw;       // device table, twn entries (forward sign)
   int twn;
+  uint32_t mag_n;                       // fmagic(n)
+  uint32_t mag_ns[WF_FFT2_MAX_STAGES];  // fmagic(Ns) of each stage
 };
 
 __host__ __device__ inline int fft2_radix(const Fft2Plan& p, int s) {
@@ -139,3 +141,225 @@ __device__ inline float2* lines(float2* a, float2* b, int nlines, int lstride, c
 }
 
 }  // namespace wffft2
+
+// ------------------------------------------------------------------------
+// Wave-private lines (lengths up to NMAX <= 1024).  One wave transforms one
+// line in its own LDS buffer; stages are in place: every lane first reads
+// and transforms all of its butterflies into registers, then all lanes
+// write.  LDS requests of one wave complete in issue order, so the phases
+// need only a compiler barrier between them, no workgroup barrier.  The
+// first forward stage reads its inputs through a functor (global loads, no
+// LDS round trip); the first inverse stage forms the resampled spectrum
+// from the forward spectrum on the fly; the last inverse stage hands its
+// outputs (e = t + r n/R for butterfly t of the lane) to a functor in
+// registers.  Inverse transforms conjugate the input and the output around
+// the forward stages.
+namespace wffft3 {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// butterflies per lane of a radix-R stage on lines up to NMAX
+template <int R, int NMAX>
+struct Im {
+  static constexpr int v = (NMAX / R + 63) / 64;
+};
+// output registers per lane of a last stage of radix RL
+template <int NMAX, int RL>
+struct Slots {
+  static constexpr int v = Im<RL, NMAX>::v * RL;
+};
+
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+template <int R, int NMAX, class Src>
+__device__ __forceinline__ void first_t(float2* buf, int n, Src& src, int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = n / R;
+  float2 v[IM][R];
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = src(t + r * m);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+      wffft2::dft<R>(v[i]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[t * R + r] = v[i][r];
+    }
+  }
+  wave_sync();
+}
+
+template <int R, int NMAX>
+__device__ __forceinline__ void mid_t(float2* buf, int n, int Ns, uint32_t mag, const float2* tw,
+                                      int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = n / R;
+  float2 v[IM][R];
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+      const int k = t - (int)fdiv((uint32_t)t, mag) * Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[t + r * m];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[k * (R - 1) + r - 1]);
+      wffft2::dft<R>(v[i]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+      const int h = (int)fdiv((uint32_t)t, mag), k = t - h * Ns;
+      float2* d = buf + h * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r * Ns] = v[i][r];
+    }
+  }
+  wave_sync();
+}
+
+// first inverse stage: v = conj(Y[t + r m]), Y[q] = sum over f = q (mod np)
+// in the band of the nc-point spectrum X (in buf) of fac[f - fmin] X[f mod nc]
+template <int R, int NMAX>
+__device__ __forceinline__ void rs_first_t(float2* buf, int np, int nc, uint32_t mag_np,
+                                           const float2* fac, int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = np / R;
+  const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
+  float2 v[IM][R];
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = t + r * m;
+        const int f0 = q - np * (int)fdiv((uint32_t)(q - fmin), mag_np);
+        float2 acc = make_float2(0.f, 0.f);
+        for (int f = f0; f <= fmax; f += np) {
+          const int bin = f < 0 ? f + nc : f;
+          acc = cadd(acc, cmul(buf[bin], fac[f - fmin]));
+        }
+        v[i][r] = conjf2(acc);
+      }
+      wffft2::dft<R>(v[i]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[t * R + r] = v[i][r];
+    }
+  }
+  wave_sync();
+}
+
+// last stage (Ns R = n): outputs e = t + r m, conjugated (inverse), to sink
+template <int R, int NMAX, class Sink>
+__device__ __forceinline__ void last_t(const float2* buf, int n, const float2* tw, Sink& sink,
+                                       int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = n / R;
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+      float2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = buf[t + r * m];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[t * (R - 1) + r - 1]);
+      wffft2::dft<R>(v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) sink(i * R + r, t + r * m, conjf2(v[r]));
+    }
+  }
+  wave_sync();
+}
+
+template <int R, int NMAX, class F>
+__device__ __forceinline__ void outputs_t(int n, F& f, int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = n / R;
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) f(i * R + r, t + r * m);
+    }
+  }
+}
+
+#define WF_FFT3_RADIX(R_, CALL) \
+  switch (R_) {                 \
+    case 4: CALL(4); break;     \
+    case 2: CALL(2); break;     \
+    case 3: CALL(3); break;     \
+    default: CALL(5); break;    \
+  }
+
+// Forward FFT of the line src(j), j < p.n, into buf (natural order).
+template <int NMAX, class Src>
+__device__ __forceinline__ void forward(const Fft2Plan& p, float2* buf, const float2* tw, Src& src,
+                                        int lane) {
+  const int R0 = fft2_radix(p, 0);
+#define WF_C(R) first_t<R, NMAX>(buf, p.n, src, lane)
+  WF_FFT3_RADIX(R0, WF_C)
+#undef WF_C
+  int Ns = R0, off = R0 - 1;
+  for (int s = 1; s < p.nstages; ++s) {
+    const int R = fft2_radix(p, s);
+#define WF_C(R) mid_t<R, NMAX>(buf, p.n, Ns, p.mag_ns[s], tw + off, lane)
+    WF_FFT3_RADIX(R, WF_C)
+#undef WF_C
+    off += Ns * (R - 1);
+    Ns *= R;
+  }
+}
+
+// Spectrum of length nc in buf -> resample (fac) -> inverse FFT of length
+// p.n (p.nstages >= 2, last radix RL) -> sink(slot, e, value).
+template <int NMAX, int RL, class Sink>
+__device__ __forceinline__ void inverse(const Fft2Plan& p, float2* buf, const float2* tw,
+                                        const float2* fac, int nc, Sink& sink, int lane) {
+  const int R0 = fft2_radix(p, 0);
+#define WF_C(R) rs_first_t<R, NMAX>(buf, p.n, nc, p.mag_n, fac, lane)
+  WF_FFT3_RADIX(R0, WF_C)
+#undef WF_C
+  int Ns = R0, off = R0 - 1;
+  for (int s = 1; s < p.nstages - 1; ++s) {
+    const int R = fft2_radix(p, s);
+#define WF_C(R) mid_t<R, NMAX>(buf, p.n, Ns, p.mag_ns[s], tw + off, lane)
+    WF_FFT3_RADIX(R, WF_C)
+#undef WF_C
+    off += Ns * (R - 1);
+    Ns *= R;
+  }
+  last_t<RL, NMAX>(buf, p.n, tw + off, sink, lane);
+}
+
+// f(slot, e) over the outputs the lane holds after inverse()
+template <int NMAX, int RL, class F>
+__device__ __forceinline__ void for_outputs(const Fft2Plan& p, F& f, int lane) {
+  outputs_t<RL, NMAX>(p.n, f, lane);
+}
+
+}  // namespace wffft3

@@ -119,8 +119,10 @@ struct AggFusedArgs {
 bool agg_fused_fits(int ntc, int npc, int ntp, int npp);
 hipError_t launch_agg_fused_mfma(const AggFusedArgs& a, hipStream_t st);
 
+// kmajor = 1: the same tables stored [8][npp/2][2 ntp + 1] (great circle k
+// contiguous), the layout of the wave-private LDS FFT theta kernel
 hipError_t launch_trans_ext(float2* Text, int ntp, int npp, int mirror, double theta0,
-                            const double h[3], double k0, hipStream_t st);
+                            const double h[3], double k0, hipStream_t st, int kmajor = 0);
 // heightfield -> 7 SoA arrays (wf_geometry.hip); *first_bad receives the
 // smallest i * numel + j of a sample with a zero vertex (INT_MAX if none)
 hipError_t launch_heightfield_points(const float* height, int numel, int numelvert, float period,
@@ -149,7 +151,7 @@ hipError_t launch_rough2d(const double* x, const double* y, int n, const Rough2d
 // (hx/hy/hz[l]: the child-centre offset magnitudes of the level-l transition)
 hipError_t launch_trans_child(float2* T, int ntp, int npp, int mirror, double theta0,
                               const double* cx, const double* cy, const double* cz, int nch,
-                              double k0, hipStream_t st);
+                              double k0, hipStream_t st, int kmajor = 0);
 hipError_t launch_child_offsets(const uint32_t* keys, int n, int r, const double* hx,
                                 const double* hy, const double* hz, double* cx, double* cy,
                                 double* cz, hipStream_t st);
@@ -175,8 +177,20 @@ struct ThetaFftArgs {
   double2* out_root;
   int flip = 0;        // transverse components: the second half of every great
                        // circle is negated on input and on output
+  int text_kmajor = 0; // Text stored [o][npp/2][2 ntp + 1] (required by the
+                       // wave-private kernel, theta_fft_wave)
 };
 hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st);
+
+// The LDS FFT kernels keep lines of up to 1024 samples wave-private (one wave
+// per line, no workgroup barriers between stages); longer lines run the
+// block-wide kernels.  The theta choice fixes the translation-table layout.
+inline bool phi_fft_wave(int npc, int npp) {
+  return (npc > npp ? npc : npp) <= 1024 && npp >= 4;
+}
+inline bool theta_fft_wave(int ntc, int ntp) {
+  return (ntc > ntp ? 2 * ntc : 2 * ntp) <= 1024 && ntp >= 2;
+}
 
 // Transverse components of the far field from the six radiated fields of each
 // solve: F [node][6 S][nt][np] (E x, y, z, H x, y, z; level grid theta_j =
