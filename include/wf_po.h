@@ -283,7 +283,8 @@ typedef struct wf_tune {
   int32_t agg_kernel;     /* level transitions: 0 = dense split-BF16 MFMA interpolation for
                              levels whose extended parent grid (2 Nt) has at most agg_fft_rows
                              rows and the LDS FFT above; 1 = the original LDS FFT kernels */
-  int32_t agg_fft_rows;   /* see agg_kernel (default 600) */
+  int32_t agg_fft_rows;   /* see agg_kernel (default 600; wf_autotune lowers it while the
+                             solves get faster) */
   int32_t calibrated;     /* 1 after wf_autotune (or a wf_set_tune of a calibrated tune) */
   int32_t agg_fused;      /* internal MFMA level transitions whose grids fit run as one
                              kernel, the phi output kept in LDS: 2 (default: phi matrix and
@@ -306,7 +307,9 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t);
 /* Times solves of `inc` (its first wavelength group, at most 8 solves) on the
  * current geometry at up to three candidate depths <= the built depth, fits
  * the cost model, picks the leaf kernel and the level-transition kernels
- * (agg_fused 2 or 0, theta_tile 0 or 2) by measured time, picks the level
+ * (agg_fused 2 or 0, theta_tile 0 or 2, then agg_fft_rows: the largest level
+ * grids move to the LDS FFT kernels one at a time, at most three, while the
+ * solves get faster) by measured time, picks the level
  * sampling constant tune.beta: the smallest of 3 and 2 below opts.beta whose
  * intensities stay within 1e-4 relative L2 (every solve of the group) of
  * the opts.beta solve, and returns the measured fastest depth in *best_depth

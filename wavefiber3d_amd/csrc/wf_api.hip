@@ -1395,17 +1395,25 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
   const int F = transverse ? 2 * S : F6;  // fields per level pattern
   *fps = transverse ? 2 : 6;
   float2* cur = (depth % 2) ? tb.patA : tb.patB;
-  const wf::LeafArgs la = leaf_args(h, gc, C, F6, cur);
+  wf::LeafArgs la = leaf_args(h, gc, C, F6, cur);
   const double dleaf = (double)gc.g[depth].nt * gc.g[depth].np;
+  const double z0 = std::sqrt((4e-7 * WF_PI) / 8.8541878e-12);  // as the epilogue
+  const bool bf16 = h->opts.precision == WF_PREC_BF16X3;
+  // the quad-symmetric leaf kernel projects onto the transverse components in
+  // its epilogue; other leaf kernels write the six fields for the projection
+  // kernel below
+  const bool leaf_proj = transverse && h->opts.precision != WF_PREC_FP32_VALU &&
+                         wf::leaf_fuses_transverse(la, bf16);
+  la.transverse = leaf_proj ? 1 : 0;
+  la.z0 = (float)z0;
   WF_TRY(timed(h, st, WF_K_LEAF, 8.0 * dleaf * n * F6, [&] {
     if (h->opts.precision == WF_PREC_FP32_VALU) return wf::launch_leaf(la, st);
-    return wf::launch_leaf_mfma(la, h->opts.precision == WF_PREC_BF16X3, st);
+    return wf::launch_leaf_mfma(la, bf16, st);
   }));
   h->stats.leaf_pairs += (int64_t)(dleaf * n);
-  if (transverse) {
+  if (transverse && !leaf_proj) {
     float2* nxt = (cur == tb.patA) ? tb.patB : tb.patA;
     const int nleaf = h->levels[depth].count;
-    const double z0 = std::sqrt((4e-7 * WF_PI) / 8.8541878e-12);  // as the epilogue
     WF_TRY(timed(h, st, WF_K_TRANS, 8.0 * (F6 + F) * nleaf * dleaf, [&] {
       return wf::launch_project_transverse(cur, nxt, nleaf, S, gc.g[depth].nt, gc.g[depth].np,
                                            theta_origin(depth, gc.g[depth].nt), z0, st);
@@ -2452,9 +2460,10 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
     h->tune.leaf_quad = quad ? 1 : 0;
     h->tune.chunk_solves = quad ? 4 : 8;
   }
-  // level-transition kernels at the model's best depth (dense MFMA levels
-  // only): the staged fused form or two passes, and the two-pass theta tile
-  // of long rows; every candidate gives the same result to FP32 rounding
+  // level-transition kernels at the model's best depth: the staged fused
+  // form or two passes and the two-pass theta tile of long rows (dense MFMA
+  // levels), then how many of the largest levels run the LDS FFT kernels;
+  // every candidate gives the same result to FP32 rounding
   if (h->tune.agg_kernel == 0) {
     static const int32_t cand[3][2] = {{2, 0}, {0, 0}, {2, 2}};  // agg_fused, theta_tile
     double tbest = 0;
@@ -2475,6 +2484,31 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
     }
     h->tune.agg_fused = cand[ibest][0];
     h->tune.theta_tile = cand[ibest][1];
+    // the extended-row threshold above which transitions take the LDS FFT
+    // kernels instead of the MFMA ones: from the current setting down, each
+    // candidate moving the next smaller level grid to the FFT; kept while
+    // measured faster
+    std::vector<int> rows2;  // extended rows of the parent grids of the transitions
+    for (int l = 0; l < rank[0].second; ++l) {
+      const int r2 = 2 * ((l == 0) ? NT : level_grid(h, l, lam_s).nt);
+      if (r2 <= h->tune.agg_fft_rows) rows2.push_back(r2);
+    }
+    std::sort(rows2.rbegin(), rows2.rend());
+    rows2.erase(std::unique(rows2.begin(), rows2.end()), rows2.end());
+    int32_t best_rows = h->tune.agg_fft_rows;
+    for (size_t i = 0; i < rows2.size() && i < 3; ++i) {
+      h->tune.agg_fft_rows = rows2[i] - 1;
+      Run r;
+      e = run(rank[0].second, r);
+      if (e != WF_OK) {
+        h->tune = saved;
+        return e;
+      }
+      if (!(r.total < tbest)) break;
+      tbest = r.total;
+      best_rows = rows2[i] - 1;
+    }
+    h->tune.agg_fft_rows = best_rows;
   }
   std::vector<Run> runs;
   for (const auto& c : rank) {

@@ -227,8 +227,15 @@ struct LeafArgs {
   float2* out;
   int precision;
   int quad;  // 1: quad-symmetric BF16x3 kernel where eligible (np even, F <= 24)
+  // 1: F = 6 S fields (S solves' E x, y, z, H x, y, z) are summed and the
+  // pattern receives the two transverse components of each solve instead,
+  // out [leaf][2 S][nt][np] as launch_project_transverse forms them
+  // (leaf_fuses_transverse tells which launches can)
+  int transverse = 0;
+  float z0 = 0.f;
 };
 hipError_t launch_leaf(const LeafArgs& a, hipStream_t st);
+bool leaf_fuses_transverse(const LeafArgs& a, bool bf16);
 // MFMA variants (F <= 48 fields per launch): split BF16 x3 or FP32 MFMA.
 hipError_t launch_leaf_mfma(const LeafArgs& a, bool bf16, hipStream_t st);
 

@@ -105,7 +105,7 @@ namespace wf {
 hipError_t launch_leaf(const LeafArgs& a, hipStream_t st) {
   const int D = a.nt * a.np;
   if (a.nleaves <= 0) return hipSuccess;
-  if (a.F % 6 != 0) return hipErrorInvalidValue;
+  if (a.F % 6 != 0 || a.transverse) return hipErrorInvalidValue;
   const unsigned tiles = (unsigned)((D + kLeafThreads - 1) / kLeafThreads);
   const int FB = (a.F % 12 == 0) ? 12 : 6;
   const int64_t nblk = (int64_t)tiles * (a.F / FB) * a.nleaves;

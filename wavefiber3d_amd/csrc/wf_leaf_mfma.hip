@@ -62,6 +62,14 @@ __device__ inline void leaf_dir(int d, int nt, int np, float s[3]) {
 
 __device__ inline float rev_reduce(float r) { return r - rintf(r); }
 
+// lanes of one wave exchange values through LDS: the wave's LDS requests
+// complete in issue order, so a compiler barrier orders the phases
+__device__ __forceinline__ void leaf_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Stage the scaled local coordinates of one chunk of points (zero-padded).
 __device__ inline void stage_points(const wf::LeafArgs& a, int base, int cnt, int npts,
                                     const double c[3], double k_rev, float4* spt) {
@@ -332,6 +340,59 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
       }
     }
   }
+  if (a.transverse) {
+    // Two transverse components per solve (F = 6 S fields): lane (col, g)
+    // stages its fields f = 8 m + 2 g + h of direction r in the wave's LDS
+    // slice [f][col], then lane (col, s = g) projects solve s onto the unit
+    // vectors t (theta, latitude) and p (phi) of the direction:
+    //   a = t.E - z0 p.H,  b = p.E + z0 t.H.
+    // The quadruple's directions flip sin(theta) (r2, r3) and cos/sin(phi)
+    // (r1, r3).
+    __syncthreads();  // every wave done with the staged points and currents
+    float2* ep = reinterpret_cast<float2*>(smem4) + wave * (MT * 8 * 16);
+    const int S = a.F / 6;
+    float sth = 0.f, cth = 0.f, sph = 0.f, cph = 0.f;
+    if (qvalid) {
+      double st, ct, sp, cp;
+      sincos(-WF_PI / 2 + WF_PI * (j + 0.5) / a.nt, &st, &ct);
+      sincos(WF_TWO_PI * k / a.np, &sp, &cp);
+      sth = (float)st;
+      cth = (float)ct;
+      sph = (float)sp;
+      cph = (float)cp;
+    }
+    const float z0 = a.z0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh)
+          ep[(m * 8 + 2 * g + hh) * 16 + col] =
+              make_float2(acc[r][m][2 * hh], acc[r][m][2 * hh + 1]);
+      leaf_wave_sync();
+      if (qvalid && g < S && (r < 2 || mirror_row)) {
+        const float sr = (r & 2) ? -sth : sth;
+        const float pc = (r & 1) ? -cph : cph, ps = (r & 1) ? -sph : sph;
+        const float t[3] = {-sr * pc, -sr * ps, cth}, p[3] = {-ps, pc, 0.f};
+        const float2* fs = ep + (6 * g) * 16 + col;
+        float2 av = make_float2(0.f, 0.f), bv = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float2 e = fs[c * 16], h = fs[(3 + c) * 16];
+          av.x += t[c] * e.x - z0 * p[c] * h.x;
+          av.y += t[c] * e.y - z0 * p[c] * h.y;
+          bv.x += p[c] * e.x + z0 * t[c] * h.x;
+          bv.y += p[c] * e.y + z0 * t[c] * h.y;
+        }
+        float2* o = a.out + ((int64_t)leaf * 2 * S + 2 * g) * D + dd[r];
+        o[0] = av;
+        o[D] = bv;
+      }
+      leaf_wave_sync();
+    }
+    return;
+  }
   if (!qvalid) return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -444,13 +505,19 @@ hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
     // direction rows one block's 64 consecutive quads can touch
     const int nph = a.np / 2;
     const int RB = 63 / nph + 2;
-    const size_t sh = (size_t)KS * 16 * sizeof(float4) + (size_t)KS * per_ks +
-                      (size_t)RB * KS * 16 * sizeof(float2) + (size_t)RB * sizeof(float);
+    size_t sh = (size_t)KS * 16 * sizeof(float4) + (size_t)KS * per_ks +
+                (size_t)RB * KS * 16 * sizeof(float2) + (size_t)RB * sizeof(float);
+    if (a.transverse) {  // the epilogue's per-wave [field][16 directions] slices
+      if (a.F % 6 != 0) return hipErrorInvalidValue;
+      const size_t ep = (size_t)4 * MT * 8 * 16 * sizeof(float2);
+      sh = sh > ep ? sh : ep;
+    }
     if (sh > 64 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL((leaf_bf16x3_quad_kernel<MT>), dim3((unsigned)nblk), dim3(kThreads), sh, st,
                        a, KS, RB);
     return hipGetLastError();
   }
+  if (a.transverse) return hipErrorInvalidValue;  // the quad kernel only
   const int dtiles = (D + 15) / 16;
   const int nsuper = (dtiles + 4 * R - 1) / (4 * R);
   const int64_t nblk = (int64_t)nsuper * a.nleaves;
@@ -469,6 +536,11 @@ hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
 }  // namespace
 
 namespace wf {
+
+bool leaf_fuses_transverse(const LeafArgs& a, bool bf16) {
+  return bf16 && a.quad && a.F % 6 == 0 && (a.F + 7) / 8 <= 3 && (a.np % 2) == 0 &&
+         a.f0 == 0 && a.F == a.Ftot;
+}
 
 // F complex fields handled per launch: up to 48 (MT = 6 tiles of 8 fields).
 hipError_t launch_leaf_mfma(const LeafArgs& a, bool bf16, hipStream_t st) {
