@@ -250,10 +250,12 @@ __global__ __launch_bounds__(512) void phi_fft3_kernel(const float2* __restrict_
     wffft3::for_outputs<NMAX, RL>(pp, put, lane);
   }
   __syncthreads();
+  // nw is a power of two (launcher): thread t -> (q = t / nw, row w = t % nw)
   float2* dst = GT + cf * (int64_t)npp * ntc + r0;
-  for (int t = threadIdx.x; t < nr * npp; t += blockDim.x) {
-    const int q = t / nr, w = t - q * nr;
-    dst[(int64_t)q * ntc + w] = lines[w * lmax + q];
+  const int sh = __builtin_ctz((unsigned)nw);
+  for (int t = threadIdx.x; t < (npp << sh); t += blockDim.x) {
+    const int q = t >> sh, w = t & (nw - 1);
+    if (w < nr) dst[(int64_t)q * ntc + w] = lines[w * lmax + q];
   }
 }
 
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(512) void phi_fft3_kernel(const float2* __restrict_
 // fields.
 constexpr int kNw = 4;
 template <int NMAX, int RL, bool ROOT>
-__global__ __launch_bounds__(64 * kNw) void theta_fft3_kernel(
+__global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
     const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, int64_t U, int nkg,
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
     const uint32_t* __restrict__ child_key, const float2* __restrict__ Text, int text_per_child,
@@ -394,7 +396,12 @@ hipError_t launch_theta_fft3(const wf::ThetaFftArgs& a, hipStream_t st) {
     if (lmax <= 768) return launch_theta_fft3_t<768, 4, ROOT>(a, st);
     return launch_theta_fft3_t<1024, 4, ROOT>(a, st);
   }
-  if (rl != 2) return hipErrorInvalidValue;  // even lengths end on radix 4 or 2
+  if (rl == 8) {
+    if (lmax <= 512) return launch_theta_fft3_t<512, 8, ROOT>(a, st);
+    if (lmax <= 768) return launch_theta_fft3_t<768, 8, ROOT>(a, st);
+    return launch_theta_fft3_t<1024, 8, ROOT>(a, st);
+  }
+  if (rl != 2) return hipErrorInvalidValue;  // even lengths end on radix 8, 4 or 2
   if (lmax <= 512) return launch_theta_fft3_t<512, 2, ROOT>(a, st);
   if (lmax <= 768) return launch_theta_fft3_t<768, 2, ROOT>(a, st);
   return launch_theta_fft3_t<1024, 2, ROOT>(a, st);
@@ -459,7 +466,7 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
     const int64_t nblk = lines_cf * ((ntc + NW - 1) / NW);
     if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
     const int rl = fft2_radix(pp, pp.nstages - 1);
-    if (rl != 2 && rl != 4) return hipErrorInvalidValue;  // even lengths end on 4 or 2
+    if (rl != 2 && rl != 4 && rl != 8) return hipErrorInvalidValue;  // even lengths end on 8, 4, 2
 #define WF_PHI3(N, RL)                                                                            \
   do {                                                                                            \
     hipError_t e = hipFuncSetAttribute((const void*)phi_fft3_kernel<N, RL>,                       \
@@ -473,6 +480,11 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
       if (lmax <= 512) WF_PHI3(512, 4);
       if (lmax <= 768) WF_PHI3(768, 4);
       WF_PHI3(1024, 4);
+    }
+    if (rl == 8) {
+      if (lmax <= 512) WF_PHI3(512, 8);
+      if (lmax <= 768) WF_PHI3(768, 8);
+      WF_PHI3(1024, 8);
     }
     if (lmax <= 512) WF_PHI3(512, 2);
     if (lmax <= 768) WF_PHI3(768, 2);

@@ -333,28 +333,33 @@ wf_status get_fft2(wf_handle* h, int n, Fft2Plan* out) {
   memset(e.plan.mag_ns, 0, sizeof e.plan.mag_ns);
   int m = n;
   std::vector<int> rad;
-  // odd radices first, then 4s, then a 2: even lengths end on radix 4 or 2
-  // (the wave-private kernels are compiled per last radix, 2 or 4); n = 4
-  // runs as two radix-2 stages (those kernels need at least two stages)
+  // odd radices first, then the power of two 2^a: 4s for even a, for odd a
+  // one 8 (a >= 3 and n <= 512) or one 2 followed by 4s.  Even lengths end on
+  // radix 8, 4 or 2
+  // (the wave-private kernels are compiled per last radix); n = 4 runs as two
+  // radix-2 stages (those kernels need at least two stages).
   for (int r : {5, 3})
     while (m % r == 0) {
       rad.push_back(r);
       m /= r;
     }
-  while (m % 4 == 0 && m != 4) {
-    rad.push_back(4);
-    m /= 4;
+  int a2 = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++a2;
   }
-  if (m == 4 && rad.empty()) {
+  if (n == 4 || n == 8) {
+    rad.push_back(n / 2 == 2 ? 2 : 4);
     rad.push_back(2);
-    m = 2;
-  } else if (m == 4) {
-    rad.push_back(4);
-    m = 1;
-  }
-  if (m == 2) {
-    rad.push_back(2);
-    m = 1;
+  } else {
+    if (a2 % 2 == 1 && a2 >= 3 && n <= 512) {  // the wave kernels' radix-8 bound
+      rad.push_back(8);
+      a2 -= 3;
+    } else if (a2 % 2 == 1) {
+      rad.push_back(2);
+      a2 -= 1;
+    }
+    for (; a2 >= 2; a2 -= 2) rad.push_back(4);
   }
   if (m != 1 || n < 1 || (int)rad.size() > WF_FFT2_MAX_STAGES)
     return set_err(WF_ERR_UNSUPPORTED, "FFT length %d is not 5-smooth", n);

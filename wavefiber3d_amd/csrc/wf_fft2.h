@@ -1,5 +1,5 @@
 // wf_fft2.h -- batched mixed-radix Stockham FFT on LDS lines for the large
-// level transitions (radices 4, 2, 3, 5; lengths up to 4096).
+// level transitions (radices 8, 4, 2, 3, 5; lengths up to 4096).
 //
 // Per stage of radix R with Ns = product of the radices already applied,
 // butterfly b of a line (b < n/R, k = b mod Ns, h = b div Ns) reads
@@ -57,6 +57,25 @@ __device__ inline void dft(float2* v) {
     v[2] = csub(a0, b0);
     v[1] = cadd(a1, b1r);
     v[3] = csub(a1, b1r);
+  } else if constexpr (R == 8) {
+    // X[2k] = DFT4(x[n] + x[n+4]), X[2k+1] = DFT4((x[n] - x[n+4]) w8^n)
+    const float h = 0.70710678118654752440f;  // 1/sqrt(2)
+    float2 a[4], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      a[n] = cadd(v[n], v[n + 4]);
+      b[n] = csub(v[n], v[n + 4]);
+    }
+    b[1] = make_float2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));   // (1 - i)/sqrt2
+    b[2] = make_float2(b[2].y, -b[2].x);                                 // -i
+    b[3] = make_float2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));  // (-1 - i)/sqrt2
+    dft<4>(a);
+    dft<4>(b);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[2 * k] = a[k];
+      v[2 * k + 1] = b[k];
+    }
   } else if constexpr (R == 3) {
     // c = cos(2 pi/3) = -1/2, s = sin(2 pi/3) = sqrt(3)/2
     const float s3 = 0.86602540378443864676f;
@@ -126,6 +145,7 @@ __device__ inline float2* lines(float2* a, float2* b, int nlines, int lstride, c
     const int R = fft2_radix(p, s);
     switch (R) {
       case 4: stage<4>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
+      case 8: stage<8>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
       case 2: stage<2>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
       case 3: stage<3>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
       default: stage<5>(a, b, p.n, nlines, lstride, Ns, tw + off, inverse); break;
@@ -162,10 +182,11 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// butterflies per lane of a radix-R stage on lines up to NMAX
+// butterflies per lane of a radix-R stage on lines up to NMAX (plans use
+// radix 8 only on lines up to 512: one butterfly per lane)
 template <int R, int NMAX>
 struct Im {
-  static constexpr int v = (NMAX / R + 63) / 64;
+  static constexpr int v = R == 8 ? 1 : (NMAX / R + 63) / 64;
 };
 // output registers per lane of a last stage of radix RL
 template <int NMAX, int RL>
@@ -241,22 +262,45 @@ __device__ __forceinline__ void rs_first_t(float2* buf, int np, int nc, uint32_t
   const int m = np / R;
   const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
   float2 v[IM][R];
+  if (np > nc) {
+    // upsampling: at most one band frequency per output bin, f = q (q <=
+    // fmax) or f = q - np (q - np >= fmin); np = nc folds the two halves of
+    // an even length's Nyquist bin into one and takes the general path
 #pragma unroll
-  for (int i = 0; i < IM; ++i) {
-    const int t = lane + 64 * i;
-    if (t < m) {
+    for (int i = 0; i < IM; ++i) {
+      const int t = lane + 64 * i;
+      if (t < m) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int q = t + r * m;
-        const int f0 = q - np * (int)fdiv((uint32_t)(q - fmin), mag_np);
-        float2 acc = make_float2(0.f, 0.f);
-        for (int f = f0; f <= fmax; f += np) {
+        for (int r = 0; r < R; ++r) {
+          const int q = t + r * m;
+          const bool lo = q <= fmax, hi = q - np >= fmin;
+          const int f = lo ? q : q - np;
           const int bin = f < 0 ? f + nc : f;
-          acc = cadd(acc, cmul(buf[bin], fac[f - fmin]));
+          float2 x = make_float2(0.f, 0.f);
+          if (lo || hi) x = cmul(buf[bin], fac[f - fmin]);
+          v[i][r] = conjf2(x);
         }
-        v[i][r] = conjf2(acc);
+        wffft2::dft<R>(v[i]);
       }
-      wffft2::dft<R>(v[i]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+      const int t = lane + 64 * i;
+      if (t < m) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int q = t + r * m;
+          const int f0 = q - np * (int)fdiv((uint32_t)(q - fmin), mag_np);
+          float2 acc = make_float2(0.f, 0.f);
+          for (int f = f0; f <= fmax; f += np) {
+            const int bin = f < 0 ? f + nc : f;
+            acc = cadd(acc, cmul(buf[bin], fac[f - fmin]));
+          }
+          v[i][r] = conjf2(acc);
+        }
+        wffft2::dft<R>(v[i]);
+      }
     }
   }
   wave_sync();
@@ -311,6 +355,7 @@ __device__ __forceinline__ void outputs_t(int n, F& f, int lane) {
 #define WF_FFT3_RADIX(R_, CALL) \
   switch (R_) {                 \
     case 4: CALL(4); break;     \
+    case 8: CALL(8); break;     \
     case 2: CALL(2); break;     \
     case 3: CALL(3); break;     \
     default: CALL(5); break;    \
