@@ -214,12 +214,12 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
 }
 
 // ---------------------------------------------------------------- wave lines --
-// phi: block = nw waves on nw consecutive theta rows of one (child, field)
-// plane; each wave transforms its row (FFT(npc), resample, inverse FFT(npp))
+// phi: block = nw (8 or 16) waves on nw consecutive theta rows of one
+// (child, field) plane; each wave transforms its row (FFT(npc), resample, inverse FFT(npp))
 // and leaves the npp outputs in its LDS line; the block then writes
 // GT[cf][q][r0 .. r0 + nw) with the rows contiguous.
 template <int NMAX, int RL>
-__global__ __launch_bounds__(512) void phi_fft3_kernel(const float2* __restrict__ X,
+__global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict__ X,
                                                        float2* __restrict__ GT, int ntc, int npc,
                                                        int npp, Fft2Plan pc, Fft2Plan pp,
                                                        const float2* __restrict__ fac_g) {
@@ -460,8 +460,11 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
   if (pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
   if (phi_fft_wave(npc, npp)) {
     if (pp.nstages < 2) return hipErrorInvalidValue;
-    constexpr int NW = 8;  // rows per block: 64 B runs of the transposed writes
-    const size_t sh = sizeof(float2) * ((size_t)pc.twn + pp.twn + npc + 1 + (size_t)NW * lmax);
+    // rows per block: 16 (128 B runs of the transposed writes, full cache
+    // lines) where the LDS fits two blocks per CU, else 8
+    const size_t tab = (size_t)pc.twn + pp.twn + npc + 1;
+    const int NW = (ntc >= 16 && sizeof(float2) * (tab + 16 * (size_t)lmax) <= 80 * 1024) ? 16 : 8;
+    const size_t sh = sizeof(float2) * (tab + (size_t)NW * lmax);
     if (sh > 160 * 1024) return hipErrorInvalidValue;
     const int64_t nblk = lines_cf * ((ntc + NW - 1) / NW);
     if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;

@@ -24,6 +24,8 @@
 //                  k = l/16; D as above.
 // Phases: local coordinates scaled to revolutions, reduced by rint, then the
 // hardware sin/cos (inputs in revolutions).
+#include <cmath>
+
 #include "wf_common.h"
 #include "wf_interp.h"
 
@@ -502,6 +504,11 @@ hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
     const size_t per_ks = (size_t)MT * 128 * sizeof(bf16x8);
     int KS = (int)((48 * 1024) / per_ks);
     KS = KS < 1 ? 1 : (KS > 8 ? 8 : KS);
+    // staged points per chunk: about 1.25x the mean leaf, so small leaves do
+    // not reserve LDS (and with it blocks per CU) for points they never have
+    const double mean = a.nleaves > 0 ? (double)a.n / a.nleaves : 0.0;
+    const int ks_mean = (int)std::ceil(1.25 * mean / 16.0);
+    if (ks_mean >= 2 && ks_mean < KS) KS = ks_mean;
     // direction rows one block's 64 consecutive quads can touch
     const int nph = a.np / 2;
     const int RB = 63 / nph + 2;
