@@ -64,10 +64,11 @@ def kernel_table(stats_csv: str, trace_csv: str | None = None, top: int = 20) ->
 
 
 def group_average_us(stats_csv: str, substr: str) -> tuple[int, float]:
-    """(calls, average duration in us) over all kernels whose name contains substr."""
+    """(calls, average duration in us) over all kernels whose name contains substr
+    (or any of its "|"-separated alternatives)."""
     calls, ns = 0, 0
     for r in csv.DictReader(open(stats_csv)):
-        if substr in short_name(r["Name"]):
+        if _matches(short_name(r["Name"]), substr):
             calls += int(r["Calls"])
             ns += int(r["TotalDurationNs"])
     return calls, (ns / calls / 1e3 if calls else float("nan"))
