@@ -266,8 +266,8 @@ wf_status wf_get_stats(wf_handle* h, wf_stats* out);
  *   + agg_s_per_byte * sum_{l<d} 48 (nodes_{l+1} Nt_{l+1} Np_{l+1} + nodes_l Nt_l Np_l)
  *   + level_s * d
  * per solve.  wf_autotune fits the three coefficients from timed solves of
- * the current geometry at the model's best candidate depths, picks the faster
- * of (quad-symmetric BF16x3 leaf kernel, 4 solves per pass) and (plain
+ * the current geometry at the model's best candidate depths, picks the fastest
+ * of (quad-symmetric BF16x3 leaf kernel, 4 or 8 solves per pass) and (plain
  * kernel, 8 solves per pass), and stores the result in the handle;
  * wf_get_tune/wf_set_tune move a calibration between handles.  Later solves
  * of the same WF_DEPTH_AUTO geometry at the tuned sampling wavelength use the
@@ -277,8 +277,8 @@ typedef struct wf_tune {
   double leaf_s_per_pair; /* leaf sum, per (point, leaf-grid direction, solve) */
   double agg_s_per_byte;  /* aggregation, per compulsory HBM byte (complex64 patterns) */
   double level_s;         /* fixed cost per level transition and solve */
-  int32_t leaf_quad;      /* 1: quad-symmetric BF16x3 leaf kernel where eligible (<= 4
-                             solves per tree pass); 0: plain */
+  int32_t leaf_quad;      /* 1: quad-symmetric BF16x3 leaf kernel where eligible (even
+                             leaf-grid n_phi); 0: plain */
   int32_t chunk_solves;   /* solves per tree pass (1..8); opts.field_chunk overrides */
   int32_t agg_kernel;     /* level transitions: 0 = dense split-BF16 MFMA interpolation for
                              levels whose extended parent grid (2 Nt) has at most agg_fft_rows

@@ -216,6 +216,25 @@ def test_tree_precision_modes(precision):
         assert rel(I[k], I[2]) < 1e-6  # identical incidences in other field slots
 
 
+@pytest.mark.parametrize("chunk", [4, 8])
+def test_quad_leaf_solves_per_pass(chunk):
+    """Quad-symmetric leaf kernel with 4 or 8 solves per tree pass (8: 48 fields
+    in 6 MFMA tiles; its epilogue projects two solves per lane group onto the
+    transverse components): every solve within 1e-4 of the FP64 oracle."""
+    s = wf.fibonacci_sphere(8000, 1.5e-6)
+    incs = [wf.Incidence(0.3 * i - 1.0, 0.7 * i, 0.45e-6, "ME"[i % 2], 1.55 + 0.05j)
+            for i in range(8)]
+    nt, np_ = 40, 80
+    print(f"case: quad leaf, {chunk} solves per pass, depth=3", flush=True)
+    with wf.POSolver(device=0) as sv:
+        sv.tune = {"leaf_quad": 1, "chunk_solves": chunk}
+        sv.set_geometry(s, 3)
+        I = sv.solve(incs, (nt, np_))
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs], nt, np_)
+    for k in range(len(incs)):
+        assert rel(I[k], Iref[k]) < 1e-4, (chunk, k, rel(I[k], Iref[k]))
+
+
 @pytest.mark.parametrize("version", [1, 2])
 @pytest.mark.parametrize("depth", [3, 4])
 def test_fused_transition_matches_two_pass(depth, version):

@@ -2445,25 +2445,29 @@ wf_status wf_autotune(wf_handle* h, const wf_incidence* inc, int32_t n_inc, wf_g
     h->tune.beta = chosen;
     rank_depths();
   }
-  // kernel variant at the model's best depth (only a BF16x3 choice; groups of
-  // <= 4 solves run the quad kernel in either setting)
+  // leaf kernel and solves per tree pass at the model's best depth (a BF16x3
+  // choice): quad-symmetric kernel with 4 or 8 solves per pass, or the plain
+  // kernel with 8
   if (h->opts.precision == WF_PREC_BF16X3 && S > 4 && h->opts.field_chunk <= 0) {
-    Run qa, qb;
-    h->tune.leaf_quad = 1;
-    h->tune.chunk_solves = 4;
-    e = run(rank[0].second, qa);
-    if (e == WF_OK) {
-      h->tune.leaf_quad = 0;
-      h->tune.chunk_solves = 8;
-      e = run(rank[0].second, qb);
+    static const int32_t cand[3][2] = {{1, 4}, {1, 8}, {0, 8}};  // leaf_quad, chunk_solves
+    double tbest = 0;
+    int ibest = 0;
+    for (int c = 0; c < 3; ++c) {
+      h->tune.leaf_quad = cand[c][0];
+      h->tune.chunk_solves = cand[c][1];
+      Run r;
+      e = run(rank[0].second, r);
+      if (e != WF_OK) {
+        h->tune = saved;
+        return e;
+      }
+      if (c == 0 || r.total < tbest) {
+        tbest = r.total;
+        ibest = c;
+      }
     }
-    if (e != WF_OK) {
-      h->tune = saved;
-      return e;
-    }
-    const bool quad = qa.total <= qb.total;
-    h->tune.leaf_quad = quad ? 1 : 0;
-    h->tune.chunk_solves = quad ? 4 : 8;
+    h->tune.leaf_quad = cand[ibest][0];
+    h->tune.chunk_solves = cand[ibest][1];
   }
   // level-transition kernels at the model's best depth: the staged fused
   // form or two passes and the two-pass theta tile of long rows (dense MFMA

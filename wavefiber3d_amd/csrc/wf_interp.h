@@ -226,7 +226,7 @@ struct LeafArgs {
   int nt, np;
   float2* out;
   int precision;
-  int quad;  // 1: quad-symmetric BF16x3 kernel where eligible (np even, F <= 24)
+  int quad;  // 1: quad-symmetric BF16x3 kernel where eligible (np even)
   // 1: F = 6 S fields (S solves' E x, y, z, H x, y, z) are summed and the
   // pattern receives the two transverse components of each solve instead,
   // out [leaf][2 S][nt][np] as launch_project_transverse forms them

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
   if (a.transverse) {
     // Two transverse components per solve (F = 6 S fields): lane (col, g)
     // stages its fields f = 8 m + 2 g + h of direction r in the wave's LDS
-    // slice [f][col], then lane (col, s = g) projects solve s onto the unit
+    // slice [f][col], then lane (col, g) projects solves g and g + 4 onto the unit
     // vectors t (theta, latitude) and p (phi) of the direction:
     //   a = t.E - z0 p.H,  b = p.E + z0 t.H.
     // The quadruple's directions flip sin(theta) (r2, r3) and cos/sin(phi)
@@ -373,23 +373,25 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
           ep[(m * 8 + 2 * g + hh) * 16 + col] =
               make_float2(acc[r][m][2 * hh], acc[r][m][2 * hh + 1]);
       leaf_wave_sync();
-      if (qvalid && g < S && (r < 2 || mirror_row)) {
+      if (qvalid && (r < 2 || mirror_row)) {
         const float sr = (r & 2) ? -sth : sth;
         const float pc = (r & 1) ? -cph : cph, ps = (r & 1) ? -sph : sph;
         const float t[3] = {-sr * pc, -sr * ps, cth}, p[3] = {-ps, pc, 0.f};
-        const float2* fs = ep + (6 * g) * 16 + col;
-        float2 av = make_float2(0.f, 0.f), bv = make_float2(0.f, 0.f);
+        for (int sv = g; sv < S; sv += 4) {  // solves g, g + 4 (S <= 8)
+          const float2* fs = ep + (6 * sv) * 16 + col;
+          float2 av = make_float2(0.f, 0.f), bv = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const float2 e = fs[c * 16], h = fs[(3 + c) * 16];
-          av.x += t[c] * e.x - z0 * p[c] * h.x;
-          av.y += t[c] * e.y - z0 * p[c] * h.y;
-          bv.x += p[c] * e.x + z0 * t[c] * h.x;
-          bv.y += p[c] * e.y + z0 * t[c] * h.y;
+          for (int c = 0; c < 3; ++c) {
+            const float2 e = fs[c * 16], h = fs[(3 + c) * 16];
+            av.x += t[c] * e.x - z0 * p[c] * h.x;
+            av.y += t[c] * e.y - z0 * p[c] * h.y;
+            bv.x += p[c] * e.x + z0 * t[c] * h.x;
+            bv.y += p[c] * e.y + z0 * t[c] * h.y;
+          }
+          float2* o = a.out + ((int64_t)leaf * 2 * S + 2 * sv) * D + dd[r];
+          o[0] = av;
+          o[D] = bv;
         }
-        float2* o = a.out + ((int64_t)leaf * 2 * S + 2 * g) * D + dd[r];
-        o[0] = av;
-        o[D] = bv;
       }
       leaf_wave_sync();
     }
@@ -496,7 +498,7 @@ template <int MT>
 hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
   constexpr int R = (MT <= 2) ? 4 : (MT <= 4 ? 2 : 1);
   const int D = a.nt * a.np;
-  if (bf16 && a.quad && MT <= 3 && (a.np % 2) == 0) {
+  if (bf16 && a.quad && (a.np % 2) == 0) {
     const int nquad = ((a.nt + 1) / 2) * (a.np / 2);
     const int nsuper = ((nquad + 15) / 16 + 3) / 4;
     const int64_t nblk = (int64_t)nsuper * a.nleaves;
@@ -545,7 +547,7 @@ hipError_t launch_mt(const wf::LeafArgs& a, bool bf16, hipStream_t st) {
 namespace wf {
 
 bool leaf_fuses_transverse(const LeafArgs& a, bool bf16) {
-  return bf16 && a.quad && a.F % 6 == 0 && (a.F + 7) / 8 <= 3 && (a.np % 2) == 0 &&
+  return bf16 && a.quad && a.F % 6 == 0 && (a.F + 7) / 8 <= 6 && (a.np % 2) == 0 &&
          a.f0 == 0 && a.F == a.Ftot;
 }
 
