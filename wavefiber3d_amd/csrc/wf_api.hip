@@ -1382,11 +1382,22 @@ wf_status forward_chunk(wf_handle* h, hipStream_t st, const GroupCtx& gc, const 
   const float* px = h->pts;
   float2* C = nullptr;
   WF_TRY(dalloc_t(h, "C", (size_t)n * F6, &C));
-  for (int q = 0; q < S; ++q)
-    WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n, [&] {
+  // one launch when the chunk's incidences are consecutive in dinc
+  bool consecutive = true;
+  for (int q = 1; q < S; ++q) consecutive = consecutive && order[c0 + q] == order[c0] + q;
+  if (consecutive) {
+    WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n * S, [&] {
       return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
-                                 px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F6, 0, st);
+                                 px + 6 * n, n, dinc + order[c0], S, 0, C, F6, 6, st);
     }));
+  } else {
+    for (int q = 0; q < S; ++q)
+      WF_TRY(timed(h, st, WF_K_CURRENTS, (28.0 + 48.0) * n, [&] {
+        return wf::launch_currents(px, px + n, px + 2 * n, px + 3 * n, px + 4 * n, px + 5 * n,
+                                   px + 6 * n, n, dinc + order[c0 + q], 1, 0, C + 6 * q, F6, 0,
+                                   st);
+      }));
+  }
   // the root reads level root_from directly when the levels above it have
   // grids larger than the output grid (their transitions would only cost)
   const int r = gc.root_from;

@@ -225,15 +225,19 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
   const double k_rev = a.k0 / WF_TWO_PI;
   const int quad = (sup * 4 + wave) * 16 + col;
   const bool qvalid = quad < nquad;
+  const bool wave_live = (sup * 4 + wave) * 16 < nquad;  // uniform per wave
   const int j = qvalid ? quad / nph : 0, k = qvalid ? quad - (quad / nph) * nph : 0;
   // rows of the block's 64 quads: jlo .. jlo + nrow - 1 (nrow <= RB, launcher)
   const int jlo = (sup * 64) / nph;
   const int nrow = min(nth - 1, (sup * 64 + 63) / nph) - jlo + 1;
   const int jrow = qvalid ? j - jlo : 0;
+  // direction cosines in FP32 (sincospi of theta/pi, phi/pi): an error of an
+  // ulp in s moves a leaf phase by ~1e-7 revolutions
+  const float inv_nt = 1.f / (float)a.nt, inv_np = 2.f / (float)a.np;
   for (int r = threadIdx.x; r < nrow; r += kThreads) {
-    double st_, ct_;
-    sincos(-WF_PI / 2 + WF_PI * (jlo + r + 0.5) / a.nt, &st_, &ct_);
-    sbz[r] = (float)st_;
+    float st_, ct_;
+    sincospif(((float)(jlo + r) + 0.5f) * inv_nt - 0.5f, &st_, &ct_);
+    sbz[r] = st_;
   }
   const bool mirror_row = (a.nt - 1 - j) != j;
   int dd[4];
@@ -244,13 +248,11 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
   // A = ax x + ay y (revolutions per metre scaled); B = s_z z comes from seb
   float ax = 0.f, ay = 0.f;
   if (qvalid) {
-    const double th = -WF_PI / 2 + WF_PI * (j + 0.5) / a.nt;
-    const double ph = WF_TWO_PI * k / a.np;
-    double st, ct, sp, cp;
-    sincos(th, &st, &ct);
-    sincos(ph, &sp, &cp);
-    ax = (float)(ct * cp);
-    ay = (float)(ct * sp);
+    float st, ct, sp, cp;
+    sincospif(((float)j + 0.5f) * inv_nt - 0.5f, &st, &ct);
+    sincospif((float)k * inv_np, &sp, &cp);
+    ax = ct * cp;
+    ay = ct * sp;
   }
   floatx4 acc[4][MT];
 #pragma unroll
@@ -292,7 +294,8 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
       scur[(ks * MT + mt) * 128 + 64 + ln] = lo;
     }
     __syncthreads();
-    for (int ks = 0; ks < ksteps; ++ks) {
+    // a wave whose 16 quads all lie past the grid only helps stage
+    for (int ks = 0; ks < (wave_live ? ksteps : 0); ++ks) {
       bf16x8 h0, l0, h2, l2;  // phases of r0 (A+B) and r2 (A-B)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -355,13 +358,8 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
     const int S = a.F / 6;
     float sth = 0.f, cth = 0.f, sph = 0.f, cph = 0.f;
     if (qvalid) {
-      double st, ct, sp, cp;
-      sincos(-WF_PI / 2 + WF_PI * (j + 0.5) / a.nt, &st, &ct);
-      sincos(WF_TWO_PI * k / a.np, &sp, &cp);
-      sth = (float)st;
-      cth = (float)ct;
-      sph = (float)sp;
-      cph = (float)cp;
+      sincospif(((float)j + 0.5f) * inv_nt - 0.5f, &sth, &cth);
+      sincospif((float)k * inv_np, &sph, &cph);
     }
     const float z0 = a.z0;
 #pragma unroll
