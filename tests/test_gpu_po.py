@@ -110,7 +110,11 @@ def agg_solver(request):
 @pytest.mark.parametrize("ntc,npc,ntp,npp,off", [
     (9, 16, 15, 28, None), (12, 20, 20, 36, None), (10, 18, 7, 12, "out"), (14, 26, 30, 60, "out"),
     (24, 48, 40, 80, None), (21, 35, 11, 22, "out"), (45, 80, 75, 144, None),
-    (36, 60, 90, 180, "out"), (30, 50, 25, 40, None)])
+    (36, 60, 90, 180, "out"), (30, 50, 25, 40, None),
+    # equal grids (the even length's Nyquist bin folds), a 240 -> 500 wave line,
+    # and theta lines above 1024 samples (the block-wide FFT kernels)
+    (24, 48, 24, 48, None), (24, 48, 24, 48, "out"), (120, 240, 250, 500, None),
+    (30, 60, 520, 1040, "out")])
 def test_interp_matches_bruteforce(agg_solver, ntc, npc, ntp, npp, off):
     solver = agg_solver
     rng = np.random.default_rng(ntc * 100 + npp)

@@ -225,7 +225,9 @@ __global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict
                                                        const float2* __restrict__ fac_g) {
   extern __shared__ float2 sm[];
   const int nw = (int)(blockDim.x >> 6), wave = (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int lmax = npc > npp ? npc : npp;
+  // odd line stride: the transposed read-out (one sample of every line per
+  // 16 lanes) then spreads over the LDS banks
+  const int lmax = (npc > npp ? npc : npp) | 1;
   float2* twc = sm;
   float2* twp = twc + pc.twn;
   float2* fac = twp + pp.twn;
@@ -463,8 +465,9 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
     // rows per block: 16 (128 B runs of the transposed writes, full cache
     // lines) where the LDS fits two blocks per CU, else 8
     const size_t tab = (size_t)pc.twn + pp.twn + npc + 1;
-    const int NW = (ntc >= 16 && sizeof(float2) * (tab + 16 * (size_t)lmax) <= 80 * 1024) ? 16 : 8;
-    const size_t sh = sizeof(float2) * (tab + (size_t)NW * lmax);
+    const size_t lstr = (size_t)(lmax | 1);  // the kernel's line stride
+    const int NW = (ntc >= 16 && sizeof(float2) * (tab + 16 * lstr) <= 80 * 1024) ? 16 : 8;
+    const size_t sh = sizeof(float2) * (tab + (size_t)NW * lstr);
     if (sh > 160 * 1024) return hipErrorInvalidValue;
     const int64_t nblk = lines_cf * ((ntc + NW - 1) / NW);
     if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
