@@ -218,7 +218,7 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
 // (child, field) plane; each wave transforms its row (FFT(npc), resample, inverse FFT(npp))
 // and leaves the npp outputs in its LDS line; the block then writes
 // GT[cf][q][r0 .. r0 + nw) with the rows contiguous.
-template <int NMAX, int RL>
+template <int NMAX, int RL, bool PRE>
 __global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict__ X,
                                                        float2* __restrict__ GT, int ntc, int npc,
                                                        int npp, Fft2Plan pc, Fft2Plan pp,
@@ -244,10 +244,10 @@ __global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict
   if (wave < nr) {
     const float2* row = X + (cf * ntc + r0 + wave) * (int64_t)npc;
     auto src = [&](int j) { return row[j]; };
-    wffft3::forward<NMAX>(pc, buf, twc, src, lane);
+    wffft3::forward<NMAX>(pc, buf, twc, src, lane, PRE ? fac : nullptr, npp);
     float2 o[wffft3::Slots<NMAX, RL>::v];
     auto keep = [&](int s, int, float2 z) { o[s] = z; };
-    wffft3::inverse<NMAX, RL>(pp, buf, twp, fac, npc, keep, lane);
+    wffft3::inverse<NMAX, RL>(pp, buf, twp, fac, npc, keep, lane, PRE);
     auto put = [&](int s, int e) { buf[e] = o[s]; };
     wffft3::for_outputs<NMAX, RL>(pp, put, lane);
   }
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict
 // so a root child's table rows are fetched into that XCD's L2 once for all
 // fields.
 constexpr int kNw = 4;
-template <int NMAX, int RL, bool ROOT>
+template <int NMAX, int RL, bool ROOT, bool PRE>
 __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
     const float2* __restrict__ GT, int F, int ntc, int npp, int ntp, int64_t U, int nkg,
     const int32_t* __restrict__ cb, const int32_t* __restrict__ ce,
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
       const float2 v = S1[-jp];
       return make_float2(sgn2 * v.x, sgn2 * v.y);
     };
-    wffft3::forward<NMAX>(pc, buf, twc, src, ln);
+    wffft3::forward<NMAX>(pc, buf, twc, src, ln, PRE ? fac : nullptr, n2p);
     const int oct = text_per_child ? c : (child_key ? (int)(child_key[c] & 7u) : 0);
     const float2* To = Text ? Text + ((int64_t)oct * half + k) * (n2p + 1) : nullptr;
     auto sink = [&](int s, int e, float2 z) {
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
         acc2.y += t2.y;
       }
     };
-    wffft3::inverse<NMAX, RL>(pp, buf, twp, fac, n2c, sink, ln);
+    wffft3::inverse<NMAX, RL>(pp, buf, twp, fac, n2c, sink, ln, PRE);
   }
   const int64_t obase = ((int64_t)p * F + f) * (int64_t)ntp * npp;
   auto put = [&](int s, int e) {
@@ -370,8 +370,8 @@ __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
   wffft3::for_outputs<NMAX, RL>(pp, put, lane);
 }
 
-template <int NMAX, int RL, bool ROOT>
-hipError_t launch_theta_fft3_t(const wf::ThetaFftArgs& a, hipStream_t st) {
+template <int NMAX, int RL, bool ROOT, bool PRE>
+hipError_t launch_theta_fft3_p(const wf::ThetaFftArgs& a, hipStream_t st) {
   const int n2c = 2 * a.ntc, n2p = 2 * a.ntp, half = a.npp / 2;
   const int lmax = n2c > n2p ? n2c : n2p;
   const size_t sh = sizeof(float2) * ((size_t)a.pc.twn + a.pp.twn + n2c + 1 + (size_t)kNw * lmax);
@@ -380,13 +380,19 @@ hipError_t launch_theta_fft3_t(const wf::ThetaFftArgs& a, hipStream_t st) {
   const int64_t U = (int64_t)a.nparents * nkg;
   const int64_t nblk = ((U + 7) / 8) * 8 * a.F;
   if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute((const void*)theta_fft3_kernel<NMAX, RL, ROOT>,
+  hipError_t e = hipFuncSetAttribute((const void*)theta_fft3_kernel<NMAX, RL, ROOT, PRE>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((theta_fft3_kernel<NMAX, RL, ROOT>), dim3((unsigned)nblk), dim3(64 * kNw), sh, st,
+  hipLaunchKernelGGL((theta_fft3_kernel<NMAX, RL, ROOT, PRE>), dim3((unsigned)nblk), dim3(64 * kNw), sh, st,
                      a.GT, a.F, a.ntc, a.npp, a.ntp, U, nkg, a.cb, a.ce, a.child_key, a.Text,
                      a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root, a.flip);
   return hipGetLastError();
+}
+
+template <int NMAX, int RL, bool ROOT>
+hipError_t launch_theta_fft3_t(const wf::ThetaFftArgs& a, hipStream_t st) {
+  if (wffft3::fused_resample(a.pc, 2 * a.ntp)) return launch_theta_fft3_p<NMAX, RL, ROOT, true>(a, st);
+  return launch_theta_fft3_p<NMAX, RL, ROOT, false>(a, st);
 }
 
 template <bool ROOT>
@@ -473,14 +479,22 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
     if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
     const int rl = fft2_radix(pp, pp.nstages - 1);
     if (rl != 2 && rl != 4 && rl != 8) return hipErrorInvalidValue;  // even lengths end on 8, 4, 2
-#define WF_PHI3(N, RL)                                                                            \
+    const bool pre = wffft3::fused_resample(pc, npp);
+#define WF_PHI3P(N, RL, P)                                                                        \
   do {                                                                                            \
-    hipError_t e = hipFuncSetAttribute((const void*)phi_fft3_kernel<N, RL>,                       \
+    hipError_t e = hipFuncSetAttribute((const void*)phi_fft3_kernel<N, RL, P>,                    \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);      \
     if (e != hipSuccess) return e;                                                                \
-    hipLaunchKernelGGL((phi_fft3_kernel<N, RL>), dim3((unsigned)nblk), dim3(64 * NW), sh, st, X,  \
-                       GT, ntc, npc, npp, pc, pp, fac);                                           \
+    hipLaunchKernelGGL((phi_fft3_kernel<N, RL, P>), dim3((unsigned)nblk), dim3(64 * NW), sh, st,  \
+                       X, GT, ntc, npc, npp, pc, pp, fac);                                        \
     return hipGetLastError();                                                                     \
+  } while (0)
+#define WF_PHI3(N, RL)         \
+  do {                         \
+    if (pre)                   \
+      WF_PHI3P(N, RL, true);   \
+    else                       \
+      WF_PHI3P(N, RL, false);  \
   } while (0)
     if (rl == 4) {
       if (lmax <= 512) WF_PHI3(512, 4);
@@ -496,6 +510,7 @@ hipError_t launch_phi_fft(const float2* X, float2* GT, int64_t lines_cf, int ntc
     if (lmax <= 768) WF_PHI3(768, 2);
     WF_PHI3(1024, 2);
 #undef WF_PHI3
+#undef WF_PHI3P
   }
   if (lmax > kLineBuf || pc.n != npc || pp.n != npp) return hipErrorInvalidValue;
   const int NT = fft_threads();

@@ -315,6 +315,75 @@ __device__ __forceinline__ void rs_first_t(float2* buf, int np, int nc, uint32_t
   wave_sync();
 }
 
+// last forward stage (Ns R = nc, so h = 0, k = t) of a strictly upsampling
+// resample (np > nc): bin b = t + r m goes straight to its padded position,
+// conj(fac[f - fmin] X[b]) at q = f mod np (f = b or b - nc in the band; the
+// even length's Nyquist bin to both q = nc/2 and np - nc/2), zeros between
+// the band's ends; the inverse transform then starts with a plain stage
+template <int R, int NMAX>
+__device__ __forceinline__ void mid_rs_t(float2* buf, int nc, const float2* tw, int np,
+                                         const float2* fac, int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = nc / R;
+  const int fmin = -(nc / 2), fmax = (nc - 1) / 2 + ((nc % 2 == 0) ? 1 : 0);
+  float2 v[IM][R];
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[t + r * m];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[t * (R - 1) + r - 1]);
+      wffft2::dft<R>(v[i]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int b = t + r * m;
+        const int f = b <= fmax ? b : b - nc;
+        buf[f >= 0 ? f : f + np] = conjf2(cmul(v[i][r], fac[f - fmin]));
+        if (b == fmax && fmax == -fmin)  // even nc: the Nyquist bin's other half
+          buf[np + fmin] = conjf2(cmul(v[i][r], fac[0]));
+      }
+    }
+  }
+  for (int q = fmax + 1 + lane; q < np + fmin; q += 64) buf[q] = make_float2(0.f, 0.f);
+  wave_sync();
+}
+
+// first stage of a transform whose input is already in buf (Ns = 1)
+template <int R, int NMAX>
+__device__ __forceinline__ void plain_first_t(float2* buf, int n, int lane) {
+  constexpr int IM = Im<R, NMAX>::v;
+  const int m = n / R;
+  float2 v[IM][R];
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[t + r * m];
+      wffft2::dft<R>(v[i]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int t = lane + 64 * i;
+    if (t < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[t * R + r] = v[i][r];
+    }
+  }
+  wave_sync();
+}
+
 // last stage (Ns R = n): outputs e = t + r m, conjugated (inverse), to sink
 template <int R, int NMAX, class Sink>
 __device__ __forceinline__ void last_t(const float2* buf, int n, const float2* tw, Sink& sink,
@@ -361,16 +430,25 @@ __device__ __forceinline__ void outputs_t(int n, F& f, int lane) {
     default: CALL(5); break;    \
   }
 
-// Forward FFT of the line src(j), j < p.n, into buf (natural order).
+// resample fused into the forward transform's last stage: strict
+// upsampling onto a line of np, a forward plan of at least two stages
+__host__ __device__ inline bool fused_resample(const Fft2Plan& pc, int np) {
+  return np > pc.n && pc.nstages >= 2;
+}
+
+// Forward FFT of the line src(j), j < p.n, into buf (natural order); with
+// fac (np > p.n, fused_resample) the last stage writes the conjugated,
+// resampled spectrum of length np instead (inverse(..., pre = true) next).
 template <int NMAX, class Src>
 __device__ __forceinline__ void forward(const Fft2Plan& p, float2* buf, const float2* tw, Src& src,
-                                        int lane) {
+                                        int lane, const float2* fac = nullptr, int np = 0) {
   const int R0 = fft2_radix(p, 0);
 #define WF_C(R) first_t<R, NMAX>(buf, p.n, src, lane)
   WF_FFT3_RADIX(R0, WF_C)
 #undef WF_C
   int Ns = R0, off = R0 - 1;
-  for (int s = 1; s < p.nstages; ++s) {
+  const int last = fac ? p.nstages - 1 : p.nstages;
+  for (int s = 1; s < last; ++s) {
     const int R = fft2_radix(p, s);
 #define WF_C(R) mid_t<R, NMAX>(buf, p.n, Ns, p.mag_ns[s], tw + off, lane)
     WF_FFT3_RADIX(R, WF_C)
@@ -378,17 +456,31 @@ __device__ __forceinline__ void forward(const Fft2Plan& p, float2* buf, const fl
     off += Ns * (R - 1);
     Ns *= R;
   }
+  if (fac) {
+    const int R = fft2_radix(p, p.nstages - 1);
+#define WF_C(R) mid_rs_t<R, NMAX>(buf, p.n, tw + off, np, fac, lane)
+    WF_FFT3_RADIX(R, WF_C)
+#undef WF_C
+  }
 }
 
 // Spectrum of length nc in buf -> resample (fac) -> inverse FFT of length
-// p.n (p.nstages >= 2, last radix RL) -> sink(slot, e, value).
+// p.n (p.nstages >= 2, last radix RL) -> sink(slot, e, value).  pre: buf
+// already holds the conjugated resampled spectrum (forward with fac).
 template <int NMAX, int RL, class Sink>
 __device__ __forceinline__ void inverse(const Fft2Plan& p, float2* buf, const float2* tw,
-                                        const float2* fac, int nc, Sink& sink, int lane) {
+                                        const float2* fac, int nc, Sink& sink, int lane,
+                                        bool pre = false) {
   const int R0 = fft2_radix(p, 0);
-#define WF_C(R) rs_first_t<R, NMAX>(buf, p.n, nc, p.mag_n, fac, lane)
-  WF_FFT3_RADIX(R0, WF_C)
+  if (pre) {
+#define WF_C(R) plain_first_t<R, NMAX>(buf, p.n, lane)
+    WF_FFT3_RADIX(R0, WF_C)
 #undef WF_C
+  } else {
+#define WF_C(R) rs_first_t<R, NMAX>(buf, p.n, nc, p.mag_n, fac, lane)
+    WF_FFT3_RADIX(R0, WF_C)
+#undef WF_C
+  }
   int Ns = R0, off = R0 - 1;
   for (int s = 1; s < p.nstages - 1; ++s) {
     const int R = fft2_radix(p, s);
