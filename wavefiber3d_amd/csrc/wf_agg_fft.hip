@@ -322,20 +322,24 @@ __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
     const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
     const float2* S0 = S + (int64_t)k * ntc;
     const float2* S1 = S + (int64_t)(k + half) * ntc + (n2c - 1);
+    // branch-free gather (address and sign selected per sample), so the
+    // stage's loads issue back to back and are waited for once
     auto src = [&](int jp) -> float2 {
-      if (jp < ntc) return S0[jp];
-      const float2 v = S1[-jp];
-      return make_float2(sgn2 * v.x, sgn2 * v.y);
+      const bool first = jp < ntc;
+      const float2 v = *(first ? S0 + jp : S1 - jp);
+      const float sg = first ? 1.f : sgn2;
+      return make_float2(sg * v.x, sg * v.y);
     };
     wffft3::forward<NMAX>(pc, buf, twc, src, ln, PRE ? fac : nullptr, n2p);
     const int oct = text_per_child ? c : (child_key ? (int)(child_key[c] & 7u) : 0);
     const float2* To = Text ? Text + ((int64_t)oct * half + k) * (n2p + 1) : nullptr;
+    const float2 tdual = (To && dual) ? To[n2p] : make_float2(1.f, 0.f);  // pole row
     auto sink = [&](int s, int e, float2 z) {
       const float2 tu = To ? cmul(To[e], z) : z;
       acc[s].x += tu.x;
       acc[s].y += tu.y;
       if (dual && e == 0) {
-        const float2 t2 = To ? cmul(To[n2p], z) : z;
+        const float2 t2 = cmul(tdual, z);
         acc2.x += t2.x;
         acc2.y += t2.y;
       }

@@ -201,10 +201,12 @@ __device__ __forceinline__ void first_t(float2* buf, int n, Src& src, int lane) 
   constexpr int IM = Im<R, NMAX>::v;
   const int m = n / R;
   float2 v[IM][R];
+  // loads under a wave-uniform guard only (lanes past m re-read butterfly
+  // m - 1), so they issue back to back instead of one divergent branch each
 #pragma unroll
   for (int i = 0; i < IM; ++i) {
-    const int t = lane + 64 * i;
-    if (t < m) {
+    if (64 * i < m) {
+      const int t = min(lane + 64 * i, m - 1);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[i][r] = src(t + r * m);
     }
