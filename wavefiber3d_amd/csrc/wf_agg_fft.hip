@@ -387,15 +387,17 @@ hipError_t launch_theta_fft3_p(const wf::ThetaFftArgs& a, hipStream_t st) {
   hipError_t e = hipFuncSetAttribute((const void*)theta_fft3_kernel<NMAX, RL, ROOT, PRE>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((theta_fft3_kernel<NMAX, RL, ROOT, PRE>), dim3((unsigned)nblk), dim3(64 * kNw), sh, st,
-                     a.GT, a.F, a.ntc, a.npp, a.ntp, U, nkg, a.cb, a.ce, a.child_key, a.Text,
-                     a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out, a.out_root, a.flip);
+  hipLaunchKernelGGL((theta_fft3_kernel<NMAX, RL, ROOT, PRE>), dim3((unsigned)nblk),
+                     dim3(64 * kNw), sh, st, a.GT, a.F, a.ntc, a.npp, a.ntp, U, nkg, a.cb, a.ce,
+                     a.child_key, a.Text, a.text_per_child, a.mirror, a.pc, a.pp, a.fac, a.out,
+                     a.out_root, a.flip);
   return hipGetLastError();
 }
 
 template <int NMAX, int RL, bool ROOT>
 hipError_t launch_theta_fft3_t(const wf::ThetaFftArgs& a, hipStream_t st) {
-  if (wffft3::fused_resample(a.pc, 2 * a.ntp)) return launch_theta_fft3_p<NMAX, RL, ROOT, true>(a, st);
+  if (wffft3::fused_resample(a.pc, 2 * a.ntp))
+    return launch_theta_fft3_p<NMAX, RL, ROOT, true>(a, st);
   return launch_theta_fft3_p<NMAX, RL, ROOT, false>(a, st);
 }
 
