@@ -368,13 +368,20 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
   const int cq = (int)threadIdx.x / ntc, cr = (int)threadIdx.x - cq * ntc;
   float4 xr[kChunk / 2];
   // child c, field f: its plane starts at (c F + f) ntc npc
+  // loads at clamped, in-range addresses (threads without a chunk read the
+  // plane's first samples), values masked after: no divergent branch around
+  // them, so they stay in flight until stash()
   auto prefetch_cf = [&](int c, int f) {
-    if (has_chunk) {
-      const float4* src = reinterpret_cast<const float4*>(
-          P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc) + (int64_t)cr * npc + kChunk * cq);
+    if (STAGE) {
+      const float2* row = P.X + ((int64_t)c * F + f) * ((int64_t)ntc * npc) +
+                          (int64_t)(has_chunk ? cr : 0) * npc;
 #pragma unroll
-      for (int h = 0; h < kChunk / 2; ++h)
-        xr[h] = (kChunk * cq + 2 * h < npc) ? src[h] : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int h = 0; h < kChunk / 2; ++h) {
+        const int k = kChunk * cq + 2 * h;
+        const bool ok = has_chunk && k < npc;
+        const float4 v = *reinterpret_cast<const float4*>(row + (ok ? k : 0));
+        xr[h] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
   };
   auto stash = [&]() {
@@ -552,7 +559,9 @@ __global__ __launch_bounds__(LB, WPE) void level_fused2_kernel(const FusedK P) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               const int e = 16 * (wave + a * NW) + 4 * g + j;
-              tf[a][j] = (kk < half && e < n2p) ? To[(int64_t)e * half + kk] : make_float2(0.f, 0.f);
+              const bool ok = kk < half && e < n2p;  // clamped read, masked value
+              const float2 v = To[ok ? (int64_t)e * half + kk : 0];
+              tf[a][j] = ok ? v : make_float2(0.f, 0.f);
             }
           floatx4 ur[MAXE], ui[MAXE];
 #pragma unroll
