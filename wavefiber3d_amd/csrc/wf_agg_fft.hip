@@ -7,7 +7,8 @@
 //
 //   phi_fft:   per block RB theta rows of one (child, field): FFT(npc) ->
 //              resample to npp (pad / fold, Nyquist halved, 1/npc) ->
-//              inverse FFT(npp) -> GT[cf][q][theta] (theta contiguous).
+//              inverse FFT(npp) -> GT[cf][q][theta] (theta contiguous,
+//              stride fft_g_stride(ntc)).
 //   theta_fft: per block W column pairs of one (parent, field); for each
 //              child the great-circle sequences (column k ascending, column
 //              k + npp/2 reversed) -> FFT(2 ntc) -> resample with the origin
@@ -89,10 +90,11 @@ __global__ __launch_bounds__(NT) void phi_fft_kernel(const float2* __restrict__ 
   __syncthreads();
   float2* Z = wffft2::lines(Ys, Xs, nr, lmax, pp, twp, 1);
   // GT[cf][q][r0 + r]: consecutive r contiguous
-  float2* dst = GT + cf * (int64_t)npp * ntc + r0;
+  const int ldg = wf::fft_g_stride(ntc);
+  float2* dst = GT + cf * (int64_t)npp * ldg + r0;
   for (int t = threadIdx.x; t < nr * npp; t += blockDim.x) {
     const int q = t / nr, r = t - q * nr;
-    dst[(int64_t)q * ntc + r] = Z[r * lmax + q];
+    dst[(int64_t)q * ldg + r] = Z[r * lmax + q];
   }
 }
 
@@ -144,15 +146,16 @@ __global__ __launch_bounds__(NT) void theta_fft_kernel(
   acc2.y = 0;
   const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
   for (int c = c0; c < c1; ++c) {
-    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
+    const int ldg = wf::fft_g_stride(ntc);
+    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ldg;
     __syncthreads();  // previous child done with a/b
     for (int t = threadIdx.x; t < nw * n2c; t += blockDim.x) {
       const int w = t / n2c, jp = t - w * n2c;
       const int k = k0 + w;
       if (jp < ntc) {
-        a[w * lmax + jp] = S[(int64_t)k * ntc + jp];
+        a[w * lmax + jp] = S[(int64_t)k * ldg + jp];
       } else {
-        const float2 v = S[(int64_t)(k + half) * ntc + (n2c - 1 - jp)];
+        const float2 v = S[(int64_t)(k + half) * ldg + (n2c - 1 - jp)];
         a[w * lmax + jp] = make_float2(sgn2 * v.x, sgn2 * v.y);
       }
     }
@@ -253,11 +256,12 @@ __global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict
   }
   __syncthreads();
   // nw is a power of two (launcher): thread t -> (q = t / nw, row w = t % nw)
-  float2* dst = GT + cf * (int64_t)npp * ntc + r0;
+  const int ldg = wf::fft_g_stride(ntc);
+  float2* dst = GT + cf * (int64_t)npp * ldg + r0;
   const int sh = __builtin_ctz((unsigned)nw);
   for (int t = threadIdx.x; t < (npp << sh); t += blockDim.x) {
     const int q = t >> sh, w = t & (nw - 1);
-    if (w < nr) dst[(int64_t)q * ntc + w] = lines[w * lmax + q];
+    if (w < nr) dst[(int64_t)q * ldg + w] = lines[w * lmax + q];
   }
 }
 
@@ -313,15 +317,16 @@ __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
   acc2.x = 0;
   acc2.y = 0;
   float2* buf = lines + wave * lmax;
+  const int ldg = wf::fft_g_stride(ntc);
   const int c0 = cb ? cb[p] : p, c1 = cb ? ce[p] : p + 1;
   for (int c = c0; c < c1; ++c) {
     // the lane index through an empty asm: the stages' lane-derived addresses
     // are recomputed per child instead of all being hoisted out of the loop
     int ln = lane;
     __asm__ volatile("" : "+v"(ln));
-    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ntc;
-    const float2* S0 = S + (int64_t)k * ntc;
-    const float2* S1 = S + (int64_t)(k + half) * ntc + (n2c - 1);
+    const float2* S = GT + ((int64_t)c * F + f) * (int64_t)npp * ldg;
+    const float2* S0 = S + (int64_t)k * ldg;
+    const float2* S1 = S + (int64_t)(k + half) * ldg + (n2c - 1);
     // branch-free gather (address and sign selected per sample), so the
     // stage's loads issue back to back and are waited for once
     auto src = [&](int jp) -> float2 {

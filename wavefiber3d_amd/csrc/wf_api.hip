@@ -938,7 +938,9 @@ wf_status wf_interp(wf_handle* h, int32_t count, int32_t ntc, int32_t npc, const
   WF_TRY(get_fft(h, 2 * ntp, &tp));
   float2* G = nullptr;
   WF_TRY(dalloc_t(h, "interp_G",
-                  std::max((size_t)count * ntc * npp, (size_t)wf::gc_elems(count, ntc, npp)), &G));
+                  std::max((size_t)count * wf::fft_g_stride(ntc) * npp,
+                           (size_t)wf::gc_elems(count, ntc, npp)),
+                  &G));
   const double off_tc = -WF_PI / 2 + WF_PI / (2.0 * ntc);
   const int mirror = (int)std::lround(ntp * (1.0 - 2.0 * off_tp / WF_PI));
   const bool mfma_ok = h->tune.agg_kernel == 0 && (mirror == 2 * ntp - 1 || mirror == 2 * ntp) &&
@@ -1350,10 +1352,11 @@ wf_status tree_buffers(wf_handle* h, const GroupCtx& gc, int F, int NP, TreeBufs
   size_t pat_max = 16, g_max = 16;
   for (int l = 1; l <= depth; ++l)
     pat_max = std::max(pat_max, (size_t)h->levels[l].count * F * gc.g[l].nt * gc.g[l].np);
-  // G holds the phi-pass output of one level: [child][F][np_parent][nt_child]
+  // G holds the phi-pass output of one level: [child][F][np_parent][fft_g_stride(nt_child)]
   // for the FFT kernels, padded great-circle rows for the MFMA kernels
   auto g_elems = [&](int64_t nchild, int ntc, int npp) {
-    return std::max((size_t)nchild * F * ntc * npp, (size_t)wf::gc_elems(nchild * F, ntc, npp));
+    return std::max((size_t)nchild * F * wf::fft_g_stride(ntc) * npp,
+                    (size_t)wf::gc_elems(nchild * F, ntc, npp));
   };
   for (int l = 1; l < depth; ++l)
     g_max = std::max(g_max, g_elems(h->levels[l + 1].count, gc.g[l + 1].nt, gc.g[l].np));

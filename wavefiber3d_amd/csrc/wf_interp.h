@@ -182,6 +182,11 @@ struct ThetaFftArgs {
 };
 hipError_t launch_theta_fft(const ThetaFftArgs& a, hipStream_t st);
 
+// theta stride of the FFT phi output G [child][F][npp][fft_g_stride(ntc)]:
+// ntc rounded up to 16 samples, so 16-row phi blocks write whole 128-byte
+// lines and every column starts line-aligned (the padding is never read)
+__host__ __device__ inline int fft_g_stride(int ntc) { return (ntc + 15) & ~15; }
+
 // The LDS FFT kernels keep lines of up to 1024 samples wave-private (one wave
 // per line, no workgroup barriers between stages); longer lines run the
 // block-wide kernels.  The theta choice fixes the translation-table layout.
