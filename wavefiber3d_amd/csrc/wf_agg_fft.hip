@@ -270,10 +270,11 @@ __global__ __launch_bounds__(1024) void phi_fft3_kernel(const float2* __restrict
 // (FFT(2 ntc), resample with the origin shift, inverse FFT(2 ntp)), multiplies
 // its outputs by the child's translation factors (k-major table: one
 // contiguous row per great circle) and accumulates them in registers (FP64 at
-// the root).  Block b -> (u = parent * nkg + k group, field): the F blocks of
-// one u are consecutive on one XCD (blocks go round-robin over the 8 XCDs),
-// so a root child's table rows are fetched into that XCD's L2 once for all
-// fields.
+// the root).  Block b -> (u = parent * nkg + k group, field): blocks go
+// round-robin over the 8 XCDs, and XCD x takes a contiguous range of u with
+// the F fields of one u consecutive, so a root child's table rows are fetched
+// into that XCD's L2 once for all fields and neighbouring k groups, which
+// write the same parent rows, share one L2.
 constexpr int kNw = 4;
 template <int NMAX, int RL, bool ROOT, bool PRE>
 __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(64 * kNw, 3) void theta_fft3_kernel(
   const int xcd = (int)(blockIdx.x & 7u);
   const int64_t j = blockIdx.x >> 3;
   const int f = (int)(j % F);
-  const int64_t u = (j / F) * 8 + xcd;
+  const int64_t u = xcd * ((U + 7) / 8) + j / F;  // XCD x: k groups [x per, (x + 1) per)
   if (u >= U) return;
   const int p = (int)(u / nkg);
   const int k = (int)(u - (int64_t)p * nkg) * kNw + wave;
