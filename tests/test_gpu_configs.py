@@ -134,11 +134,36 @@ def test_autotune_picks_beta_under_its_gate_on_the_fiber():
         t = sv.tune
         print(f"  tune {t}", flush=True)
         assert t["beta"] == 2.0
+        # the FFT threshold only moves down from the default, one level at a time
+        assert t["agg_fft_rows"] <= 600 and t["chunk_solves"] in (4, 8)
         I = sv.solve(incs[:2], grid, out="torch").cpu().numpy()
         I64 = sv.solve_fp64_direct(dev, incs[:2], grid).cpu().numpy()
     e = max(rel(I[k], I64[k]) for k in range(2))
     print(f"  rel-L2 vs FP64 {e:.2e}", flush=True)
     assert e < TREE_GATE
+
+
+def test_fiber_fft_and_mfma_levels_agree():
+    """The fiber's 125 x 240 -> 240 x 480 and 72 x 128 -> 125 x 240 transitions
+    on the wave-private LDS FFT kernels (agg_fft_rows 479, 249) or on the MFMA
+    kernels (600): the same interpolation, so the solves agree far inside the
+    tree's error against the FP64 direct sum."""
+    s = wf.fiber_segment(40e-6, 30e-6, 30e-6, 0.1e-6, seed=0)
+    incs = [wf.Incidence(wf.reference_incidence_theta(t), 0.0, 0.4e-6, "E", 1.55 + 0.02j)
+            for t in (60, 90)]
+    grid = (360, 720)
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in s.arrays()]
+    out = {}
+    with wf.POSolver(device=0, beta=BETA) as sv:
+        sv.set_geometry(dev, 6)
+        for rows in (600, 479, 249):
+            print(f"case: fiber depth 6, agg_fft_rows {rows}", flush=True)
+            sv.tune = {"agg_fft_rows": rows}
+            out[rows] = sv.solve(incs, grid, out="torch").cpu().numpy()
+    for rows in (479, 249):
+        e = max(rel(out[rows][j], out[600][j]) for j in range(len(incs)))
+        print(f"  agg_fft_rows {rows}: max rel-L2 vs the MFMA levels {e:.2e}", flush=True)
+        assert e < 3e-5
 
 
 def test_fiber_two_pass_tiles_agree():
