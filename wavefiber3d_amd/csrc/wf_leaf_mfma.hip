@@ -215,6 +215,7 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
   const int D = a.nt * a.np;
   const int nth = (a.nt + 1) / 2, nph = a.np / 2;
+  const bool perm = (MT % 3 == 0) && a.transverse;  // solve-per-lane-group field order
   const int nquad = nth * nph;
   const int qtiles = (nquad + 15) / 16;
   const int nsuper = (qtiles + 3) / 4;
@@ -275,7 +276,12 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
     for (int idx = threadIdx.x; idx < ksteps * MT * 64; idx += kThreads) {
       const int ln = idx & 63, mt = (idx >> 6) % MT, ks = idx / (64 * MT);
       const int cc = ln & 15, gg = ln >> 4;
-      const int f = mt * 8 + (cc >> 1), part = cc & 1;
+      // transverse passes of 3 or 6 tiles order the fields so that lane group
+      // g of tiles 3q .. 3q + 2 holds the six fields of solve g + 4q
+      const int fl = cc >> 1;
+      const int f = perm ? 6 * ((fl >> 1) + 4 * (mt / 3)) + 2 * (mt % 3) + (fl & 1)
+                         : mt * 8 + fl;
+      const int part = cc & 1;
       bf16x8 hi, lo;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -353,8 +359,6 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
     //   a = t.E - z0 p.H,  b = p.E + z0 t.H.
     // The quadruple's directions flip sin(theta) (r2, r3) and cos/sin(phi)
     // (r1, r3).
-    __syncthreads();  // every wave done with the staged points and currents
-    float2* ep = reinterpret_cast<float2*>(smem4) + wave * (MT * 8 * 16);
     const int S = a.F / 6;
     float sth = 0.f, cth = 0.f, sph = 0.f, cph = 0.f;
     if (qvalid) {
@@ -362,6 +366,43 @@ __global__ __launch_bounds__(kThreads) void leaf_bf16x3_quad_kernel(wf::LeafArgs
       sincospif((float)k * inv_np, &sph, &cph);
     }
     const float z0 = a.z0;
+    if constexpr (MT % 3 == 0) {
+      // permuted fields: tile 3q + i, half h holds component 2 i + h of solve
+      // g + 4 q (E x, y, z, H x, y, z), projected in registers
+      if (!qvalid) return;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r >= 2 && !mirror_row) break;
+        const float sr = (r & 2) ? -sth : sth;
+        const float pc = (r & 1) ? -cph : cph, ps = (r & 1) ? -sph : sph;
+        const float t[3] = {-sr * pc, -sr * ps, cth}, p[3] = {-ps, pc, 0.f};
+#pragma unroll
+        for (int q = 0; q < MT / 3; ++q) {
+          const int sv = g + 4 * q;
+          if (sv >= S) continue;
+          float2 fld[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c)
+            fld[c] = make_float2(acc[r][3 * q + c / 2][2 * (c & 1)],
+                                 acc[r][3 * q + c / 2][2 * (c & 1) + 1]);
+          float2 av = make_float2(0.f, 0.f), bv = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const float2 e = fld[c], h = fld[3 + c];
+            av.x += t[c] * e.x - z0 * p[c] * h.x;
+            av.y += t[c] * e.y - z0 * p[c] * h.y;
+            bv.x += p[c] * e.x + z0 * t[c] * h.x;
+            bv.y += p[c] * e.y + z0 * t[c] * h.y;
+          }
+          float2* o = a.out + ((int64_t)leaf * 2 * S + 2 * sv) * D + dd[r];
+          o[0] = av;
+          o[D] = bv;
+        }
+      }
+      return;
+    }
+    __syncthreads();  // every wave done with the staged points and currents
+    float2* ep = reinterpret_cast<float2*>(smem4) + wave * (MT * 8 * 16);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
