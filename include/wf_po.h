@@ -278,7 +278,11 @@ typedef struct wf_tune {
   double agg_s_per_byte;  /* aggregation, per compulsory HBM byte (complex64 patterns) */
   double level_s;         /* fixed cost per level transition and solve */
   int32_t leaf_quad;      /* 1: quad-symmetric BF16x3 leaf kernel where eligible (even
-                             leaf-grid n_phi); 0: plain */
+                             leaf-grid n_phi), a block running several groups of 64
+                             direction quads of its leaf on one staging of the leaf's
+                             points where that pays; 2: the same kernel, one group per
+                             block; 3: all of a leaf's groups in one block; 0: plain.
+                             Results are the same for 1, 2 and 3 */
   int32_t chunk_solves;   /* solves per tree pass (1..8); opts.field_chunk overrides */
   int32_t agg_kernel;     /* level transitions: 0 = dense split-BF16 MFMA interpolation for
                              levels whose extended parent grid (2 Nt) has at most agg_fft_rows

@@ -239,6 +239,32 @@ def test_quad_leaf_solves_per_pass(chunk):
         assert rel(I[k], Iref[k]) < 1e-4, (chunk, k, rel(I[k], Iref[k]))
 
 
+@pytest.mark.parametrize("chunk", [2, 4, 8])
+def test_quad_leaf_groups_per_block(chunk):
+    """Quad leaf kernel with all of a leaf's groups of 64 direction quads in one
+    block (leaf_quad 3; the 12 x 24 leaf grid has two groups) against one group
+    per block (leaf_quad 2): bit-identical (each group runs the same operations
+    on the same staged values), and within 1e-4 of the FP64 oracle.  Leaves of
+    up to one staging chunk of points are staged once for both groups, larger
+    ones per group; 2 solves per pass (2 tiles) take the epilogue through the
+    LDS and restage per group, 4 and 8 (3 and 6 tiles) project in registers."""
+    s = wf.fibonacci_sphere(8000, 1.5e-6)
+    incs = [wf.Incidence(0.3 * i - 1.0, 0.7 * i, 0.45e-6, "ME"[i % 2], 1.55 + 0.05j)
+            for i in range(8)]
+    nt, np_ = 40, 80
+    out = {}
+    for mode in (2, 3):
+        print(f"case: quad leaf mode {mode}, {chunk} solves per pass, depth=3", flush=True)
+        with wf.POSolver(device=0) as sv:
+            sv.tune = {"leaf_quad": mode, "chunk_solves": chunk}
+            sv.set_geometry(s, 3)
+            out[mode] = sv.solve(incs, (nt, np_))
+    assert np.array_equal(out[3], out[2]), rel(out[3], out[2])
+    Iref, _ = oracle.solve_direct(s.arrays(), [t.as_tuple() for t in incs], nt, np_)
+    for k in range(len(incs)):
+        assert rel(out[3][k], Iref[k]) < 1e-4, (chunk, k, rel(out[3][k], Iref[k]))
+
+
 @pytest.mark.parametrize("version", [1, 2])
 @pytest.mark.parametrize("depth", [3, 4])
 def test_fused_transition_matches_two_pass(depth, version):

@@ -2345,7 +2345,7 @@ wf_status wf_set_tune(wf_handle* h, const wf_tune* t) {
                    "wf_tune: agg_kernel must be 0 or 1, agg_fused 0, 1 or 2, agg_fft_rows >= 0, "
                    "theta_tile 0..3, phi_rows 0, 16 or 32, 0 <= beta < 100");
   h->tune = *t;
-  h->tune.leaf_quad = t->leaf_quad ? 1 : 0;
+  h->tune.leaf_quad = (t->leaf_quad >= 0 && t->leaf_quad <= 3) ? t->leaf_quad : 1;
   return WF_OK;
 }
 

@@ -231,7 +231,10 @@ struct LeafArgs {
   int nt, np;
   float2* out;
   int precision;
-  int quad;  // 1: quad-symmetric BF16x3 kernel where eligible (np even)
+  // quad-symmetric BF16x3 kernel where eligible (np even): 1 (groups of 64
+  // direction quads per block chosen by the launcher), 2 (one group per
+  // block), 3 (all of a leaf's groups in one block); 0: plain kernels
+  int quad;
   // 1: F = 6 S fields (S solves' E x, y, z, H x, y, z) are summed and the
   // pattern receives the two transverse components of each solve instead,
   // out [leaf][2 S][nt][np] as launch_project_transverse forms them
