@@ -112,9 +112,10 @@ def agg_solver(request):
     (24, 48, 40, 80, None), (21, 35, 11, 22, "out"), (45, 80, 75, 144, None),
     (36, 60, 90, 180, "out"), (30, 50, 25, 40, None),
     # equal grids (the even length's Nyquist bin folds), a 240 -> 500 wave line,
-    # and theta lines above 1024 samples (the block-wide FFT kernels)
+    # theta lines above 1024 samples (the block-wide FFT kernels), and phi rows
+    # of exactly twice the child's length
     (24, 48, 24, 48, None), (24, 48, 24, 48, "out"), (120, 240, 250, 500, None),
-    (30, 60, 520, 1040, "out")])
+    (30, 60, 520, 1040, "out"), (20, 40, 36, 80, None), (60, 120, 100, 240, "out")])
 def test_interp_matches_bruteforce(agg_solver, ntc, npc, ntp, npp, off):
     solver = agg_solver
     rng = np.random.default_rng(ntc * 100 + npp)
