@@ -3,9 +3,12 @@
 // Restates util/tangentplane.h:18-80 (LocalTangentPlane), util/util.h:365-382
 // (cos_internal_angle, rs, rp with n0 = 1) and fuses util/fields.h:106-126
 // (field_times_area) and :128-150 (multiply_current_factor).  One thread per
-// (point, incidence); the math is FP64 (a few hundred flops per point, far
-// below the HBM time of the 48 B x S written), so the lit/shadow and
-// normal-incidence decisions are the ones the FP64 oracle takes.
+// point runs every incidence, so with the point-major layout its stores fill
+// whole cache lines (one thread per (point, incidence) wrote 48 B pieces of
+// lines shared with other blocks: 1.7x the time on the 663k-point fiber); the
+// math is FP64 (a few hundred flops per point and incidence), so the
+// lit/shadow and normal-incidence decisions are the ones the FP64 oracle
+// takes.
 #include "wf_common.h"
 #include "wf_currents.h"
 
@@ -30,16 +33,19 @@ __global__ __launch_bounds__(256) void currents_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     const float* __restrict__ pnx, const float* __restrict__ pny, const float* __restrict__ pnz,
     const float* __restrict__ parea, int64_t n, const WfIncDev* __restrict__ incs,
-    OutT* __restrict__ out, int64_t stride_n, int64_t stride_s) {
-  const int s = blockIdx.y;
-  const WfIncDev inc = incs[s];
+    OutT* __restrict__ out, int64_t stride_n, int64_t stride_s, int n_inc) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double p[3] = {px[i], py[i], pz[i]};
     const double nr[3] = {pnx[i], pny[i], pnz[i]};
-    double2 cur[6];
-    wfdev::current_point(p, nr, parea[i], inc, cur);
-    store6<OutT>(out, i * stride_n + (int64_t)s * stride_s, cur);
+    const double ar = parea[i];
+    // every incidence of the point from one thread: with the point-major
+    // layout (stride_s = 6) its stores fill whole cache lines
+    for (int s = 0; s < n_inc; ++s) {
+      double2 cur[6];
+      wfdev::current_point(p, nr, ar, incs[s], cur);
+      store6<OutT>(out, i * stride_n + (int64_t)s * stride_s, cur);
+    }
   }
 }
 
@@ -121,13 +127,13 @@ hipError_t launch_currents(const float* x, const float* y, const float* z, const
   if (n <= 0 || n_inc <= 0) return hipSuccess;
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65535) blocks = 65535;
-  dim3 grid((unsigned)blocks, (unsigned)n_inc);
+  dim3 grid((unsigned)blocks);
   if (fp64)
     hipLaunchKernelGGL(currents_kernel<double2>, grid, dim3(256), 0, st, x, y, z, nx, ny, nz,
-                       area, n, inc, (double2*)out, stride_n, stride_s);
+                       area, n, inc, (double2*)out, stride_n, stride_s, n_inc);
   else
     hipLaunchKernelGGL(currents_kernel<float2>, grid, dim3(256), 0, st, x, y, z, nx, ny, nz,
-                       area, n, inc, (float2*)out, stride_n, stride_s);
+                       area, n, inc, (float2*)out, stride_n, stride_s, n_inc);
   return hipGetLastError();
 }
 
