@@ -3,9 +3,11 @@
 // Restates util/tangentplane.h:18-80 (LocalTangentPlane), util/util.h:365-382
 // (cos_internal_angle, rs, rp with n0 = 1) and fuses util/fields.h:106-126
 // (field_times_area) and :128-150 (multiply_current_factor).  One thread per
-// point runs every incidence, so with the point-major layout its stores fill
-// whole cache lines (one thread per (point, incidence) wrote 48 B pieces of
-// lines shared with other blocks: 1.7x the time on the 663k-point fiber); the
+// point runs every incidence (incidences split over a few block rows only
+// when the points alone fill fewer than 1024 blocks), so with the
+// point-major layout its stores fill whole cache lines (one thread per
+// (point, incidence) wrote 48 B pieces of lines shared with other blocks:
+// 1.7x the time on the 663k-point fiber); the
 // math is FP64 (a few hundred flops per point and incidence), so the
 // lit/shadow and normal-incidence decisions are the ones the FP64 oracle
 // takes.
@@ -39,9 +41,10 @@ __global__ __launch_bounds__(256) void currents_kernel(
     const double p[3] = {px[i], py[i], pz[i]};
     const double nr[3] = {pnx[i], pny[i], pnz[i]};
     const double ar = parea[i];
-    // every incidence of the point from one thread: with the point-major
+    // the point's incidences blockIdx.y, + gridDim.y, ... from one thread
+    // (all of them unless the point grid is small): with the point-major
     // layout (stride_s = 6) its stores fill whole cache lines
-    for (int s = 0; s < n_inc; ++s) {
+    for (int s = blockIdx.y; s < n_inc; s += gridDim.y) {
       double2 cur[6];
       wfdev::current_point(p, nr, ar, incs[s], cur);
       store6<OutT>(out, i * stride_n + (int64_t)s * stride_s, cur);
@@ -127,7 +130,9 @@ hipError_t launch_currents(const float* x, const float* y, const float* z, const
   if (n <= 0 || n_inc <= 0) return hipSuccess;
   int64_t blocks = (n + 255) / 256;
   if (blocks > 65535) blocks = 65535;
-  dim3 grid((unsigned)blocks);
+  // incidence groups only where the points alone give fewer than 1024 blocks
+  const int64_t groups = std::min<int64_t>(n_inc, std::max<int64_t>(1, 1024 / blocks));
+  dim3 grid((unsigned)blocks, (unsigned)groups);
   if (fp64)
     hipLaunchKernelGGL(currents_kernel<double2>, grid, dim3(256), 0, st, x, y, z, nx, ny, nz,
                        area, n, inc, (double2*)out, stride_n, stride_s, n_inc);
