@@ -127,8 +127,10 @@ def cpu_baseline(surf, incs, grid, budget_s: float):
     arrays = surf.arrays()
 
     def run(nd):
-        step = max(1, len(dirs_all) // nd)
-        dirs = dirs_all[::step][:nd]
+        # every step-th direction over the whole grid (no truncation: the
+        # sample spans every row, the intensity peak included)
+        step = max(1, -(-len(dirs_all) // nd))
+        dirs = dirs_all[::step]
         t0 = time.perf_counter()
         I, _ = oracle.solve_direct(arrays, sample_incs, 0, 0, dirs=dirs, nthreads=nthreads)
         return dirs, step, time.perf_counter() - t0, I
